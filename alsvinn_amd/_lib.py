@@ -1,0 +1,121 @@
+"""ctypes binding of libalsvinn_hip.so (include/alsvinn_hip.h).
+
+The library must be built in-tree (`python -c "import __graft_entry__ as g; g.build()"`
+or `python alsvinn_amd/build.py`). There is no fallback: loading fails loudly
+when the library is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libalsvinn_hip.so")
+
+EQ_BURGERS, EQ_EULER1, EQ_EULER2, EQ_EULER3 = 0, 1, 2, 3
+FLUX_HLL, FLUX_HLL3, FLUX_GODUNOV = 0, 1, 2
+RECON_NONE, RECON_WENO2, RECON_ENO2 = 0, 1, 2
+INT_FE, INT_RK2, INT_RK3 = 0, 1, 2
+BC_NEUMANN, BC_PERIODIC = 0, 1
+FP64, FP32 = 0, 1
+
+E_OK, E_INVALID, E_HIP, E_CONSTRAINT = 0, 1, 2, 3
+
+# Every function include/alsvinn_hip.h declares.
+EXPORTS = [
+    "alsvinn_hip_last_error",
+    "alsvinn_hip_ghost_cells",
+    "alsvinn_hip_ensemble_create",
+    "alsvinn_hip_ensemble_destroy",
+    "alsvinn_hip_ensemble_layout",
+    "alsvinn_hip_ensemble_upload",
+    "alsvinn_hip_ensemble_download",
+    "alsvinn_hip_ensemble_state_pointer",
+    "alsvinn_hip_ensemble_apply_boundary",
+    "alsvinn_hip_ensemble_reset_time",
+    "alsvinn_hip_ensemble_step",
+    "alsvinn_hip_ensemble_run",
+    "alsvinn_hip_ensemble_synchronize",
+    "alsvinn_hip_ensemble_sample_info",
+    "alsvinn_hip_ensemble_record_dt",
+    "alsvinn_hip_ensemble_dt_history",
+    "alsvinn_hip_ensemble_rhs",
+    "alsvinn_hip_ensemble_last_elapsed_ms",
+]
+
+
+class Scheme(ctypes.Structure):
+    _fields_ = [
+        ("equation", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("reconstruction", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("boundary", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+    ]
+
+
+class Sample(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("num_adjusters", ctypes.c_int),
+        ("origin", ctypes.c_double * 3),
+        ("top", ctypes.c_double * 3),
+        ("end_time", ctypes.c_double),
+        ("interval", ctypes.c_double * 2),
+        ("saved0", ctypes.c_longlong * 2),
+        ("max_steps", ctypes.c_longlong),
+    ]
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("alsvinn_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Loads the library (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libalsvinn_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.alsvinn_hip_last_error.restype = ctypes.c_char_p
+    L.alsvinn_hip_last_error.argtypes = []
+    L.alsvinn_hip_ghost_cells.argtypes = [ctypes.POINTER(Scheme)]
+    L.alsvinn_hip_ensemble_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
+                                              ctypes.POINTER(Sample), vp, ctypes.POINTER(vp)]
+    L.alsvinn_hip_ensemble_destroy.argtypes = [vp]
+    L.alsvinn_hip_ensemble_layout.argtypes = [vp, ctypes.c_int, ip, llp, llp]
+    L.alsvinn_hip_ensemble_upload.argtypes = [vp, ctypes.c_int, dp]
+    L.alsvinn_hip_ensemble_download.argtypes = [vp, ctypes.c_int, dp]
+    L.alsvinn_hip_ensemble_state_pointer.argtypes = [vp, ctypes.POINTER(vp)]
+    L.alsvinn_hip_ensemble_apply_boundary.argtypes = [vp]
+    L.alsvinn_hip_ensemble_reset_time.argtypes = [vp]
+    L.alsvinn_hip_ensemble_step.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_ensemble_run.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_ensemble_synchronize.argtypes = [vp]
+    L.alsvinn_hip_ensemble_sample_info.argtypes = [vp, ctypes.c_int, dp, llp, dp, ip, ip]
+    L.alsvinn_hip_ensemble_record_dt.argtypes = [vp, ctypes.c_longlong]
+    L.alsvinn_hip_ensemble_dt_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_longlong]
+    L.alsvinn_hip_ensemble_rhs.argtypes = [vp, ctypes.c_int, dp, dp]
+    L.alsvinn_hip_ensemble_last_elapsed_ms.argtypes = [vp, dp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise HipError(rc, load().alsvinn_hip_last_error().decode())
+    return rc

@@ -1,0 +1,638 @@
+// Host side of the hip platform's C ABI: the ensemble object (a batch of
+// solves of one scheme), its buffers, the per-step launch sequence and the
+// captured HIP graph of the timestep loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/alsvinn_hip.h"
+#include "ensemble_kernels.hpp"
+#include "internal.hpp"
+
+using namespace alsvinn_hip;
+
+namespace alsvinn_hip {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// End of a timestep (Simulator::incrementSolution tail, Simulator::atEnd,
+// FixedIntervalWriter::write): t += dt, save counters, stop flag, wave-speed
+// reset. One thread per sample.
+__global__ void bookkeeping_kernel(const SampleDesc* __restrict__ desc, SampleState* state,
+                                   unsigned long long* ws, double* dt_hist, long long dt_cap,
+                                   int nsamples) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsamples) return;
+    const SampleDesc D = desc[s];
+    SampleState S = state[s];
+    if (!S.active) return;
+    const double dt = sample_dt(D, S, ws + 3 * s);
+    if (dt_hist && S.steps < dt_cap) dt_hist[(long long)s * dt_cap + S.steps] = dt;
+    S.t = S.t + dt;
+    S.steps += 1;
+    S.last_dt = dt;
+    for (int k = 0; k < D.nadj; ++k)
+        if (S.t >= double(S.saved[k]) * D.interval[k] + 0.0) S.saved[k] += 1;
+    ws[3 * s + 0] = 0;
+    ws[3 * s + 1] = 0;
+    ws[3 * s + 2] = 0;
+    if (S.status != 0 || S.t >= D.end_time || (D.max_steps >= 0 && S.steps >= D.max_steps))
+        S.active = 0;
+    state[s] = S;
+}
+
+__global__ void count_active_kernel(const SampleState* __restrict__ state, int nsamples,
+                                    int* count) {
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    int local = 0;
+    for (int s = threadIdx.x; s < nsamples; s += blockDim.x) local += state[s].active ? 1 : 0;
+    atomicAdd(&acc, local);
+    __syncthreads();
+    if (threadIdx.x == 0) *count = acc;
+}
+
+
+}  // namespace alsvinn_hip
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return fail(ALSVINN_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct alsvinn_hip_ensemble {
+    alsvinn_hip_scheme scheme;
+    int nsamples = 0;
+    int nvar = 0;
+    int dim = 0;
+    int ghosts = 0;
+    size_t esize = 8;
+    std::vector<SampleDesc> desc;
+    std::vector<alsvinn_hip_sample> spec;
+    long long total = 0;  // elements per buffer
+    void* U = nullptr;
+    void* A = nullptr;
+    void* B = nullptr;
+    SampleDesc* d_desc = nullptr;
+    SampleState* d_state = nullptr;
+    unsigned long long* d_ws = nullptr;
+    WaveTask* d_tasks = nullptr;
+    int ntasks = 0;
+    int* d_count = nullptr;
+    int* h_count = nullptr;
+    double* d_dt_hist = nullptr;
+    long long dt_cap = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    long long max_vstride = 0;
+    KernelSet<double> k64;
+    KernelSet<float> k32;
+};
+
+namespace {
+
+int validate(const alsvinn_hip_scheme* s) {
+    if (!s) return fail(ALSVINN_HIP_E_INVALID, "scheme is NULL");
+    if (s->equation < 0 || s->equation > 3)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown equation " + std::to_string(s->equation));
+    if (s->reconstruction < 0 || s->reconstruction > 2)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction " + std::to_string(s->reconstruction));
+    if (s->integrator < 0 || s->integrator > 2)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown integrator " + std::to_string(s->integrator));
+    if (s->boundary < 0 || s->boundary > 1)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown boundary " + std::to_string(s->boundary));
+    if (s->precision < 0 || s->precision > 1)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown precision " + std::to_string(s->precision));
+    if (s->equation == ALSVINN_HIP_EQ_BURGERS && s->flux != ALSVINN_HIP_FLUX_GODUNOV)
+        return fail(ALSVINN_HIP_E_INVALID, "burgers supports the godunov flux only");
+    if (s->equation != ALSVINN_HIP_EQ_BURGERS && s->flux != ALSVINN_HIP_FLUX_HLL &&
+        s->flux != ALSVINN_HIP_FLUX_HLL3)
+        return fail(ALSVINN_HIP_E_INVALID, "euler supports the hll and hll3 fluxes only");
+    return 0;
+}
+
+int ghost_cells(const alsvinn_hip_scheme& s) {
+    return s.reconstruction == ALSVINN_HIP_RECON_NONE ? 1 : 2;
+}
+
+template <class T>
+void* buf_at(void* base, long long off) {
+    return (void*)((T*)base + off);
+}
+
+// Band height (rows per wave task) for a sample of ny rows: whole columns
+// unless the ensemble would then launch fewer than ~4 waves per SIMD.
+int band_rows(long long tasks_per_band, int ny) {
+    const long long target_waves = 4096;
+    int ty = ny;
+    while (ty > 8 && tasks_per_band * ((ny + ty - 1) / ty) < target_waves) ty = (ty + 1) / 2;
+    return ty;
+}
+
+void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
+    const int G = e->ghosts;
+    const int W = 64 - 2 * G;
+    long long strips = 0;
+    for (int s = 0; s < e->nsamples; ++s) strips += (e->desc[s].nx + W - 1) / W;
+    for (int s = 0; s < e->nsamples; ++s) {
+        const SampleDesc& D = e->desc[s];
+        const int ty = e->dim >= 2 ? band_rows(strips, D.ny) : 1;
+        for (int y0 = 0; y0 < D.ny; y0 += ty) {
+            for (int x0 = 0; x0 < D.nx; x0 += W) {
+                WaveTask t;
+                t.sample = s;
+                t.x0 = x0;
+                t.ncols = std::min(W, D.nx - x0);
+                t.y0 = y0;
+                t.y1 = std::min(D.ny, y0 + ty);
+                t.z0 = 0;
+                t.z1 = std::max(1, D.nz);
+                t.pad = 0;
+                tasks.push_back(t);
+            }
+        }
+    }
+}
+
+template <class T>
+int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k) {
+    StageArgs<T> a;
+    a.desc = e->d_desc;
+    a.state = e->d_state;
+    a.ws = e->d_ws;
+    a.tasks = e->d_tasks;
+    a.ntasks = e->ntasks;
+    a.gamma = e->scheme.gamma;
+    const int blocks = (e->ntasks + 3) / 4;
+    T* U = (T*)e->U;
+    T* A = (T*)e->A;
+    T* B = (T*)e->B;
+    const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->max_vstride + 1023) / 1024));
+    const dim3 pgrid(bx, e->nsamples);
+
+    // stage 1: A = L(U), wave speeds
+    a.mode = MODE_RHS;
+    a.lazy = 0;
+    a.check = 0;
+    a.in = U;
+    a.in2 = U;
+    a.out = A;
+    k.stage(a, blocks, e->stream);
+    if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_FORWARD_EULER) {
+        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, A, U, U, 1, e->scheme.gamma);
+    } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK2) {
+        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, A, U, A, 0, e->scheme.gamma);
+        a.mode = MODE_RK2_1;
+        a.in = A;
+        a.in2 = U;
+        a.out = U;
+        a.check = 1;
+        k.stage(a, blocks, e->stream);
+    } else {
+        a.mode = MODE_RK3_1;
+        a.lazy = 1;
+        a.in = A;
+        a.in2 = U;
+        a.out = B;
+        k.stage(a, blocks, e->stream);
+        a.mode = MODE_RK3_2;
+        a.lazy = 0;
+        a.in = B;
+        a.in2 = U;
+        a.out = U;
+        a.check = 1;
+        k.stage(a, blocks, e->stream);
+    }
+    hipLaunchKernelGGL(bookkeeping_kernel, dim3((e->nsamples + 255) / 256), dim3(256), 0,
+                       e->stream, e->d_desc, e->d_state, e->d_ws, e->d_dt_hist, e->dt_cap,
+                       e->nsamples);
+    return 0;
+}
+
+int enqueue_steps(alsvinn_hip_ensemble* e, int nsteps) {
+    for (int i = 0; i < nsteps; ++i) {
+        if (e->esize == 8)
+            enqueue_step<double>(e, e->k64);
+        else
+            enqueue_step<float>(e, e->k32);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int init_state(alsvinn_hip_ensemble* e) {
+    std::vector<SampleState> st(e->nsamples);
+    for (int s = 0; s < e->nsamples; ++s) {
+        SampleState& S = st[s];
+        std::memset(&S, 0, sizeof(S));
+        S.t = 0;
+        S.steps = 0;
+        S.saved[0] = e->spec[s].saved0[0];
+        S.saved[1] = e->spec[s].saved0[1];
+        S.active = (0 < e->spec[s].end_time && e->spec[s].max_steps != 0) ? 1 : 0;
+        S.status = 0;
+    }
+    HIP_TRY(hipMemcpy(e->d_state, st.data(), sizeof(SampleState) * st.size(), hipMemcpyHostToDevice));
+    std::vector<unsigned long long> z(3 * (size_t)e->nsamples, 0ull);
+    HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void destroy_graph(alsvinn_hip_ensemble* e) {
+    if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    e->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* alsvinn_hip_last_error(void) { return g_last_error.c_str(); }
+
+int alsvinn_hip_ghost_cells(const alsvinn_hip_scheme* scheme) {
+    if (validate(scheme)) return -1;
+    return ghost_cells(*scheme);
+}
+
+int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
+                                const alsvinn_hip_sample* samples, void* stream,
+                                alsvinn_hip_ensemble** out) {
+    if (int rc = validate(scheme)) return rc;
+    if (!out || !samples || nsamples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
+    *out = nullptr;
+    auto* e = new alsvinn_hip_ensemble();
+    e->scheme = *scheme;
+    e->nsamples = nsamples;
+    e->nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
+    e->ghosts = ghost_cells(*scheme);
+    e->esize = scheme->precision == ALSVINN_HIP_FP64 ? 8 : 4;
+    const int G = e->ghosts;
+    long long off = 0;
+    for (int s = 0; s < nsamples; ++s) {
+        const alsvinn_hip_sample& p = samples[s];
+        const int dim = 1 + (p.ny > 1) + (p.nz > 1);
+        if (s == 0) e->dim = dim;
+        if (dim != e->dim || p.nx < 1 || p.ny < 1 || p.nz < 1) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "all samples of an ensemble must have the same dimension");
+        }
+        if (p.nx < G || (p.ny > 1 && p.ny < G) || (p.nz > 1 && p.nz < G)) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "grid smaller than the number of ghost cells");
+        }
+        if (p.num_adjusters < 0 || p.num_adjusters > 2) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "num_adjusters must be 0..2");
+        }
+        SampleDesc D;
+        std::memset(&D, 0, sizeof(D));
+        D.nx = p.nx;
+        D.ny = p.ny;
+        D.nz = p.nz;
+        D.gx = G;
+        D.gy = p.ny > 1 ? G : 0;
+        D.gz = p.nz > 1 ? G : 0;
+        D.tx = p.nx + 2 * D.gx;
+        D.ty = p.ny + 2 * D.gy;
+        D.tz = p.nz + 2 * D.gz;
+        D.vstride = (long long)D.tx * D.ty * D.tz;
+        D.base = off;
+        D.bc = scheme->boundary;
+        const int n[3] = {p.nx, p.ny, p.nz};
+        for (int k = 0; k < 3; ++k) D.dx[k] = (p.top[k] - p.origin[k]) / double(n[k]);
+        if (p.ny == 1) D.dx[1] = 1;
+        if (p.nz == 1) D.dx[2] = 1;
+        D.cfl = scheme->cfl;
+        D.end_time = p.end_time;
+        D.nadj = p.num_adjusters;
+        D.interval[0] = p.interval[0];
+        D.interval[1] = p.interval[1];
+        D.max_steps = p.max_steps;
+        off += D.vstride * e->nvar;
+        e->max_vstride = std::max(e->max_vstride, D.vstride);
+        e->desc.push_back(D);
+        e->spec.push_back(p);
+    }
+    e->total = off;
+    int rc = 0;
+    if (e->esize == 8)
+        rc = select_kernels<double>(*scheme, e->dim, e->k64);
+    else
+        rc = select_kernels<float>(*scheme, e->dim, e->k32);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    auto cleanup = [&](int code) {
+        alsvinn_hip_ensemble_destroy(e);
+        return code;
+    };
+#define HIP_TRY_C(expr)                                                                       \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return cleanup(fail(ALSVINN_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+    if (stream) {
+        e->stream = (hipStream_t)stream;
+    } else {
+        HIP_TRY_C(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        e->own_stream = true;
+    }
+    const size_t bytes = (size_t)e->total * e->esize;
+    HIP_TRY_C(hipMalloc(&e->U, bytes));
+    HIP_TRY_C(hipMalloc(&e->A, bytes));
+    if (scheme->integrator == ALSVINN_HIP_INTEGRATOR_RK3) HIP_TRY_C(hipMalloc(&e->B, bytes));
+    HIP_TRY_C(hipMemset(e->U, 0, bytes));
+    HIP_TRY_C(hipMemset(e->A, 0, bytes));
+    if (e->B) HIP_TRY_C(hipMemset(e->B, 0, bytes));
+    HIP_TRY_C(hipMalloc(&e->d_desc, sizeof(SampleDesc) * nsamples));
+    HIP_TRY_C(hipMalloc(&e->d_state, sizeof(SampleState) * nsamples));
+    HIP_TRY_C(hipMalloc(&e->d_ws, sizeof(unsigned long long) * 3 * nsamples));
+    HIP_TRY_C(hipMalloc(&e->d_count, sizeof(int)));
+    HIP_TRY_C(hipHostMalloc(&e->h_count, sizeof(int)));
+    HIP_TRY_C(hipMemcpy(e->d_desc, e->desc.data(), sizeof(SampleDesc) * nsamples, hipMemcpyHostToDevice));
+    std::vector<WaveTask> tasks;
+    build_tasks(e, tasks);
+    e->ntasks = (int)tasks.size();
+    HIP_TRY_C(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
+    HIP_TRY_C(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
+    HIP_TRY_C(hipEventCreate(&e->ev0));
+    HIP_TRY_C(hipEventCreate(&e->ev1));
+#undef HIP_TRY_C
+    if (int r = init_state(e)) return cleanup(r);
+    *out = e;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
+    if (!e) return 0;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    destroy_graph(e);
+    hipFree(e->U);
+    hipFree(e->A);
+    hipFree(e->B);
+    hipFree(e->d_desc);
+    hipFree(e->d_state);
+    hipFree(e->d_ws);
+    hipFree(e->d_tasks);
+    hipFree(e->d_count);
+    hipFree(e->d_dt_hist);
+    if (e->h_count) hipHostFree(e->h_count);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_layout(const alsvinn_hip_ensemble* e, int sample, int dims6[6],
+                                long long* offset, long long* elements_per_variable) {
+    if (!e || sample < 0 || sample >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad sample index");
+    const SampleDesc& D = e->desc[sample];
+    if (dims6) {
+        dims6[0] = D.tx;
+        dims6[1] = D.ty;
+        dims6[2] = D.tz;
+        dims6[3] = D.gx;
+        dims6[4] = D.gy;
+        dims6[5] = D.gz;
+    }
+    if (offset) *offset = D.base;
+    if (elements_per_variable) *elements_per_variable = D.vstride;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_upload(alsvinn_hip_ensemble* e, int sample, const double* host) {
+    if (!e || !host || sample < 0 || sample >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad upload arguments");
+    const SampleDesc& D = e->desc[sample];
+    const size_t n = (size_t)D.vstride * e->nvar;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->esize == 8) {
+        HIP_TRY(hipMemcpy(buf_at<double>(e->U, D.base), host, n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+        HIP_TRY(hipMemcpy(buf_at<float>(e->U, D.base), tmp.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int alsvinn_hip_ensemble_download(alsvinn_hip_ensemble* e, int sample, double* host) {
+    if (!e || !host || sample < 0 || sample >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad download arguments");
+    const SampleDesc& D = e->desc[sample];
+    const size_t n = (size_t)D.vstride * e->nvar;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->esize == 8) {
+        HIP_TRY(hipMemcpy(host, buf_at<double>(e->U, D.base), n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIP_TRY(hipMemcpy(tmp.data(), buf_at<float>(e->U, D.base), n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+    }
+    return 0;
+}
+
+int alsvinn_hip_ensemble_state_pointer(alsvinn_hip_ensemble* e, void** ptr) {
+    if (!e || !ptr) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    *ptr = e->U;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_apply_boundary(alsvinn_hip_ensemble* e) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->max_vstride + 1023) / 1024));
+    if (e->esize == 8)
+        hipLaunchKernelGGL(ghost_fill_kernel<double>, dim3(bx, e->nsamples), dim3(256), 0, e->stream,
+                           e->d_desc, (double*)e->U, e->nvar);
+    else
+        hipLaunchKernelGGL(ghost_fill_kernel<float>, dim3(bx, e->nsamples), dim3(256), 0, e->stream,
+                           e->d_desc, (float*)e->U, e->nvar);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_ensemble_reset_time(alsvinn_hip_ensemble* e) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return init_state(e);
+}
+
+int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps) {
+    if (!e || nsteps < 0) return fail(ALSVINN_HIP_E_INVALID, "bad step arguments");
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    if (int rc = enqueue_steps(e, nsteps)) return rc;
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_run(alsvinn_hip_ensemble* e, int steps_per_sync) {
+    if (!e || steps_per_sync < 1) return fail(ALSVINN_HIP_E_INVALID, "bad run arguments");
+    if (!e->graph_exec || e->graph_steps != steps_per_sync) {
+        destroy_graph(e);
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_steps(e, steps_per_sync);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        e->graph = g;
+        HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+        e->graph_steps = steps_per_sync;
+    }
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    for (;;) {
+        HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+        hipLaunchKernelGGL(count_active_kernel, dim3(1), dim3(256), 0, e->stream, e->d_state,
+                           e->nsamples, e->d_count);
+        HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (*e->h_count == 0) break;
+    }
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<SampleState> st(e->nsamples);
+    HIP_TRY(hipMemcpy(st.data(), e->d_state, sizeof(SampleState) * st.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->nsamples; ++s)
+        if (st[s].status != 0)
+            return fail(ALSVINN_HIP_E_CONSTRAINT,
+                        "Simulation state does not obey constraints! sample " + std::to_string(s) +
+                            " at time " + std::to_string(st[s].t) + ", number of timesteps performed " +
+                            std::to_string(st[s].steps));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_synchronize(alsvinn_hip_ensemble* e) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_sample_info(alsvinn_hip_ensemble* e, int sample, double* t,
+                                     long long* steps, double* last_dt, int* status, int* active) {
+    if (!e || sample < 0 || sample >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad sample index");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    SampleState S;
+    HIP_TRY(hipMemcpy(&S, e->d_state + sample, sizeof(S), hipMemcpyDeviceToHost));
+    if (t) *t = S.t;
+    if (steps) *steps = S.steps;
+    if (last_dt) *last_dt = S.last_dt;
+    if (status) *status = S.status;
+    if (active) *active = S.active;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_record_dt(alsvinn_hip_ensemble* e, long long capacity) {
+    if (!e || capacity < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    hipFree(e->d_dt_hist);
+    e->d_dt_hist = nullptr;
+    e->dt_cap = 0;
+    if (capacity > 0) {
+        HIP_TRY(hipMalloc(&e->d_dt_hist, sizeof(double) * capacity * e->nsamples));
+        HIP_TRY(hipMemset(e->d_dt_hist, 0, sizeof(double) * capacity * e->nsamples));
+        e->dt_cap = capacity;
+    }
+    return 0;
+}
+
+int alsvinn_hip_ensemble_dt_history(alsvinn_hip_ensemble* e, int sample, double* host,
+                                    long long capacity) {
+    if (!e || sample < 0 || sample >= e->nsamples || !host) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    if (!e->d_dt_hist) return fail(ALSVINN_HIP_E_INVALID, "dt recording is off");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const long long n = std::min(capacity, e->dt_cap);
+    HIP_TRY(hipMemcpy(host, e->d_dt_hist + (long long)sample * e->dt_cap, sizeof(double) * n,
+                      hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_out,
+                             double wave_speeds[3]) {
+    if (!e || sample < 0 || sample >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad sample index");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<unsigned long long> z(3 * (size_t)e->nsamples, 0ull);
+    HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+    const SampleDesc& D = e->desc[sample];
+    const size_t n = (size_t)D.vstride * e->nvar;
+    const int blocks = (e->ntasks + 3) / 4;
+    if (e->esize == 8) {
+        StageArgs<double> a;
+        std::memset(&a, 0, sizeof(a));
+        a.desc = e->d_desc;
+        a.state = e->d_state;
+        a.ws = e->d_ws;
+        a.tasks = e->d_tasks;
+        a.ntasks = e->ntasks;
+        a.gamma = e->scheme.gamma;
+        a.mode = MODE_RHS;
+        a.in = (double*)e->U;
+        a.in2 = (double*)e->U;
+        a.out = (double*)e->A;
+        e->k64.stage(a, blocks, e->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (host_out) HIP_TRY(hipMemcpy(host_out, buf_at<double>(e->A, D.base), n * 8, hipMemcpyDeviceToHost));
+    } else {
+        StageArgs<float> a;
+        std::memset(&a, 0, sizeof(a));
+        a.desc = e->d_desc;
+        a.state = e->d_state;
+        a.ws = e->d_ws;
+        a.tasks = e->d_tasks;
+        a.ntasks = e->ntasks;
+        a.gamma = e->scheme.gamma;
+        a.mode = MODE_RHS;
+        a.in = (float*)e->U;
+        a.in2 = (float*)e->U;
+        a.out = (float*)e->A;
+        e->k32.stage(a, blocks, e->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (host_out) {
+            std::vector<float> tmp(n);
+            HIP_TRY(hipMemcpy(tmp.data(), buf_at<float>(e->A, D.base), n * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) host_out[i] = tmp[i];
+        }
+    }
+    unsigned long long w[3];
+    HIP_TRY(hipMemcpy(w, e->d_ws + 3 * sample, sizeof(w), hipMemcpyDeviceToHost));
+    if (wave_speeds)
+        for (int k = 0; k < 3; ++k) std::memcpy(&wave_speeds[k], &w[k], 8);
+    HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_last_elapsed_ms(alsvinn_hip_ensemble* e, double* ms) {
+    if (!e || !ms) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    HIP_TRY(hipEventSynchronize(e->ev1));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, e->ev0, e->ev1));
+    *ms = f;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Internal declarations shared by the hip platform's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/alsvinn_hip.h"
+#include "ensemble_kernels.hpp"
+
+namespace alsvinn_hip {
+
+int fail(int code, const std::string& msg);
+
+// Launchers of the kernels one scheme (equation, reconstruction, flux, grid
+// dimension, precision) needs.
+template <class T>
+struct KernelSet {
+    void (*stage)(const StageArgs<T>& a, int blocks, hipStream_t s) = nullptr;
+    void (*axpy)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
+                 const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
+                 double gamma) = nullptr;
+};
+
+template <class T, class Eq, class Recon, class Flux, int DIM>
+void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <class T, class Eq>
+void launch_axpy(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
+                 const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
+                 double gamma) {
+    hipLaunchKernelGGL((axpy_kernel<T, Eq>), grid, dim3(256), 0, s, desc, state, ws, X, Y, out,
+                       check, gamma);
+}
+
+template <class T, class Eq, class Flux, int DIM>
+int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
+    k.axpy = &launch_axpy<T, Eq>;
+    switch (sc.reconstruction) {
+        case ALSVINN_HIP_RECON_NONE: k.stage = &launch_stage<T, Eq, ReconNone, Flux, DIM>; return 0;
+        case ALSVINN_HIP_RECON_WENO2: k.stage = &launch_stage<T, Eq, ReconWENO2, Flux, DIM>; return 0;
+        case ALSVINN_HIP_RECON_ENO2: k.stage = &launch_stage<T, Eq, ReconENO2, Flux, DIM>; return 0;
+    }
+    return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction");
+}
+
+// Defined per equation family in kernels_*.hip.
+template <class T>
+int select_burgers(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
+template <class T>
+int select_euler1(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
+template <class T>
+int select_euler2(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
+
+template <class T>
+int select_kernels(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
+    switch (sc.equation) {
+        case ALSVINN_HIP_EQ_BURGERS: return select_burgers<T>(sc, dim, k);
+        case ALSVINN_HIP_EQ_EULER1: return select_euler1<T>(sc, dim, k);
+        case ALSVINN_HIP_EQ_EULER2: return select_euler2<T>(sc, dim, k);
+    }
+    return fail(ALSVINN_HIP_E_INVALID, "equation not available on the hip platform: " +
+                                            std::to_string(sc.equation));
+}
+
+}  // namespace alsvinn_hip

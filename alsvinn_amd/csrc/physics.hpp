@@ -1,0 +1,325 @@
+// Device-side physics for the hip platform: Euler<nsd> / Burgers equations,
+// HLL / HLL3 / Godunov face fluxes and none / WENO2 / ENO2 reconstruction.
+//
+// Every expression keeps the association order of the reference's CPU path so
+// that FP64 results are bit-identical (the library is built with
+// -ffp-contract=off; '/' and sqrt are IEEE correctly rounded on gfx950):
+//   Euler::computeExtra / computePointFlux   alsfvm/include/alsfvm/equation/euler/Euler.hpp:172-211
+//   HLL::computeFlux / computeHLLSpeeds      alsfvm/include/alsfvm/numflux/euler/HLL.hpp:51-115
+//   HLL3::computeFlux / computeHLLSpeeds     alsfvm/include/alsfvm/numflux/euler/HLL3.hpp:52-148
+//   burgers::Godunov::computeFlux            alsfvm/include/alsfvm/numflux/burgers/Godunov.hpp:40-60
+//   WENO2::reconstruct                       alsfvm/include/alsfvm/reconstruction/WENO2.hpp:37-71
+//   ENOCPU<2> shift + coefficients           alsfvm/src/reconstruction/ENOCPU.cpp:143-176,
+//                                            alsfvm/src/reconstruction/ENOCoefficients.cpp:27-32
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace alsvinn_hip {
+
+// glibc's fmin/fmax on x86-64 (minsd/maxsd with NaN fall-through): for
+// ordered operands min returns x only if x < y, max only if x > y.
+template <class T>
+__device__ __forceinline__ T ref_fmin(T x, T y) {
+    if (x != x) return y;
+    if (y != y) return x;
+    return (x < y) ? x : y;
+}
+template <class T>
+__device__ __forceinline__ T ref_fmax(T x, T y) {
+    if (x != x) return y;
+    if (y != y) return x;
+    return (x > y) ? x : y;
+}
+// std::max(a, b) = (a < b) ? b : a (keeps a when b is NaN).
+template <class T>
+__device__ __forceinline__ T std_max(T a, T b) {
+    return (a < b) ? b : a;
+}
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+
+// ------------------------------------------------------------------ equations
+
+template <class T, int NSD>
+struct EulerEq {
+    static constexpr int nvar = NSD + 2;
+    static constexpr int nsd = NSD;
+    static constexpr bool is_euler = true;
+    T gamma;
+    struct All {
+        T q[nvar];
+        T u[NSD];
+        T p;
+    };
+    __device__ __forceinline__ void make(const T* q, All& a) const {
+#pragma unroll
+        for (int i = 0; i < nvar; ++i) a.q[i] = q[i];
+        T dot = q[1] * q[1];
+        if (NSD > 1) dot = dot + q[2] * q[2];
+        if (NSD > 2) dot = dot + q[3] * q[3];
+        const T ie = q[nvar - 1] - T(0.5) * dot / q[0];
+#pragma unroll
+        for (int k = 0; k < NSD; ++k) a.u[k] = q[1 + k] / q[0];
+        a.p = (gamma - T(1)) * ie;
+    }
+    template <int d>
+    __device__ __forceinline__ void pointFlux(const All& a, T* F) const {
+        F[0] = a.q[1 + d];
+#pragma unroll
+        for (int k = 0; k < NSD; ++k) F[1 + k] = a.q[1 + k] * a.u[d];
+        F[1 + d] += a.p;
+        F[nvar - 1] = (a.q[nvar - 1] + a.p) * a.u[d];
+    }
+    // CPUCellComputer::obeysConstraints + Euler::obeysConstraints:
+    // every conserved and extra value finite, rho > 0, p > 0.
+    __device__ __forceinline__ bool obeys(const T* q) const {
+        All a;
+        make(q, a);
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < nvar; ++i) ok = ok && __builtin_isfinite(q[i]);
+#pragma unroll
+        for (int k = 0; k < NSD; ++k) ok = ok && __builtin_isfinite(a.u[k]);
+        ok = ok && __builtin_isfinite(a.p);
+        return ok && (q[0] > T(0)) && (a.p > T(0));
+    }
+};
+
+template <class T>
+struct BurgersEq {
+    static constexpr int nvar = 1;
+    static constexpr int nsd = 3;
+    static constexpr bool is_euler = false;
+    T gamma;
+    struct All {
+        T q[1];
+    };
+    __device__ __forceinline__ void make(const T* q, All& a) const { a.q[0] = q[0]; }
+    template <int d>
+    __device__ __forceinline__ void pointFlux(const All& a, T* F) const {
+        F[0] = a.q[0] * a.q[0] / T(2);
+    }
+    __device__ __forceinline__ bool obeys(const T* q) const { return __builtin_isfinite(q[0]); }
+};
+
+// ------------------------------------------------------------------ fluxes
+
+struct FluxHLL {
+    template <int d, class T, int NSD>
+    __device__ __forceinline__ static T compute(const EulerEq<T, NSD>& eq,
+                                                const typename EulerEq<T, NSD>::All& L,
+                                                const typename EulerEq<T, NSD>::All& R, T* F) {
+        constexpr int nv = NSD + 2;
+        const T g = eq.gamma;
+        const T wL = dsqrt(L.q[0]);
+        const T wR = dsqrt(R.q[0]);
+        const T rho = (L.q[0] + R.q[0]) / T(2);
+        const T wsum = wL + wR;
+        const T ud = (L.u[d] * wL + R.u[d] * wR) / wsum;
+        const T p = (L.p * wL + R.p * wR) / wsum;
+        const T cs = dsqrt(g * p / rho);
+        const T sL = ref_fmin(L.u[d] - dsqrt(g * L.p / L.q[0]), ud - cs);
+        const T sR = ref_fmax(R.u[d] + dsqrt(g * R.p / R.q[0]), ud + cs);
+        if (sL == T(0)) {
+#pragma unroll
+            for (int i = 0; i < nv; ++i) F[i] = T(0);
+        } else if (sL > T(0)) {
+            eq.template pointFlux<d>(L, F);
+        } else if (sR < T(0)) {
+            eq.template pointFlux<d>(R, F);
+        } else {
+            T fL[nv], fR[nv];
+            eq.template pointFlux<d>(L, fL);
+            eq.template pointFlux<d>(R, fR);
+            const T srl = sR * sL;
+            const T den = sR - sL;
+#pragma unroll
+            for (int i = 0; i < nv; ++i)
+                F[i] = (fL[i] * sR - fR[i] * sL + (R.q[i] - L.q[i]) * srl) / den;
+        }
+        return ref_fmax(dabs(sL), dabs(sR));
+    }
+};
+
+struct FluxHLL3 {
+    template <int d, class T, int NSD>
+    __device__ __forceinline__ static T compute(const EulerEq<T, NSD>& eq,
+                                                const typename EulerEq<T, NSD>::All& L,
+                                                const typename EulerEq<T, NSD>::All& R, T* F) {
+        constexpr int nv = NSD + 2;
+        const T g = eq.gamma;
+        const T wL = dsqrt(L.q[0]);
+        const T wR = dsqrt(R.q[0]);
+        const T rho = (L.q[0] + R.q[0]) / T(2);
+        const T wsum = wL + wR;
+        const T ud = (L.u[d] * wL + R.u[d] * wR) / wsum;
+        const T p = (wL * L.p + wR * R.p) / wsum;
+        const T cfL = dsqrt(g * L.p / L.q[0]);
+        const T cfR = dsqrt(g * R.p / R.q[0]);
+        const T corr = T(0.5) * ref_fmax(T(0), L.u[d] - R.u[d]);
+        const T cs = dsqrt(g * p / rho);
+        const T sL = ref_fmin(L.u[d] + corr - cfL, ud - cs);
+        const T sR = ref_fmax(R.u[d] - corr + cfR, ud + cs);
+        if (sL == T(0)) {
+#pragma unroll
+            for (int i = 0; i < nv; ++i) F[i] = T(0);
+        } else if (sL > T(0)) {
+            eq.template pointFlux<d>(L, F);
+        } else if (sR < T(0)) {
+            eq.template pointFlux<d>(R, F);
+        } else {
+            // The star state: only the branch taken is evaluated; the values
+            // are the ones the reference computes before branching.
+            T fL[nv], fR[nv];
+            eq.template pointFlux<d>(L, fL);
+            eq.template pointFlux<d>(R, fR);
+            const T udl = L.u[d] - sL;
+            const T udr = R.u[d] - sR;
+            const T aa = udr * R.q[0] - udl * L.q[0];
+            const T sm = (R.q[1 + d] * udr - L.q[1 + d] * udl + R.p - L.p) / aa;
+            T us[NSD];
+#pragma unroll
+            for (int k = 0; k < NSD; ++k) {
+                if (k == d)
+                    us[k] = sm;
+                else
+                    us[k] = (fR[1 + k] - fL[1 + k] - R.q[1 + k] * sR + L.q[1 + k] * sL) / aa;
+            }
+            // The reference evaluates the two branches with the same formula
+            // on mirrored operands; select the operands, then evaluate once.
+            const bool left = sm >= T(0);
+            const T sX = left ? sL : sR;
+            const T udx = left ? udl : udr;
+            const T rhoX = left ? L.q[0] : R.q[0];
+            const T pX = left ? L.p : R.p;
+            const T uX = left ? L.u[d] : R.u[d];
+            const T EX = left ? L.q[nv - 1] : R.q[nv - 1];
+            T mid[nv];
+            const T smx = sm - sX;
+            mid[0] = rhoX * udx / smx;
+#pragma unroll
+            for (int k = 0; k < NSD; ++k) mid[1 + k] = us[k] * mid[0];
+            const T ps = pX + rhoX * (uX - sm) * udx;
+            mid[nv - 1] = (udx * EX + pX * uX - ps * sm) / smx;
+#pragma unroll
+            for (int i = 0; i < nv; ++i) {
+                const T fX = left ? fL[i] : fR[i];
+                const T qX = left ? L.q[i] : R.q[i];
+                F[i] = fX + (mid[i] - qX) * sX;
+            }
+        }
+        return ref_fmax(dabs(sL), dabs(sR));
+    }
+};
+
+struct FluxGodunov {
+    template <int d, class T>
+    __device__ __forceinline__ static T compute(const BurgersEq<T>& eq,
+                                                const typename BurgersEq<T>::All& L,
+                                                const typename BurgersEq<T>::All& R, T* F) {
+        typename BurgersEq<T>::All a, b;
+        a.q[0] = ref_fmax(L.q[0], T(0));
+        b.q[0] = ref_fmin(R.q[0], T(0));
+        T fl[1], fr[1];
+        eq.template pointFlux<d>(a, fl);
+        eq.template pointFlux<d>(b, fr);
+        F[0] = ref_fmax(fl[0], fr[0]);
+        return ref_fmax(dabs(L.q[0]), dabs(R.q[0]));
+    }
+};
+
+// ------------------------------------------------------------------ reconstruction
+// Each reconstruction maps the three cells (i-1, i, i+1) along one direction
+// to the left (i-1/2) and right (i+1/2) face values of cell i.
+
+struct ReconNone {
+    static constexpr int ghosts = 1;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            L[v] = q0[v];
+            R[v] = q0[v];
+        }
+    }
+};
+
+struct ReconWENO2 {
+    static constexpr int ghosts = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R) {
+        // The reference adds the double literal 1e-8 (WENOCoefficients.hpp:21),
+        // so the weights are formed in double even for a float state.
+        typedef double W;
+        const W eps = 1e-8;
+        const T i0 = q0[0];
+        const T d0 = qp[0] - i0;
+        const T d1 = i0 - qm[0];
+        const T b0 = d0 * d0;
+        const T b1 = d1 * d1;
+        const W e0 = eps + W(b0);
+        const W e1 = eps + W(b1);
+        const W D0 = 3 * e0 * e0;
+        const W D1 = 3 * e1 * e1;
+        // The weights are rounded to the state type as in the reference
+        // (a0Left etc. are `real`). 2/D == 2*(1/D) exactly while 1/D is a
+        // normal number (D < 2^1021, i.e. |w_{i+1}-w_i| < 2^510), so two of
+        // the four divisions are shared.
+        const T a0L = T(1 / D0);
+        const T a1R = T(1 / D1);
+        const T a1L = T(2) * a1R;
+        const T a0R = T(2) * a0L;
+        const T w0L = a0L / (a0L + a1L);
+        const T w1L = a1L / (a0L + a1L);
+        const T w0R = a0R / (a0R + a1R);
+        const T w1R = a1R / (a0R + a1R);
+        const T cL = T(3) * w0L + w1L;
+        const T cR = T(3) * w1R + w0R;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            L[v] = T(0.5 * W(w1L * qm[v] + cL * q0[v] - w0L * qp[v]));
+            R[v] = T(0.5 * W(w0R * qp[v] + cR * q0[v] - w1R * qm[v]));
+        }
+    }
+};
+
+struct ReconENO2 {
+    static constexpr int ghosts = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R) {
+        // Divided differences of variable 0: dd_{i-1} = q_i - q_{i-1},
+        // dd_i = q_{i+1} - q_i; shift left when |dd_{i-1}| < |dd_i|.
+        const T ddl = q0[0] - qm[0];
+        const T ddr = qp[0] - q0[0];
+        const bool shift = dabs(ddl) < dabs(ddr);
+        // Coefficient rows {3/2,-1/2}, {1/2,1/2}, {-1/2,3/2}: with shift the
+        // stencil is (i-1, i) with rows 1 (left) and 2 (right); else (i, i+1)
+        // with rows 0 and 1.
+        const T cl0 = shift ? T(0.5) : T(1.5);
+        const T cl1 = shift ? T(0.5) : T(-0.5);
+        const T cr0 = shift ? T(-0.5) : T(0.5);
+        const T cr1 = shift ? T(1.5) : T(0.5);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const T a = shift ? qm[v] : q0[v];
+            const T b = shift ? q0[v] : qp[v];
+            T l = T(0);
+            T r = T(0);
+            l += cl0 * a;
+            r += cr0 * a;
+            l += cl1 * b;
+            r += cr1 * b;
+            L[v] = l;
+            R[v] = r;
+        }
+    }
+};
+
+}  // namespace alsvinn_hip

@@ -1,0 +1,146 @@
+"""Python host interface of the hip platform (thin layer over the C ABI).
+
+`Ensemble` is a batch of solves of one scheme advanced together on one GPU:
+the on-device analogue of the reference's per-sample Simulator loop
+(alsuq/src/run/Runner.cpp:42-78, alsfvm/src/simulator/Simulator.cpp:77-85).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_scheme(equation, flux, reconstruction, integrator, boundary, gamma=1.4, cfl=None,
+                precision=_lib.FP64):
+    s = _lib.Scheme()
+    s.equation, s.flux, s.reconstruction = equation, flux, reconstruction
+    s.integrator, s.boundary, s.precision = integrator, boundary, precision
+    s.gamma = gamma
+    if cfl is None:
+        # SimulatorSetup.cpp:306-320: cfl "auto" is 0.9 without reconstruction, else 0.475
+        cfl = 0.9 if reconstruction == _lib.RECON_NONE else 0.475
+    s.cfl = cfl
+    return s
+
+
+def make_sample(n, origin, top, end_time, saves=(1,), max_steps=-1):
+    """One solve: grid n=(nx,ny,nz) on [origin, top], FixedInterval adjusters
+    with `saves` saves each (interval end_time/saves, first save at 1*interval)."""
+    p = _lib.Sample()
+    p.nx, p.ny, p.nz = n
+    for k in range(3):
+        p.origin[k] = origin[k]
+        p.top[k] = top[k]
+    p.end_time = end_time
+    p.num_adjusters = len(saves)
+    for k, ns in enumerate(saves):
+        p.interval[k] = end_time / ns
+        p.saved0[k] = 1
+    p.max_steps = max_steps
+    return p
+
+
+class Ensemble:
+    def __init__(self, scheme, samples, stream=None):
+        L = _lib.load()
+        self.scheme = scheme
+        self.samples = list(samples)
+        arr = (_lib.Sample * len(self.samples))(*self.samples)
+        h = ctypes.c_void_p()
+        check(L.alsvinn_hip_ensemble_create(ctypes.byref(scheme), len(self.samples), arr,
+                                            stream, ctypes.byref(h)))
+        self.h = h
+        self.nvar = 1 if scheme.equation == _lib.EQ_BURGERS else scheme.equation + 2
+
+    def close(self):
+        if self.h:
+            _lib.load().alsvinn_hip_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def layout(self, sample):
+        dims = (ctypes.c_int * 6)()
+        off = ctypes.c_longlong()
+        n = ctypes.c_longlong()
+        check(_lib.load().alsvinn_hip_ensemble_layout(self.h, sample, dims, ctypes.byref(off),
+                                                      ctypes.byref(n)))
+        return tuple(dims), off.value, n.value
+
+    def volume_shape(self, sample):
+        (tx, ty, tz, gx, gy, gz), _, _ = self.layout(sample)
+        return (self.nvar, tz, ty, tx)
+
+    def upload(self, sample, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == self.volume_shape(sample), (u.shape, self.volume_shape(sample))
+        check(_lib.load().alsvinn_hip_ensemble_upload(self.h, sample, _dp(u)))
+
+    def download(self, sample):
+        u = np.empty(self.volume_shape(sample))
+        check(_lib.load().alsvinn_hip_ensemble_download(self.h, sample, _dp(u)))
+        return u
+
+    def apply_boundary(self):
+        check(_lib.load().alsvinn_hip_ensemble_apply_boundary(self.h))
+
+    def reset_time(self):
+        check(_lib.load().alsvinn_hip_ensemble_reset_time(self.h))
+
+    def step(self, n=1):
+        check(_lib.load().alsvinn_hip_ensemble_step(self.h, n))
+
+    def run(self, steps_per_sync=16):
+        check(_lib.load().alsvinn_hip_ensemble_run(self.h, steps_per_sync))
+
+    def synchronize(self):
+        check(_lib.load().alsvinn_hip_ensemble_synchronize(self.h))
+
+    def info(self, sample):
+        t = ctypes.c_double()
+        steps = ctypes.c_longlong()
+        dt = ctypes.c_double()
+        st = ctypes.c_int()
+        act = ctypes.c_int()
+        check(_lib.load().alsvinn_hip_ensemble_sample_info(self.h, sample, ctypes.byref(t),
+                                                           ctypes.byref(steps), ctypes.byref(dt),
+                                                           ctypes.byref(st), ctypes.byref(act)))
+        return {"t": t.value, "steps": steps.value, "last_dt": dt.value, "status": st.value,
+                "active": act.value}
+
+    def record_dt(self, capacity):
+        check(_lib.load().alsvinn_hip_ensemble_record_dt(self.h, capacity))
+        self._dt_cap = capacity
+
+    def dt_history(self, sample):
+        out = np.zeros(self._dt_cap)
+        check(_lib.load().alsvinn_hip_ensemble_dt_history(self.h, sample, _dp(out), self._dt_cap))
+        steps = self.info(sample)["steps"]
+        return out[: min(steps, self._dt_cap)]
+
+    def rhs(self, sample):
+        out = np.zeros(self.volume_shape(sample))
+        ws = np.zeros(3)
+        check(_lib.load().alsvinn_hip_ensemble_rhs(self.h, sample, _dp(out), _dp(ws)))
+        return out, ws
+
+    def last_elapsed_ms(self):
+        ms = ctypes.c_double()
+        check(_lib.load().alsvinn_hip_ensemble_last_elapsed_ms(self.h, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,154 @@
+/*
+ * alsvinn_hip.h — C ABI of the MI355X-native "hip" platform for the alsfvm
+ * finite-volume timestep and the alsuq sample statistics.
+ *
+ * Conventions (the reference's DLLWriter precedent,
+ * alsfvm/include/alsfvm/io/DLLWriter.hpp:16-55): plain C types only; every
+ * function returns 0 on success and a non-zero status otherwise, with the
+ * message available from alsvinn_hip_last_error() (thread-local). The C++
+ * side of a caller turns a non-zero status into THROW, as the reference's
+ * CUDA_SAFE_CALL does. Calls are asynchronous on the ensemble's HIP stream
+ * unless stated otherwise; host copies synchronise.
+ *
+ * Volumes are structure-of-arrays with ghost layers on the active axes,
+ * index var*(tx*ty*tz) + z*tx*ty + y*tx + x over the padded extents
+ * (alsfvm/src/volume/Volume.cpp:44-60, alsfvm/include/alsfvm/memory/View.hpp:116-118).
+ */
+#ifndef ALSVINN_HIP_H
+#define ALSVINN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ALSVINN_HIP_EQ_BURGERS = 0,
+    ALSVINN_HIP_EQ_EULER1 = 1,
+    ALSVINN_HIP_EQ_EULER2 = 2,
+    ALSVINN_HIP_EQ_EULER3 = 3
+};
+enum { ALSVINN_HIP_FLUX_HLL = 0, ALSVINN_HIP_FLUX_HLL3 = 1, ALSVINN_HIP_FLUX_GODUNOV = 2 };
+enum { ALSVINN_HIP_RECON_NONE = 0, ALSVINN_HIP_RECON_WENO2 = 1, ALSVINN_HIP_RECON_ENO2 = 2 };
+enum {
+    ALSVINN_HIP_INTEGRATOR_FORWARD_EULER = 0,
+    ALSVINN_HIP_INTEGRATOR_RK2 = 1,
+    ALSVINN_HIP_INTEGRATOR_RK3 = 2
+};
+enum { ALSVINN_HIP_BC_NEUMANN = 0, ALSVINN_HIP_BC_PERIODIC = 1 };
+enum { ALSVINN_HIP_FP64 = 0, ALSVINN_HIP_FP32 = 1 };
+
+/* Status codes. */
+enum {
+    ALSVINN_HIP_OK = 0,
+    ALSVINN_HIP_E_INVALID = 1,   /* bad argument / unsupported combination */
+    ALSVINN_HIP_E_HIP = 2,       /* HIP runtime error */
+    ALSVINN_HIP_E_CONSTRAINT = 3 /* a sample violated the state constraints */
+};
+
+/* The numerical scheme: equation, flux, reconstruction, integrator, boundary,
+ * storage precision. Mirrors the <fvm> options the reference's factories are
+ * keyed on (NumericalFluxFactory.cpp:43-200, ReconstructionFactory.cpp:41-355,
+ * IntegratorFactory.cpp:31-44, BoundaryFactory.cpp:38-75,
+ * SimulatorSetup.cpp:306-337 for cfl). */
+typedef struct alsvinn_hip_scheme {
+    int equation;
+    int flux;
+    int reconstruction;
+    int integrator;
+    int boundary;
+    int precision;
+    double gamma;
+    double cfl;
+} alsvinn_hip_scheme;
+
+/* One solve of an ensemble: its grid (Grid.cpp:24-41), end time and the
+ * FixedInterval time-step adjusters attached to it (FixedIntervalWriter.cpp:45-53,
+ * FixedIntervalStatistics.cpp:33-41: next save = saved*interval). */
+typedef struct alsvinn_hip_sample {
+    int nx, ny, nz;
+    int num_adjusters;   /* 0..2 */
+    double origin[3];
+    double top[3];
+    double end_time;
+    double interval[2];
+    long long saved0[2]; /* initial save counters (1 when the initial state is not written) */
+    long long max_steps; /* -1: unbounded */
+} alsvinn_hip_sample;
+
+typedef struct alsvinn_hip_ensemble alsvinn_hip_ensemble;
+
+const char* alsvinn_hip_last_error(void);
+
+/* Number of ghost layers the scheme needs: max(flux stencil - 1, reconstruction
+ * ghosts) (NumericalFluxCPU.cpp:226-231). */
+int alsvinn_hip_ghost_cells(const alsvinn_hip_scheme* scheme);
+
+/* Ensemble: many solves of one scheme advanced together. Replaces the
+ * reference's sequential sample loop (alsuq/src/run/Runner.cpp:42-78) and the
+ * Simulator objects it creates per sample (Simulator.cpp:26-64). stream may be
+ * NULL (the library creates one). The ensemble owns its state buffers. */
+int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
+                                const alsvinn_hip_sample* samples, void* stream,
+                                alsvinn_hip_ensemble** out);
+int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e);
+
+/* Padded extents (tx,ty,tz), ghost layers (gx,gy,gz) and the element offset of
+ * a sample's volume in the state buffer. */
+int alsvinn_hip_ensemble_layout(const alsvinn_hip_ensemble* e, int sample, int dims6[6],
+                                long long* offset, long long* elements_per_variable);
+
+/* Host copies of a sample's full conserved volume (nvar x tz x ty x tx, FP64 on
+ * the host whatever the storage precision). Synchronous. */
+int alsvinn_hip_ensemble_upload(alsvinn_hip_ensemble* e, int sample, const double* host);
+int alsvinn_hip_ensemble_download(alsvinn_hip_ensemble* e, int sample, double* host);
+
+/* Device pointer of the current conserved state buffer (all samples). */
+int alsvinn_hip_ensemble_state_pointer(alsvinn_hip_ensemble* e, void** ptr);
+
+/* Boundary::applyBoundaryConditions on every sample's current state
+ * (Simulator::setInitialValue, Simulator.cpp:160). */
+int alsvinn_hip_ensemble_apply_boundary(alsvinn_hip_ensemble* e);
+
+/* Resets every sample to t = 0, steps = 0, initial save counters, active. */
+int alsvinn_hip_ensemble_reset_time(alsvinn_hip_ensemble* e);
+
+/* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
+ * every sample that has not reached its end time. Asynchronous. */
+int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps);
+
+/* Steps until every sample reached its end time (or failed), replaying a
+ * captured HIP graph of steps_per_sync steps between host checks. Synchronous.
+ * Returns ALSVINN_HIP_E_CONSTRAINT if a sample violated the constraints. */
+int alsvinn_hip_ensemble_run(alsvinn_hip_ensemble* e, int steps_per_sync);
+
+int alsvinn_hip_ensemble_synchronize(alsvinn_hip_ensemble* e);
+
+/* Per-sample time, steps performed, last dt, status (0 ok, 2 constraint
+ * violation) and active flag. Synchronous. */
+int alsvinn_hip_ensemble_sample_info(alsvinn_hip_ensemble* e, int sample, double* t,
+                                     long long* steps, double* last_dt, int* status,
+                                     int* active);
+
+/* Records the dt of the first `capacity` steps of every sample; read back with
+ * alsvinn_hip_ensemble_dt_history. */
+int alsvinn_hip_ensemble_record_dt(alsvinn_hip_ensemble* e, long long capacity);
+int alsvinn_hip_ensemble_dt_history(alsvinn_hip_ensemble* e, int sample, double* host,
+                                    long long capacity);
+
+/* One evaluation of the spatial operator on every sample's current state:
+ * ConservedSystem::operator() = NumericalFlux::computeFlux over all regions
+ * (ConservedSystem.cpp:31-103, NumericalFluxCPU.cpp:170-221). The result
+ * (interior cells) and the per-direction max wave speeds of one sample are
+ * copied to the host. Synchronous. */
+int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_out,
+                             double wave_speeds[3]);
+
+/* Time of the kernels enqueued by the last alsvinn_hip_ensemble_step/run
+ * call, from HIP events on the ensemble's stream (milliseconds). */
+int alsvinn_hip_ensemble_last_elapsed_ms(alsvinn_hip_ensemble* e, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALSVINN_HIP_H */

@@ -1,0 +1,178 @@
+"""Parity of the hip platform against the CPU oracle (FP64, bit-exact).
+
+Every test runs the HIP kernels through the C ABI (alsvinn_amd -> libalsvinn_hip.so)
+and the oracle (oracle/liboracle.so) on the same initial data, then compares
+the conserved variables cell for cell (interior and ghost cells), the dt
+sequence and the number of steps. Integer/index work and the CFL max are
+exact; the FP64 arithmetic follows the reference's operation order, so the
+expected difference is zero (the test tolerance is 0 ulp).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name):
+    print("CASE", name, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_setup(eq, flux, recon, integ, bc, n, origin, top, cfl, T, saves=(1,), max_steps=-1):
+    return O.make_setup(eq, flux, recon, integ, bc, n, origin, top, cfl=cfl, end_time=T,
+                        saves=saves, max_steps=max_steps)
+
+
+def _run_both(hip, eq, flux, recon, integ, bc, n, origin, top, cfl, T, u0, max_steps=-1,
+              saves=(1,)):
+    s = _oracle_setup(eq, flux, recon, integ, bc, n, origin, top, cfl, T, saves, max_steps)
+    rc, ref, t_ref, steps_ref, dts_ref = O.run(s, u0)
+    assert rc == 0
+    scheme = hip.make_scheme(eq, flux, recon, integ, bc, gamma=1.4, cfl=cfl)
+    sample = hip.make_sample(n, origin, top, T, saves=saves, max_steps=max_steps)
+    with hip.Ensemble(scheme, [sample]) as e:
+        e.record_dt(max(1, steps_ref + 1))
+        e.upload(0, u0)
+        e.apply_boundary()
+        e.run(steps_per_sync=8)
+        got = e.download(0)
+        info = e.info(0)
+        dts = e.dt_history(0)
+    return ref, got, (t_ref, steps_ref, dts_ref), (info, dts)
+
+
+def _assert_bit_exact(ref, got):
+    assert ref.shape == got.shape
+    diff = np.flatnonzero(ref.view(np.uint64) != got.view(np.uint64))
+    if diff.size:
+        i = diff[0]
+        idx = np.unravel_index(i, ref.shape)
+        rel = np.abs(ref - got).sum() / max(np.abs(ref).sum(), 1e-300)
+        pytest.fail("%d cells differ; first at %s: ref=%r got=%r; rel L1=%.3e"
+                    % (diff.size, idx, ref[idx], got[idx], rel))
+
+
+def test_sod_fe_hll_bit_exact(hip):
+    _print_case("sod 1024 hll none fe")
+    from alsvinn_amd import initial_data as ic
+    u0 = ic.sod(1024, ghosts=1)
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER1, O.FLUX_HLL, O.RECON_NONE, O.INT_FE, O.BC_NEUMANN, (1024, 1, 1),
+        (-1, 0, 0), (1, 0, 0), 0.9, 0.2, u0)
+    assert info["steps"] == steps_ref
+    assert info["t"] == t_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+@pytest.mark.parametrize("recon,flux,integ", [
+    (O.RECON_WENO2, O.FLUX_HLL3, O.INT_RK3),
+    (O.RECON_NONE, O.FLUX_HLL, O.INT_RK2),
+    (O.RECON_ENO2, O.FLUX_HLL3, O.INT_FE),
+])
+def test_euler1_schemes_bit_exact(hip, recon, flux, integ):
+    _print_case("euler1 sod 200 recon=%d flux=%d integ=%d" % (recon, flux, integ))
+    from alsvinn_amd import initial_data as ic
+    g = 1 if recon == O.RECON_NONE else 2
+    u0 = ic.sod(200, ghosts=g)
+    cfl = 0.9 if recon == O.RECON_NONE else 0.475
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER1, flux, recon, integ, O.BC_NEUMANN, (200, 1, 1), (-1, 0, 0), (1, 0, 0),
+        cfl, 0.1, u0)
+    assert info["steps"] == steps_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_kh_weno2_hll3_rk3_bit_exact(hip):
+    _print_case("kh 64x64 weno2 hll3 rk3 periodic 40 steps")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(1, 40)[0]
+    u0 = ic.kelvin_helmholtz_2d(64, X, ghosts=2)
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC, (64, 64, 1),
+        (0, 0, 0), (1, 1, 0), 0.475, 2.0, u0, max_steps=40)
+    assert info["steps"] == steps_ref == 40
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_riemann2d_neumann_bit_exact(hip):
+    _print_case("riemann4 80x72? no: 96x96 none hll fe neumann")
+    from alsvinn_amd import initial_data as ic
+    params = ic.riemann_parameters(np.linspace(0.05, 0.95, 16))
+    u0 = ic.riemann_four_quadrant(96, params, ghosts=1)
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER2, O.FLUX_HLL, O.RECON_NONE, O.INT_FE, O.BC_NEUMANN, (96, 96, 1),
+        (0, 0, 0), (1, 1, 0), 0.9, 0.05, u0)
+    assert info["steps"] == steps_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_burgers_eno2_godunov_rk2_bit_exact(hip):
+    _print_case("burgers 4096 eno2 godunov rk2 periodic")
+    from alsvinn_amd import initial_data as ic
+    u0 = ic.burgers_fbm(4096, 0.5, seed=7, ghosts=2)
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_ENO2, O.INT_RK2, O.BC_PERIODIC,
+        (4096, 1, 1), (0, 0, 0), (1, 0, 0), 0.475, 0.05, u0)
+    assert info["steps"] == steps_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_rhs_operator_matches_conserved_system(hip):
+    """One ConservedSystem::operator() evaluation on a non-trivial state:
+    interior RHS and the per-direction max wave speeds, bit for bit."""
+    _print_case("rhs operator kh 48x40")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(2, 40)[1]
+    n = (48, 40, 1)
+    s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC, n,
+                     (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=1.0)
+    # KH on a non-square grid: build from a 48x48 field cut to 40 rows
+    u_sq = ic.kelvin_helmholtz_2d(48, X, ghosts=2)
+    u0 = np.ascontiguousarray(u_sq[:, :, :44, :])
+    rng = np.random.default_rng(3)
+    u0[0] *= 1.0 + 0.05 * rng.random(u0[0].shape)
+    u0[1] += 0.1 * rng.standard_normal(u0[1].shape)
+    u0 = O.apply_boundary(s, u0)
+    out_ref, ws_ref = O.system_rhs(s, u0)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    with hip.Ensemble(scheme, [hip.make_sample(n, (0, 0, 0), (1, 1, 0), 1.0)]) as e:
+        e.upload(0, u0)
+        out, ws = e.rhs(0)
+    g = 2
+    _assert_bit_exact(out_ref[:, :, g:-g, g:-g], out[:, :, g:-g, g:-g])
+    assert ws[0] == ws_ref[0] and ws[1] == ws_ref[1]
+
+
+def test_ensemble_of_samples_equals_single_runs(hip):
+    """Samples advanced together give the same bits as each run alone."""
+    _print_case("ensemble 3 kh 32x32 vs single")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    Xs = sampling.stl_mersenne_uniform(3, 40)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    samples = [hip.make_sample((32, 32, 1), (0, 0, 0), (1, 1, 0), 0.3) for _ in range(3)]
+    u0s = [ic.kelvin_helmholtz_2d(32, X) for X in Xs]
+    with hip.Ensemble(scheme, samples) as e:
+        for i, u in enumerate(u0s):
+            e.upload(i, u)
+        e.apply_boundary()
+        e.run(4)
+        batch = [e.download(i) for i in range(3)]
+        steps = [e.info(i)["steps"] for i in range(3)]
+    for i in range(3):
+        s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                         (32, 32, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=0.3)
+        rc, ref, t, st, _ = O.run(s, u0s[i])
+        assert rc == 0 and st == steps[i]
+        _assert_bit_exact(ref, batch[i])
