@@ -33,6 +33,8 @@ EXPORTS = [
     "alsvinn_hip_ensemble_reset_time",
     "alsvinn_hip_ensemble_step",
     "alsvinn_hip_ensemble_run",
+    "alsvinn_hip_ensemble_run_steps",
+    "alsvinn_hip_ensemble_step_timed",
     "alsvinn_hip_ensemble_synchronize",
     "alsvinn_hip_ensemble_sample_info",
     "alsvinn_hip_ensemble_record_dt",
@@ -105,6 +107,8 @@ def load():
     L.alsvinn_hip_ensemble_reset_time.argtypes = [vp]
     L.alsvinn_hip_ensemble_step.argtypes = [vp, ctypes.c_int]
     L.alsvinn_hip_ensemble_run.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_ensemble_run_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.alsvinn_hip_ensemble_step_timed.argtypes = [vp, ctypes.c_int, dp, llp]
     L.alsvinn_hip_ensemble_synchronize.argtypes = [vp]
     L.alsvinn_hip_ensemble_sample_info.argtypes = [vp, ctypes.c_int, dp, llp, dp, ip, ip]
     L.alsvinn_hip_ensemble_record_dt.argtypes = [vp, ctypes.c_longlong]

@@ -170,8 +170,26 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
     }
 }
 
+// Optional per-kernel timing: HIP events around every launch on the
+// ensemble's stream, synchronised after each kernel (measurement mode only).
+struct KernelTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms[3] = {0, 0, 0};  // stage kernels, elementwise, bookkeeping
+    long long count[3] = {0, 0, 0};
+    hipStream_t s = nullptr;
+    void begin() { hipEventRecord(a, s); }
+    void end(int kind) {
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float f = 0;
+        hipEventElapsedTime(&f, a, b);
+        ms[kind] += f;
+        count[kind] += 1;
+    }
+};
+
 template <class T>
-int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k) {
+int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm = nullptr) {
     StageArgs<T> a;
     a.desc = e->d_desc;
     a.state = e->d_state;
@@ -193,44 +211,56 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k) {
     a.in = U;
     a.in2 = U;
     a.out = A;
-    k.stage(a, blocks, e->stream);
+    auto stage = [&]() {
+        if (tm) tm->begin();
+        k.stage(a, blocks, e->stream);
+        if (tm) tm->end(0);
+    };
+    auto axpy = [&](const T* X, const T* Y, T* O, int check) {
+        if (tm) tm->begin();
+        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, X, Y, O, check, e->scheme.gamma);
+        if (tm) tm->end(1);
+    };
+    stage();
     if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_FORWARD_EULER) {
-        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, A, U, U, 1, e->scheme.gamma);
+        axpy(A, U, U, 1);
     } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK2) {
-        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, A, U, A, 0, e->scheme.gamma);
+        axpy(A, U, A, 0);
         a.mode = MODE_RK2_1;
         a.in = A;
         a.in2 = U;
         a.out = U;
         a.check = 1;
-        k.stage(a, blocks, e->stream);
+        stage();
     } else {
         a.mode = MODE_RK3_1;
         a.lazy = 1;
         a.in = A;
         a.in2 = U;
         a.out = B;
-        k.stage(a, blocks, e->stream);
+        stage();
         a.mode = MODE_RK3_2;
         a.lazy = 0;
         a.in = B;
         a.in2 = U;
         a.out = U;
         a.check = 1;
-        k.stage(a, blocks, e->stream);
+        stage();
     }
+    if (tm) tm->begin();
     hipLaunchKernelGGL(bookkeeping_kernel, dim3((e->nsamples + 255) / 256), dim3(256), 0,
                        e->stream, e->d_desc, e->d_state, e->d_ws, e->d_dt_hist, e->dt_cap,
                        e->nsamples);
+    if (tm) tm->end(2);
     return 0;
 }
 
-int enqueue_steps(alsvinn_hip_ensemble* e, int nsteps) {
+int enqueue_steps(alsvinn_hip_ensemble* e, int nsteps, KernelTimer* tm = nullptr) {
     for (int i = 0; i < nsteps; ++i) {
         if (e->esize == 8)
-            enqueue_step<double>(e, e->k64);
+            enqueue_step<double>(e, e->k64, tm);
         else
-            enqueue_step<float>(e, e->k32);
+            enqueue_step<float>(e, e->k32, tm);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -485,23 +515,57 @@ int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps) {
     return 0;
 }
 
+static int ensure_graph(alsvinn_hip_ensemble* e, int steps) {
+    if (e->graph_exec && e->graph_steps == steps) return 0;
+    destroy_graph(e);
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_steps(e, steps);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    e->graph = g;
+    HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+    e->graph_steps = steps;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_run_steps(alsvinn_hip_ensemble* e, int nsteps, int steps_per_graph) {
+    if (!e || nsteps < 0 || steps_per_graph < 1 || nsteps % steps_per_graph != 0)
+        return fail(ALSVINN_HIP_E_INVALID, "nsteps must be a multiple of steps_per_graph");
+    if (int rc = ensure_graph(e, steps_per_graph)) return rc;
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    for (int i = 0; i < nsteps / steps_per_graph; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_step_timed(alsvinn_hip_ensemble* e, int nsteps, double kernel_ms[3],
+                                    long long launches[3]) {
+    if (!e || nsteps < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    KernelTimer tm;
+    tm.s = e->stream;
+    HIP_TRY(hipEventCreate(&tm.a));
+    HIP_TRY(hipEventCreate(&tm.b));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    int rc = enqueue_steps(e, nsteps, &tm);
+    hipStreamSynchronize(e->stream);
+    hipEventDestroy(tm.a);
+    hipEventDestroy(tm.b);
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) {
+        if (kernel_ms) kernel_ms[k] = tm.ms[k];
+        if (launches) launches[k] = tm.count[k];
+    }
+    return 0;
+}
+
 int alsvinn_hip_ensemble_run(alsvinn_hip_ensemble* e, int steps_per_sync) {
     if (!e || steps_per_sync < 1) return fail(ALSVINN_HIP_E_INVALID, "bad run arguments");
-    if (!e->graph_exec || e->graph_steps != steps_per_sync) {
-        destroy_graph(e);
-        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(e, steps_per_sync);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(e->stream, &g);
-        if (rc) {
-            if (g) hipGraphDestroy(g);
-            return rc;
-        }
-        if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-        e->graph = g;
-        HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
-        e->graph_steps = steps_per_sync;
-    }
+    if (int rc = ensure_graph(e, steps_per_sync)) return rc;
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
     for (;;) {
         HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));

@@ -109,6 +109,15 @@ class Ensemble:
     def run(self, steps_per_sync=16):
         check(_lib.load().alsvinn_hip_ensemble_run(self.h, steps_per_sync))
 
+    def run_steps(self, nsteps, steps_per_graph=10):
+        check(_lib.load().alsvinn_hip_ensemble_run_steps(self.h, nsteps, steps_per_graph))
+
+    def step_timed(self, nsteps):
+        ms = np.zeros(3)
+        n = (ctypes.c_longlong * 3)()
+        check(_lib.load().alsvinn_hip_ensemble_step_timed(self.h, nsteps, _dp(ms), n))
+        return ms, list(n)
+
     def synchronize(self):
         check(_lib.load().alsvinn_hip_ensemble_synchronize(self.h))
 

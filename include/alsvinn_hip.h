@@ -121,6 +121,18 @@ int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps);
  * Returns ALSVINN_HIP_E_CONSTRAINT if a sample violated the constraints. */
 int alsvinn_hip_ensemble_run(alsvinn_hip_ensemble* e, int steps_per_sync);
 
+/* Replays the captured graph of steps_per_graph timesteps nsteps/steps_per_graph
+ * times without host checks (samples past their end time skip on the device).
+ * Asynchronous. */
+int alsvinn_hip_ensemble_run_steps(alsvinn_hip_ensemble* e, int nsteps, int steps_per_graph);
+
+/* Measurement mode: enqueues nsteps timesteps with HIP events around every
+ * kernel (synchronising after each) and returns the summed kernel times in
+ * milliseconds and launch counts for [stage kernels, elementwise kernels,
+ * bookkeeping]. Synchronous. */
+int alsvinn_hip_ensemble_step_timed(alsvinn_hip_ensemble* e, int nsteps, double kernel_ms[3],
+                                    long long launches[3]);
+
 int alsvinn_hip_ensemble_synchronize(alsvinn_hip_ensemble* e);
 
 /* Per-sample time, steps performed, last dt, status (0 ok, 2 constraint
