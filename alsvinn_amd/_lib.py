@@ -31,6 +31,9 @@ EXPORTS = [
     "alsvinn_hip_ensemble_state_pointer",
     "alsvinn_hip_ensemble_apply_boundary",
     "alsvinn_hip_ensemble_reset_time",
+    "alsvinn_hip_ensemble_set_schedule",
+    "alsvinn_hip_ensemble_get_schedule",
+    "alsvinn_hip_ensemble_autotune",
     "alsvinn_hip_ensemble_step",
     "alsvinn_hip_ensemble_run",
     "alsvinn_hip_ensemble_run_steps",
@@ -105,6 +108,9 @@ def load():
     L.alsvinn_hip_ensemble_state_pointer.argtypes = [vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_ensemble_apply_boundary.argtypes = [vp]
     L.alsvinn_hip_ensemble_reset_time.argtypes = [vp]
+    L.alsvinn_hip_ensemble_set_schedule.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.alsvinn_hip_ensemble_get_schedule.argtypes = [vp, ip, ip, ip]
+    L.alsvinn_hip_ensemble_autotune.argtypes = [vp, ip, ip]
     L.alsvinn_hip_ensemble_step.argtypes = [vp, ctypes.c_int]
     L.alsvinn_hip_ensemble_run.argtypes = [vp, ctypes.c_int]
     L.alsvinn_hip_ensemble_run_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int]

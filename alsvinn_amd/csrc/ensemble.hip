@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/alsvinn_hip.h"
@@ -101,6 +103,9 @@ struct alsvinn_hip_ensemble {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     long long max_vstride = 0;
+    int variant = 0;    // stage kernel register budget (kStageVariants)
+    int band_rows = 0;  // rows per wave task; 0 = heuristic
+    bool tuned = false;
     KernelSet<double> k64;
     KernelSet<float> k32;
 };
@@ -145,46 +150,109 @@ int band_rows(long long tasks_per_band, int ny) {
     return ty;
 }
 
+// Wave tasks: every sample is cut into bands of rows and strips of W = 64-2G
+// interior columns. A full strip fills a wave; the narrower last strips of
+// samples with the same rows are packed together, up to kMaxSeg per wave.
 void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
     const int G = e->ghosts;
     const int W = 64 - 2 * G;
-    long long strips = 0;
-    for (int s = 0; s < e->nsamples; ++s) strips += (e->desc[s].nx + W - 1) / W;
+    long long waves = 0;
+    for (int s = 0; s < e->nsamples; ++s) waves += (e->desc[s].nx + W - 1) / W;
+    std::map<std::tuple<int, int, int>, size_t> open;  // (ny, y0, y1) -> task index
     for (int s = 0; s < e->nsamples; ++s) {
         const SampleDesc& D = e->desc[s];
-        const int ty = e->dim >= 2 ? band_rows(strips, D.ny) : 1;
+        const int ty = e->dim < 2 ? 1
+                       : (e->band_rows > 0 ? std::min(e->band_rows, D.ny) : band_rows(waves, D.ny));
         for (int y0 = 0; y0 < D.ny; y0 += ty) {
-            for (int x0 = 0; x0 < D.nx; x0 += W) {
+            const int y1 = std::min(D.ny, y0 + ty);
+            int x0 = 0;
+            for (; x0 + W <= D.nx; x0 += W) {
                 WaveTask t;
-                t.sample = s;
-                t.x0 = x0;
-                t.ncols = std::min(W, D.nx - x0);
+                std::memset(&t, 0, sizeof(t));
+                t.nseg = 1;
                 t.y0 = y0;
-                t.y1 = std::min(D.ny, y0 + ty);
+                t.y1 = y1;
                 t.z0 = 0;
                 t.z1 = std::max(1, D.nz);
-                t.pad = 0;
+                t.sample[0] = s;
+                t.x0[0] = x0;
+                t.ncols[0] = W;
+                t.lane0[0] = 0;
                 tasks.push_back(t);
+            }
+            if (x0 < D.nx) {
+                const int ncols = D.nx - x0;
+                const int lanes = ncols + 2 * G;
+                const auto key = std::make_tuple(D.ny, y0, y1);
+                auto it = open.find(key);
+                bool placed = false;
+                if (it != open.end()) {
+                    WaveTask& t = tasks[it->second];
+                    const int used = t.lane0[t.nseg - 1] + t.ncols[t.nseg - 1] + 2 * G;
+                    if (t.nseg < kMaxSeg && used + lanes <= 64) {
+                        t.sample[t.nseg] = s;
+                        t.x0[t.nseg] = x0;
+                        t.ncols[t.nseg] = ncols;
+                        t.lane0[t.nseg] = used;
+                        t.nseg++;
+                        placed = true;
+                    }
+                }
+                if (!placed) {
+                    WaveTask t;
+                    std::memset(&t, 0, sizeof(t));
+                    t.nseg = 1;
+                    t.y0 = y0;
+                    t.y1 = y1;
+                    t.z0 = 0;
+                    t.z1 = std::max(1, D.nz);
+                    t.sample[0] = s;
+                    t.x0[0] = x0;
+                    t.ncols[0] = ncols;
+                    t.lane0[0] = 0;
+                    open[key] = tasks.size();
+                    tasks.push_back(t);
+                }
             }
         }
     }
 }
 
-// Optional per-kernel timing: HIP events around every launch on the
-// ensemble's stream, synchronised after each kernel (measurement mode only).
+// Optional per-kernel timing: a pair of HIP events around every launch on
+// the ensemble's stream (no host synchronisation between launches); the
+// durations are summed once the stream has drained.
 struct KernelTimer {
-    hipEvent_t a = nullptr, b = nullptr;
-    double ms[3] = {0, 0, 0};  // stage kernels, elementwise, bookkeeping
-    long long count[3] = {0, 0, 0};
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
     hipStream_t s = nullptr;
-    void begin() { hipEventRecord(a, s); }
-    void end(int kind) {
+    void begin() {
+        hipEvent_t a;
+        hipEventCreate(&a);
+        hipEventRecord(a, s);
+        ev.push_back(a);
+    }
+    void end(int k) {
+        hipEvent_t b;
+        hipEventCreate(&b);
         hipEventRecord(b, s);
-        hipEventSynchronize(b);
-        float f = 0;
-        hipEventElapsedTime(&f, a, b);
-        ms[kind] += f;
-        count[kind] += 1;
+        ev.push_back(b);
+        kind.push_back(k);
+    }
+    void collect(double ms[3], long long count[3]) {
+        hipStreamSynchronize(s);
+        for (int k = 0; k < 3; ++k) {
+            ms[k] = 0;
+            count[k] = 0;
+        }
+        for (size_t i = 0; i < kind.size(); ++i) {
+            float f = 0;
+            hipEventElapsedTime(&f, ev[2 * i], ev[2 * i + 1]);
+            ms[kind[i]] += f;
+            count[kind[i]] += 1;
+        }
+        for (auto x : ev) hipEventDestroy(x);
+        ev.clear();
+        kind.clear();
     }
 };
 
@@ -213,7 +281,7 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     a.out = A;
     auto stage = [&]() {
         if (tm) tm->begin();
-        k.stage(a, blocks, e->stream);
+        k.stage[e->variant](a, blocks, e->stream);
         if (tm) tm->end(0);
     };
     auto axpy = [&](const T* X, const T* Y, T* O, int check) {
@@ -290,6 +358,45 @@ void destroy_graph(alsvinn_hip_ensemble* e) {
     e->graph_exec = nullptr;
     e->graph = nullptr;
     e->graph_steps = 0;
+}
+
+int upload_tasks(alsvinn_hip_ensemble* e) {
+    std::vector<WaveTask> tasks;
+    build_tasks(e, tasks);
+    hipFree(e->d_tasks);
+    e->d_tasks = nullptr;
+    e->ntasks = (int)tasks.size();
+    HIP_TRY(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
+    HIP_TRY(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Times `reps` first-stage launches (reconstruct + flux + divergence +
+// wave speeds on the current state; writes only the scratch buffer and the
+// wave-speed slots, which are cleared afterwards).
+template <class T>
+double time_first_stage(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps) {
+    StageArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.desc = e->d_desc;
+    a.state = e->d_state;
+    a.ws = e->d_ws;
+    a.tasks = e->d_tasks;
+    a.ntasks = e->ntasks;
+    a.gamma = e->scheme.gamma;
+    a.mode = MODE_RHS;
+    a.in = (T*)e->U;
+    a.in2 = (T*)e->U;
+    a.out = (T*)e->A;
+    const int blocks = (e->ntasks + 3) / 4;
+    k.stage[e->variant](a, blocks, e->stream);  // warm-up
+    hipEventRecord(e->ev0, e->stream);
+    for (int i = 0; i < reps; ++i) k.stage[e->variant](a, blocks, e->stream);
+    hipEventRecord(e->ev1, e->stream);
+    hipEventSynchronize(e->ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e->ev0, e->ev1);
+    return ms / reps;
 }
 
 }  // namespace
@@ -507,6 +614,70 @@ int alsvinn_hip_ensemble_reset_time(alsvinn_hip_ensemble* e) {
     return init_state(e);
 }
 
+int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int variant) {
+    if (!e || rows < 0 || variant < 0 || variant >= kStageVariants)
+        return fail(ALSVINN_HIP_E_INVALID, "bad schedule");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    e->band_rows = rows;
+    e->variant = variant;
+    e->tuned = true;
+    return upload_tasks(e);
+}
+
+int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
+                                      int* ntasks) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    if (rows) *rows = e->band_rows;
+    if (variant) *variant = e->variant;
+    if (ntasks) *ntasks = e->ntasks;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* variant) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    int maxny = 1;
+    for (const auto& D : e->desc) maxny = std::max(maxny, D.ny);
+    std::vector<int> cand;
+    if (e->dim >= 2) {
+        for (int r = maxny; r >= 8; r = (r + 1) / 2) {
+            cand.push_back(r);
+            if (r == 8) break;
+        }
+    } else {
+        cand.push_back(1);
+    }
+    double best = 1e300;
+    int best_rows = cand[0], best_var = 0;
+    const int reps = 2;
+    for (int v = 0; v < kStageVariants; ++v) {
+        for (int r : cand) {
+            e->variant = v;
+            e->band_rows = r;
+            if (int rc = upload_tasks(e)) return rc;
+            const double ms = e->esize == 8 ? time_first_stage<double>(e, e->k64, reps)
+                                            : time_first_stage<float>(e, e->k32, reps);
+            if (ms < best) {
+                best = ms;
+                best_rows = r;
+                best_var = v;
+            }
+        }
+    }
+    std::vector<unsigned long long> z(3 * (size_t)e->nsamples, 0ull);
+    HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+    e->variant = best_var;
+    e->band_rows = best_rows;
+    e->tuned = true;
+    if (int rc = upload_tasks(e)) return rc;
+    HIP_TRY(hipGetLastError());
+    if (rows) *rows = best_rows;
+    if (variant) *variant = best_var;
+    return 0;
+}
+
 int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps) {
     if (!e || nsteps < 0) return fail(ALSVINN_HIP_E_INVALID, "bad step arguments");
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
@@ -548,17 +719,16 @@ int alsvinn_hip_ensemble_step_timed(alsvinn_hip_ensemble* e, int nsteps, double 
     if (!e || nsteps < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
     KernelTimer tm;
     tm.s = e->stream;
-    HIP_TRY(hipEventCreate(&tm.a));
-    HIP_TRY(hipEventCreate(&tm.b));
     HIP_TRY(hipStreamSynchronize(e->stream));
     int rc = enqueue_steps(e, nsteps, &tm);
-    hipStreamSynchronize(e->stream);
-    hipEventDestroy(tm.a);
-    hipEventDestroy(tm.b);
+    double ms[3];
+    long long n[3];
+    tm.collect(ms, n);
     if (rc) return rc;
+    HIP_TRY(hipGetLastError());
     for (int k = 0; k < 3; ++k) {
-        if (kernel_ms) kernel_ms[k] = tm.ms[k];
-        if (launches) launches[k] = tm.count[k];
+        if (kernel_ms) kernel_ms[k] = ms[k];
+        if (launches) launches[k] = n[k];
     }
     return 0;
 }
@@ -656,7 +826,7 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
         a.in = (double*)e->U;
         a.in2 = (double*)e->U;
         a.out = (double*)e->A;
-        e->k64.stage(a, blocks, e->stream);
+        e->k64.stage[e->variant](a, blocks, e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (host_out) HIP_TRY(hipMemcpy(host_out, buf_at<double>(e->A, D.base), n * 8, hipMemcpyDeviceToHost));
@@ -673,7 +843,7 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
         a.in = (float*)e->U;
         a.in2 = (float*)e->U;
         a.out = (float*)e->A;
-        e->k32.stage(a, blocks, e->stream);
+        e->k32.stage[e->variant](a, blocks, e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (host_out) {

@@ -14,18 +14,29 @@ int fail(int code, const std::string& msg);
 
 // Launchers of the kernels one scheme (equation, reconstruction, flux, grid
 // dimension, precision) needs.
+constexpr int kStageVariants = 3;  // 2, 3, 4 waves per SIMD
+inline int variant_waves(int v) { return 2 + v; }
+
 template <class T>
 struct KernelSet {
-    void (*stage)(const StageArgs<T>& a, int blocks, hipStream_t s) = nullptr;
+    void (*stage[kStageVariants])(const StageArgs<T>& a, int blocks, hipStream_t s) = {};
     void (*axpy)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                  const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
                  double gamma) = nullptr;
 };
 
-template <class T, class Eq, class Recon, class Flux, int DIM>
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
 void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS>), dim3(blocks), dim3(256), 0,
+                       s, a);
+}
+
+template <class T, class Eq, class Recon, class Flux, int DIM>
+void set_stage(KernelSet<T>& k) {
+    k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
+    k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
+    k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 4>;
 }
 
 template <class T, class Eq>
@@ -40,9 +51,9 @@ template <class T, class Eq, class Flux, int DIM>
 int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
     k.axpy = &launch_axpy<T, Eq>;
     switch (sc.reconstruction) {
-        case ALSVINN_HIP_RECON_NONE: k.stage = &launch_stage<T, Eq, ReconNone, Flux, DIM>; return 0;
-        case ALSVINN_HIP_RECON_WENO2: k.stage = &launch_stage<T, Eq, ReconWENO2, Flux, DIM>; return 0;
-        case ALSVINN_HIP_RECON_ENO2: k.stage = &launch_stage<T, Eq, ReconENO2, Flux, DIM>; return 0;
+        case ALSVINN_HIP_RECON_NONE: set_stage<T, Eq, ReconNone, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_WENO2: set_stage<T, Eq, ReconWENO2, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_ENO2: set_stage<T, Eq, ReconENO2, Flux, DIM>(k); return 0;
     }
     return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction");
 }

@@ -53,14 +53,21 @@ struct SampleState {
     int status;  // 0 ok, 2 constraint violation
 };
 
-// A strip of one sample: interior columns [x0, x0+ncols), rows [y0, y1),
-// planes [z0, z1).
+// A wavefront's work: up to four segments, each a strip of interior
+// columns [x0, x0+ncols) of one sample occupying lanes
+// [lane0, lane0 + ncols + 2G) (G halo lanes on each side), all marching
+// over the same rows [y0, y1). Packing the narrow last strips of several
+// samples into one wave keeps the lanes busy.
+constexpr int kMaxSeg = 4;
 struct WaveTask {
-    int sample;
-    int x0, ncols;
+    int nseg;
     int y0, y1;
     int z0, z1;
-    int pad;
+    int pad[3];
+    int sample[kMaxSeg];
+    int x0[kMaxSeg];
+    int ncols[kMaxSeg];
+    int lane0[kMaxSeg];
 };
 
 template <class T>
@@ -90,8 +97,9 @@ __device__ __forceinline__ double sample_dt(const SampleDesc& D, const SampleSta
         total += __longlong_as_double((long long)ws[d]) / D.dx[d];
     }
     double dt = D.cfl / total;
-    for (int k = 0; k < D.nadj; ++k) {
-        if (S.saved[k] > 0) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (k < D.nadj && S.saved[k] > 0) {
             const double next = double(S.saved[k]) * D.interval[k] + 0.0;
             const double cand = next - S.t;
             dt = (cand < dt) ? cand : dt;
@@ -100,35 +108,39 @@ __device__ __forceinline__ double sample_dt(const SampleDesc& D, const SampleSta
     return dt;
 }
 
-// Padded positions holding the value of interior cell c on one axis: the
-// cell itself plus its ghost images (Neumann mirror / periodic wrap).
-__device__ __forceinline__ int axis_images(int c, int N, int G, int bc, int* pos) {
-    pos[0] = c + G;
-    int n = 1;
-    if (G == 0) return n;
-    if (bc == 1) {
-        if (c >= N - G) pos[n++] = c - N + G;
-        if (c < G) pos[n++] = c + G + N;
-    } else {
-        if (c < G) pos[n++] = G - 1 - c;
-        if (c >= N - G) pos[n++] = 2 * N + G - 1 - c;
-    }
-    return n;
+// Lane i receives the value of lane i-1 (lane_from_lower) or lane i+1
+// (lane_from_upper); the lane without a source keeps its own value. DPP
+// wave_shr:1 / wave_shl:1 moves, two 32-bit halves for a double.
+__device__ __forceinline__ int dpp_shr1(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_shl1(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double lane_from_lower(double v) {
+    return __hiloint2double(dpp_shr1(__double2hiint(v)), dpp_shr1(__double2loint(v)));
+}
+__device__ __forceinline__ double lane_from_upper(double v) {
+    return __hiloint2double(dpp_shl1(__double2hiint(v)), dpp_shl1(__double2loint(v)));
+}
+__device__ __forceinline__ float lane_from_lower(float v) {
+    return __int_as_float(dpp_shr1(__float_as_int(v)));
+}
+__device__ __forceinline__ float lane_from_upper(float v) {
+    return __int_as_float(dpp_shl1(__float_as_int(v)));
 }
 
+// Max over the lanes of each contiguous segment; the first lane of a
+// segment ends up holding its segment's max.
 template <class T>
-__device__ __forceinline__ T shfl_up1(T v) {
-    return __shfl_up(v, 1, 64);
-}
-template <class T>
-__device__ __forceinline__ T shfl_down1(T v) {
-    return __shfl_down(v, 1, 64);
-}
-
-template <class T>
-__device__ __forceinline__ T wave_max(T v) {
+__device__ __forceinline__ T segment_max(T v, int seg) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = std_max(v, __shfl_xor(v, off, 64));
+    for (int off = 1; off < 64; off <<= 1) {
+        const T o = __shfl_down(v, off, 64);
+        const int so = __shfl_down(seg, off, 64);
+        if (lane + off < 64 && so == seg) v = std_max(v, o);
+    }
     return v;
 }
 
@@ -181,9 +193,49 @@ __device__ __forceinline__ void combine(int mode, T c, const T* Lv, const T* uin
     }
 }
 
-// 1D and 2D stage kernel (DIM = active grid dimensions).
-template <class T, class Eq, class Recon, class Flux, int DIM>
-__global__ __launch_bounds__(256) void stage_kernel_12(StageArgs<T> a) {
+// Writes one result cell to its padded position and to every ghost cell
+// that mirrors it (at most one image per axis side).
+template <class T, int NV, int DIM>
+__device__ __forceinline__ void store_cell(T* out, long long vs, int tx, int col, int row, int nx,
+                                           int ny, int gx, int gy, int bc, const T* res) {
+    int xa = col + gx, xb = -1, xc = -1;
+    if (bc == 1) {
+        if (col >= nx - gx) xb = col - nx + gx;
+        if (col < gx) xc = col + gx + nx;
+    } else {
+        if (col < gx) xb = gx - 1 - col;
+        if (col >= nx - gx) xc = 2 * nx + gx - 1 - col;
+    }
+    int ya = row + gy, yb = -1, yc = -1;
+    if (DIM >= 2) {
+        if (bc == 1) {
+            if (row >= ny - gy) yb = row - ny + gy;
+            if (row < gy) yc = row + gy + ny;
+        } else {
+            if (row < gy) yb = gy - 1 - row;
+            if (row >= ny - gy) yc = 2 * ny + gy - 1 - row;
+        }
+    }
+#pragma unroll
+    for (int iy = 0; iy < 3; ++iy) {
+        const int py = iy == 0 ? ya : (iy == 1 ? yb : yc);
+        if (py < 0) continue;
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix) {
+            const int px = ix == 0 ? xa : (ix == 1 ? xb : xc);
+            if (px < 0) continue;
+            const long long o = (long long)py * tx + px;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
+        }
+    }
+}
+
+// 1D and 2D stage kernel (DIM = active grid dimensions). WPS is the
+// register budget expressed as waves per SIMD (2: no spills for FP64 Euler
+// 2D; 3 and 4 trade spills for occupancy); the autotuner picks one.
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
+__global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     const int lane = threadIdx.x & 63;
@@ -193,19 +245,28 @@ __global__ __launch_bounds__(256) void stage_kernel_12(StageArgs<T> a) {
     eq.gamma = T(a.gamma);
 
     for (int w = wave; w < a.ntasks; w += nwaves) {
-        const WaveTask tk = a.tasks[w];
-        const SampleDesc D = a.desc[tk.sample];
-        SampleState* Sp = a.state + tk.sample;
-        if (!Sp->active) continue;  // uniform across the wave
+        const WaveTask& tk = a.tasks[w];
+        // this lane's segment
+        int seg = -1;
+        for (int k = 0; k < tk.nseg; ++k)
+            if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
+        const bool in_strip = seg >= 0;
+        const int sk = in_strip ? seg : 0;
+        const int sample = tk.sample[sk];
+        const int li = lane - tk.lane0[sk];
+        const int ncols = tk.ncols[sk];
+        const SampleDesc D = a.desc[sample];
+        SampleState* Sp = a.state + sample;
+        const bool active = in_strip && Sp->active;
+        if (!__any(active)) continue;  // wave-uniform
         const bool need_dt = a.lazy || a.mode != MODE_RHS;
         T c = T(0);
-        if (need_dt) c = T(sample_dt(D, *Sp, a.ws + 3 * tk.sample) / D.dx[0]);
+        if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
 
-        const int col = tk.x0 - G + lane;  // interior x coordinate of this lane
-        const bool in_strip = lane < tk.ncols + 2 * G;
-        const bool interior = lane >= G && lane < G + tk.ncols;
-        const bool xface = lane >= G - 1 && lane < G + tk.ncols;  // face col+1/2
-        const int px = in_strip ? col + D.gx : D.gx;               // padded column
+        const int col = tk.x0[sk] - G + li;  // interior x coordinate of this lane
+        const bool interior = active && li >= G && li < G + ncols;
+        const bool xface = active && li >= G - 1 && li < G + ncols;  // face col+1/2
+        const int px = in_strip ? col + D.gx : D.gx;                 // padded column
         const long long vs = D.vstride;
         const T* in = a.in + D.base;
         const T* in2 = a.in2 + D.base;
@@ -220,9 +281,9 @@ __global__ __launch_bounds__(256) void stage_kernel_12(StageArgs<T> a) {
             return (long long)py * D.tx + px;
         };
 
-        T yc[NV], yp[NV];          // rows r, r+1 of the stage input
-        T Ry[NV];                  // right (r+1/2) reconstruction of row r in y
-        T Gm[NV];                  // y face flux at r-1/2
+        T yc[NV], yp[NV];  // rows r, r+1 of the stage input
+        T Ry[NV];          // right (r+1/2) reconstruction of row r in y
+        T Gm[NV];          // y face flux at r-1/2
         const int gy = D.gy;
         if constexpr (DIM >= 2) {
             const int pr = tk.y0 + gy;
@@ -245,34 +306,35 @@ __global__ __launch_bounds__(256) void stage_kernel_12(StageArgs<T> a) {
 
         for (int r = tk.y0; r < tk.y1; ++r) {
             const int pr = r + gy;
+            T qn[NV];
+            // next row first: its latency overlaps the x sweep
+            if constexpr (DIM >= 2) load_cell<T, NV>(in, in2, a.lazy, c, row_index(pr + 2), vs, qn);
             T out_v[NV];
             // ---- x direction on row r (first: out = 0 + x divergence)
             {
                 T xm[NV], xp[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    xm[v] = shfl_up1(yc[v]);
-                    xp[v] = shfl_down1(yc[v]);
+                    xm[v] = lane_from_lower(yc[v]);
+                    xp[v] = lane_from_upper(yc[v]);
                 }
                 T Lx[NV], Rx[NV];
                 Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx);
                 T Ln[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) Ln[v] = shfl_down1(Lx[v]);
+                for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
                 T Fp[NV];
                 const T s = face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp);
                 if (xface) wsx = std_max(wsx, s);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const T Fm = shfl_up1(Fp[v]);
+                    const T Fm = lane_from_lower(Fp[v]);
                     const T t0 = (T(-1.0) * Fp[v]) + Fm;
                     out_v[v] = T(0) + t0;
                 }
             }
             // ---- y direction: out += (-G(r+1/2)) + G(r-1/2)
             if constexpr (DIM >= 2) {
-                T qn[NV];
-                load_cell<T, NV>(in, in2, a.lazy, c, row_index(pr + 2), vs, qn);
                 T Ln[NV], Rn[NV];
                 Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn);  // row r+1
                 T Gp[NV];
@@ -284,59 +346,34 @@ __global__ __launch_bounds__(256) void stage_kernel_12(StageArgs<T> a) {
                     Gm[v] = Gp[v];
                     Ry[v] = Rn[v];
                 }
-                // epilogue below uses yc (row r); shift afterwards
-                if (interior) {
-                    const long long i0 = (long long)pr * D.tx + px;
-                    T u0[NV], res[NV];
-                    if (a.mode != MODE_RHS) {
+            }
+            if (interior) {
+                const long long i0 = (long long)pr * D.tx + px;
+                T u0[NV], res[NV];
+                if (a.mode != MODE_RHS) {
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
-                    }
-                    combine<T, NV>(a.mode, c, out_v, yc, u0, res);
-                    if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                    int xs[3], ys[3];
-                    const int nxi = axis_images(col, D.nx, D.gx, D.bc, xs);
-                    const int nyi = axis_images(r, D.ny, D.gy, D.bc, ys);
-                    for (int iy = 0; iy < nyi; ++iy)
-                        for (int ix = 0; ix < nxi; ++ix) {
-                            const long long o = (long long)ys[iy] * D.tx + xs[ix];
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
-                        }
+                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
                 }
+                combine<T, NV>(a.mode, c, out_v, yc, u0, res);
+                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
+                store_cell<T, NV, DIM>(out, vs, D.tx, col, r, D.nx, D.ny, D.gx, D.gy, D.bc, res);
+            }
+            if constexpr (DIM >= 2) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     yc[v] = yp[v];
                     yp[v] = qn[v];
                 }
-            } else {
-                if (interior) {
-                    const long long i0 = px;
-                    T u0[NV], res[NV];
-                    if (a.mode != MODE_RHS) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
-                    }
-                    combine<T, NV>(a.mode, c, out_v, yc, u0, res);
-                    if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                    int xs[3];
-                    const int nxi = axis_images(col, D.nx, D.gx, D.bc, xs);
-                    for (int ix = 0; ix < nxi; ++ix) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) out[xs[ix] + v * vs] = res[v];
-                    }
-                }
             }
         }
 
         if (a.mode == MODE_RHS && !a.lazy) {
-            const T mx = wave_max(wsx);
-            const T my = wave_max(wsy);
-            if (lane == 0) {
-                atomicMax(a.ws + 3 * tk.sample + 0,
-                          (unsigned long long)__double_as_longlong(double(mx)));
+            const T mx = segment_max(wsx, seg);
+            const T my = segment_max(wsy, seg);
+            if (in_strip && li == 0 && Sp->active) {
+                atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
                 if (DIM >= 2)
-                    atomicMax(a.ws + 3 * tk.sample + 1,
+                    atomicMax(a.ws + 3 * sample + 1,
                               (unsigned long long)__double_as_longlong(double(my)));
             }
         }

@@ -103,6 +103,21 @@ class Ensemble:
     def reset_time(self):
         check(_lib.load().alsvinn_hip_ensemble_reset_time(self.h))
 
+    def set_schedule(self, rows, variant):
+        check(_lib.load().alsvinn_hip_ensemble_set_schedule(self.h, rows, variant))
+
+    def schedule(self):
+        r, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().alsvinn_hip_ensemble_get_schedule(self.h, ctypes.byref(r), ctypes.byref(v),
+                                                            ctypes.byref(n)))
+        return {"rows": r.value, "variant": v.value, "waves_per_simd": 2 + v.value,
+                "tasks": n.value}
+
+    def autotune(self):
+        r, v = ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().alsvinn_hip_ensemble_autotune(self.h, ctypes.byref(r), ctypes.byref(v)))
+        return self.schedule()
+
     def step(self, n=1):
         check(_lib.load().alsvinn_hip_ensemble_step(self.h, n))
 

@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
 
@@ -105,6 +106,8 @@ def main():
     log("generating %d samples of %d^2 (reference sampler draws)" % (M, n))
     X = sampling.stl_mersenne_uniform(M, 40)
     e = make_batch(hip, M, n, args.precision, X)
+    sched = e.autotune()
+    log("autotuned schedule: %s" % sched)
     log("warmup %d steps" % warm)
     e.run_steps(warm, per_graph)
     e.synchronize()
@@ -128,6 +131,8 @@ def main():
     plain = None
     if not args.no_plain:
         ep = make_batch(hip, 1, n, args.precision, X[:1])
+        psched = ep.autotune()
+        log("plain schedule: %s" % psched)
         ep.run_steps(per_graph, per_graph)
         ep.synchronize()
         ps = max(per_graph, (args.plain_steps // per_graph) * per_graph)
@@ -141,7 +146,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(n, X[0], 12)
+        probe = cpu_baseline(n, X[0], 4)
+        steps_cpu = int(max(4, min(4000, args.cpu_seconds * probe["value"] / (n * n))))
+        cpu = cpu_baseline(n, X[0], steps_cpu)
         log("cpu baseline: %.3e cell-updates/s on %d threads" % (cpu["value"], cpu["cores"]))
 
     B = STAGES * 3 * NVAR * esize
@@ -162,7 +169,7 @@ def main():
                                "SSP-RK3, CFL 0.475, periodic" % (n, M),
                    "samples": M, "grid": [n, n], "cells_per_step": cells,
                    "bytes_per_cell_update": B,
-                   "hbm_fraction": value * B / HBM_PEAK},
+                   "hbm_fraction": value * B / HBM_PEAK, "schedule": sched},
         "plain": plain,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,

@@ -112,6 +112,18 @@ int alsvinn_hip_ensemble_apply_boundary(alsvinn_hip_ensemble* e);
 /* Resets every sample to t = 0, steps = 0, initial save counters, active. */
 int alsvinn_hip_ensemble_reset_time(alsvinn_hip_ensemble* e);
 
+/* Work schedule of the stage kernels: rows per wavefront task (0 = built-in
+ * heuristic) and the register-budget variant (0, 1, 2 = 2, 3, 4 waves per
+ * SIMD). Results do not depend on the schedule, only the speed does. */
+int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int variant);
+int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
+                                      int* ntasks);
+
+/* Autotuner: times the first-stage kernel on the current state for every
+ * variant and a ladder of band heights, keeps the fastest schedule.
+ * Synchronous; leaves the conserved state untouched. */
+int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* variant);
+
 /* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
  * every sample that has not reached its end time. Asynchronous. */
 int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps);

@@ -176,3 +176,42 @@ def test_ensemble_of_samples_equals_single_runs(hip):
         rc, ref, t, st, _ = O.run(s, u0s[i])
         assert rc == 0 and st == steps[i]
         _assert_bit_exact(ref, batch[i])
+
+
+@pytest.mark.parametrize("n", [(76, 24, 1), (16, 16, 1)])
+def test_packed_strips_bit_exact(hip, n):
+    """Grids whose last strip is narrow: several samples' strips share one
+    wavefront (76 = 60 + 16 columns; 16 columns = one narrow strip). Each
+    sample must still match its own oracle run bit for bit."""
+    _print_case("packed strips %s" % (n,))
+    from alsvinn_amd import initial_data as ic
+    rng = np.random.default_rng(11)
+    M = 5
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    samples = [hip.make_sample(n, (0, 0, 0), (1, 1, 0), 0.2) for _ in range(M)]
+    s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC, n,
+                     (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=0.2)
+    shape = O.volume_shape(s)
+    u0s = []
+    for i in range(M):
+        u = np.zeros(shape)
+        g = 2
+        rho = 1.0 + 0.5 * rng.random((n[1], n[0]))
+        vx = 0.3 * rng.standard_normal((n[1], n[0]))
+        vy = 0.3 * rng.standard_normal((n[1], n[0]))
+        p = 1.0 + 0.5 * rng.random((n[1], n[0]))
+        cons = ic.conserved_euler(rho, [vx, vy], p, 1.4)
+        for k in range(4):
+            u[k, 0, g:-g, g:-g] = cons[k]
+        u0s.append(u)
+    with hip.Ensemble(scheme, samples) as e:
+        for i, u in enumerate(u0s):
+            e.upload(i, u)
+        e.apply_boundary()
+        e.run(5)
+        got = [e.download(i) for i in range(M)]
+        steps = [e.info(i)["steps"] for i in range(M)]
+    for i in range(M):
+        rc, ref, t, st, _ = O.run(s, u0s[i])
+        assert rc == 0 and st == steps[i]
+        _assert_bit_exact(ref, got[i])
