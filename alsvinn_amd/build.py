@@ -20,6 +20,7 @@ SOURCES = [
     "kernels_burgers.hip",
     "kernels_euler1.hip",
     "kernels_euler2.hip",
+    "kernels_euler3.hip",
 ]
 
 FLAGS = [

@@ -103,7 +103,7 @@ struct alsvinn_hip_ensemble {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     long long max_vstride = 0;
-    int variant = 0;    // stage kernel register budget (kStageVariants)
+    int variant = kDefaultVariant;  // stage kernel register budget
     int band_rows = 0;  // rows per wave task; 0 = heuristic
     bool tuned = false;
     KernelSet<double> k64;
@@ -157,63 +157,68 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
     const int G = e->ghosts;
     const int W = 64 - 2 * G;
     long long waves = 0;
-    for (int s = 0; s < e->nsamples; ++s) waves += (e->desc[s].nx + W - 1) / W;
-    std::map<std::tuple<int, int, int>, size_t> open;  // (ny, y0, y1) -> task index
+    for (int s = 0; s < e->nsamples; ++s)
+        waves += (long long)((e->desc[s].nx + W - 1) / W) * (e->dim == 3 ? e->desc[s].ny : 1);
+    // (ny, nz, y0, y1, z0, z1) -> index of the task still open for packing
+    std::map<std::tuple<int, int, int, int, int, int>, size_t> open;
+    auto add_strips = [&](int s, int y0, int y1, int z0, int z1) {
+        const SampleDesc& D = e->desc[s];
+        int x0 = 0;
+        for (; x0 + W <= D.nx; x0 += W) {
+            WaveTask t;
+            std::memset(&t, 0, sizeof(t));
+            t.nseg = 1;
+            t.y0 = y0;
+            t.y1 = y1;
+            t.z0 = z0;
+            t.z1 = z1;
+            t.sample[0] = s;
+            t.x0[0] = x0;
+            t.ncols[0] = W;
+            tasks.push_back(t);
+        }
+        if (x0 < D.nx) {
+            const int ncols = D.nx - x0;
+            const int lanes = ncols + 2 * G;
+            const auto key = std::make_tuple(D.ny, D.nz, y0, y1, z0, z1);
+            auto it = open.find(key);
+            if (it != open.end()) {
+                WaveTask& t = tasks[it->second];
+                const int used = t.lane0[t.nseg - 1] + t.ncols[t.nseg - 1] + 2 * G;
+                if (t.nseg < kMaxSeg && used + lanes <= 64) {
+                    t.sample[t.nseg] = s;
+                    t.x0[t.nseg] = x0;
+                    t.ncols[t.nseg] = ncols;
+                    t.lane0[t.nseg] = used;
+                    t.nseg++;
+                    return;
+                }
+            }
+            WaveTask t;
+            std::memset(&t, 0, sizeof(t));
+            t.nseg = 1;
+            t.y0 = y0;
+            t.y1 = y1;
+            t.z0 = z0;
+            t.z1 = z1;
+            t.sample[0] = s;
+            t.x0[0] = x0;
+            t.ncols[0] = ncols;
+            open[key] = tasks.size();
+            tasks.push_back(t);
+        }
+    };
     for (int s = 0; s < e->nsamples; ++s) {
         const SampleDesc& D = e->desc[s];
-        const int ty = e->dim < 2 ? 1
-                       : (e->band_rows > 0 ? std::min(e->band_rows, D.ny) : band_rows(waves, D.ny));
-        for (int y0 = 0; y0 < D.ny; y0 += ty) {
-            const int y1 = std::min(D.ny, y0 + ty);
-            int x0 = 0;
-            for (; x0 + W <= D.nx; x0 += W) {
-                WaveTask t;
-                std::memset(&t, 0, sizeof(t));
-                t.nseg = 1;
-                t.y0 = y0;
-                t.y1 = y1;
-                t.z0 = 0;
-                t.z1 = std::max(1, D.nz);
-                t.sample[0] = s;
-                t.x0[0] = x0;
-                t.ncols[0] = W;
-                t.lane0[0] = 0;
-                tasks.push_back(t);
-            }
-            if (x0 < D.nx) {
-                const int ncols = D.nx - x0;
-                const int lanes = ncols + 2 * G;
-                const auto key = std::make_tuple(D.ny, y0, y1);
-                auto it = open.find(key);
-                bool placed = false;
-                if (it != open.end()) {
-                    WaveTask& t = tasks[it->second];
-                    const int used = t.lane0[t.nseg - 1] + t.ncols[t.nseg - 1] + 2 * G;
-                    if (t.nseg < kMaxSeg && used + lanes <= 64) {
-                        t.sample[t.nseg] = s;
-                        t.x0[t.nseg] = x0;
-                        t.ncols[t.nseg] = ncols;
-                        t.lane0[t.nseg] = used;
-                        t.nseg++;
-                        placed = true;
-                    }
-                }
-                if (!placed) {
-                    WaveTask t;
-                    std::memset(&t, 0, sizeof(t));
-                    t.nseg = 1;
-                    t.y0 = y0;
-                    t.y1 = y1;
-                    t.z0 = 0;
-                    t.z1 = std::max(1, D.nz);
-                    t.sample[0] = s;
-                    t.x0[0] = x0;
-                    t.ncols[0] = ncols;
-                    t.lane0[0] = 0;
-                    open[key] = tasks.size();
-                    tasks.push_back(t);
-                }
-            }
+        if (e->dim == 3) {
+            // one row per task, bands of planes marched in z
+            const int tz = e->band_rows > 0 ? std::min(e->band_rows, D.nz) : band_rows(waves, D.nz);
+            for (int z0 = 0; z0 < D.nz; z0 += tz)
+                for (int y = 0; y < D.ny; ++y) add_strips(s, y, y + 1, z0, std::min(D.nz, z0 + tz));
+        } else {
+            const int ty = e->dim < 2 ? 1
+                           : (e->band_rows > 0 ? std::min(e->band_rows, D.ny) : band_rows(waves, D.ny));
+            for (int y0 = 0; y0 < D.ny; y0 += ty) add_strips(s, y0, std::min(D.ny, y0 + ty), 0, 1);
         }
     }
 }
@@ -639,7 +644,7 @@ int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* varia
     HIP_TRY(hipStreamSynchronize(e->stream));
     destroy_graph(e);
     int maxny = 1;
-    for (const auto& D : e->desc) maxny = std::max(maxny, D.ny);
+    for (const auto& D : e->desc) maxny = std::max(maxny, e->dim == 3 ? D.nz : D.ny);
     std::vector<int> cand;
     if (e->dim >= 2) {
         for (int r = maxny; r >= 8; r = (r + 1) / 2) {

@@ -14,8 +14,9 @@ int fail(int code, const std::string& msg);
 
 // Launchers of the kernels one scheme (equation, reconstruction, flux, grid
 // dimension, precision) needs.
-constexpr int kStageVariants = 3;  // 2, 3, 4 waves per SIMD
-inline int variant_waves(int v) { return 2 + v; }
+constexpr int kStageVariants = 3;  // register budgets for 1, 2, 3 waves per SIMD
+constexpr int kDefaultVariant = 1;
+inline int variant_waves(int v) { return 1 + v; }
 
 template <class T>
 struct KernelSet {
@@ -28,15 +29,18 @@ struct KernelSet {
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
 void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS>), dim3(blocks), dim3(256), 0,
-                       s, a);
+    if constexpr (DIM == 3)
+        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS>), dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS>), dim3(blocks), dim3(256), 0,
+                           s, a);
 }
 
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
-    k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
-    k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
-    k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 4>;
+    k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 1>;
+    k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
+    k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
 }
 
 template <class T, class Eq>
@@ -65,6 +69,8 @@ template <class T>
 int select_euler1(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
 template <class T>
 int select_euler2(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
+template <class T>
+int select_euler3(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k);
 
 template <class T>
 int select_kernels(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
@@ -72,6 +78,7 @@ int select_kernels(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
         case ALSVINN_HIP_EQ_BURGERS: return select_burgers<T>(sc, dim, k);
         case ALSVINN_HIP_EQ_EULER1: return select_euler1<T>(sc, dim, k);
         case ALSVINN_HIP_EQ_EULER2: return select_euler2<T>(sc, dim, k);
+        case ALSVINN_HIP_EQ_EULER3: return select_euler3<T>(sc, dim, k);
     }
     return fail(ALSVINN_HIP_E_INVALID, "equation not available on the hip platform: " +
                                             std::to_string(sc.equation));

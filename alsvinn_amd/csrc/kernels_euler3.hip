@@ -1,0 +1,16 @@
+// Kernel instantiations for the 3D Euler equations (HLL, HLL3).
+#include "internal.hpp"
+
+namespace alsvinn_hip {
+
+template <class T>
+int select_euler3(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
+    if (dim != 3) return fail(ALSVINN_HIP_E_INVALID, "euler3 needs a 3D grid");
+    if (sc.flux == ALSVINN_HIP_FLUX_HLL) return select_recon<T, EulerEq<T, 3>, FluxHLL, 3>(sc, k);
+    return select_recon<T, EulerEq<T, 3>, FluxHLL3, 3>(sc, k);
+}
+
+template int select_euler3<double>(const alsvinn_hip_scheme&, int, KernelSet<double>&);
+template int select_euler3<float>(const alsvinn_hip_scheme&, int, KernelSet<float>&);
+
+}  // namespace alsvinn_hip

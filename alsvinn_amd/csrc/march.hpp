@@ -193,40 +193,49 @@ __device__ __forceinline__ void combine(int mode, T c, const T* Lv, const T* uin
     }
 }
 
+// Padded positions of interior index c on one axis: the cell itself (a)
+// plus the ghost cells that mirror it (b: bottom side, c: top side; -1 when
+// none).
+__device__ __forceinline__ void axis_images(int c, int N, int G, int bc, int& a, int& b, int& d) {
+    a = c + G;
+    b = -1;
+    d = -1;
+    if (G == 0) return;
+    if (bc == 1) {
+        if (c >= N - G) b = c - N + G;
+        if (c < G) d = c + G + N;
+    } else {
+        if (c < G) b = G - 1 - c;
+        if (c >= N - G) d = 2 * N + G - 1 - c;
+    }
+}
+
 // Writes one result cell to its padded position and to every ghost cell
 // that mirrors it (at most one image per axis side).
 template <class T, int NV, int DIM>
-__device__ __forceinline__ void store_cell(T* out, long long vs, int tx, int col, int row, int nx,
-                                           int ny, int gx, int gy, int bc, const T* res) {
-    int xa = col + gx, xb = -1, xc = -1;
-    if (bc == 1) {
-        if (col >= nx - gx) xb = col - nx + gx;
-        if (col < gx) xc = col + gx + nx;
-    } else {
-        if (col < gx) xb = gx - 1 - col;
-        if (col >= nx - gx) xc = 2 * nx + gx - 1 - col;
-    }
-    int ya = row + gy, yb = -1, yc = -1;
-    if (DIM >= 2) {
-        if (bc == 1) {
-            if (row >= ny - gy) yb = row - ny + gy;
-            if (row < gy) yc = row + gy + ny;
-        } else {
-            if (row < gy) yb = gy - 1 - row;
-            if (row >= ny - gy) yc = 2 * ny + gy - 1 - row;
-        }
-    }
+__device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDesc& D, int col,
+                                           int row, int plane, const T* res) {
+    int x[3], y[3], z[3];
+    axis_images(col, D.nx, D.gx, D.bc, x[0], x[1], x[2]);
+    y[0] = 0;
+    y[1] = y[2] = -1;
+    z[0] = 0;
+    z[1] = z[2] = -1;
+    if (DIM >= 2) axis_images(row, D.ny, D.gy, D.bc, y[0], y[1], y[2]);
+    if (DIM >= 3) axis_images(plane, D.nz, D.gz, D.bc, z[0], z[1], z[2]);
 #pragma unroll
-    for (int iy = 0; iy < 3; ++iy) {
-        const int py = iy == 0 ? ya : (iy == 1 ? yb : yc);
-        if (py < 0) continue;
+    for (int iz = 0; iz < (DIM >= 3 ? 3 : 1); ++iz) {
+        if (z[iz] < 0) continue;
 #pragma unroll
-        for (int ix = 0; ix < 3; ++ix) {
-            const int px = ix == 0 ? xa : (ix == 1 ? xb : xc);
-            if (px < 0) continue;
-            const long long o = (long long)py * tx + px;
+        for (int iy = 0; iy < (DIM >= 2 ? 3 : 1); ++iy) {
+            if (y[iy] < 0) continue;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
+            for (int ix = 0; ix < 3; ++ix) {
+                if (x[ix] < 0) continue;
+                const long long o = ((long long)z[iz] * D.ty + y[iy]) * D.tx + x[ix];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
+            }
         }
     }
 }
@@ -356,7 +365,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 }
                 combine<T, NV>(a.mode, c, out_v, yc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, DIM>(out, vs, D.tx, col, r, D.nx, D.ny, D.gx, D.gy, D.bc, res);
+                store_cell<T, NV, DIM>(out, vs, D, col, r, 0, res);
             }
             if constexpr (DIM >= 2) {
 #pragma unroll
@@ -375,6 +384,160 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 if (DIM >= 2)
                     atomicMax(a.ws + 3 * sample + 1,
                               (unsigned long long)__double_as_longlong(double(my)));
+            }
+        }
+    }
+}
+
+// 3D stage kernel. A wavefront owns x-strips of one row y and marches in z
+// with a register window (z neighbours); x neighbours come from adjacent
+// lanes, y neighbours from loads of rows y-2..y+2 of the current plane (the
+// y reconstructions of rows y-1 and y+1 are recomputed per row task).
+template <class T, class Eq, class Recon, class Flux, int WPS>
+__global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
+    constexpr int NV = Eq::nvar;
+    constexpr int G = Recon::ghosts;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    Eq eq;
+    eq.gamma = T(a.gamma);
+
+    for (int w = wave; w < a.ntasks; w += nwaves) {
+        const WaveTask& tk = a.tasks[w];
+        int seg = -1;
+        for (int k = 0; k < tk.nseg; ++k)
+            if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
+        const bool in_strip = seg >= 0;
+        const int sk = in_strip ? seg : 0;
+        const int sample = tk.sample[sk];
+        const int li = lane - tk.lane0[sk];
+        const int ncols = tk.ncols[sk];
+        const SampleDesc D = a.desc[sample];
+        SampleState* Sp = a.state + sample;
+        const bool active = in_strip && Sp->active;
+        if (!__any(active)) continue;
+        const bool need_dt = a.lazy || a.mode != MODE_RHS;
+        T c = T(0);
+        if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
+
+        const int col = tk.x0[sk] - G + li;
+        const bool interior = active && li >= G && li < G + ncols;
+        const bool xface = active && li >= G - 1 && li < G + ncols;
+        const int px = in_strip ? col + D.gx : D.gx;
+        const int row = tk.y0;  // one interior row per task
+        const int py = row + D.gy;
+        const long long vs = D.vstride;
+        const long long plane_stride = (long long)D.tx * D.ty;
+        const T* in = a.in + D.base;
+        const T* in2 = a.in2 + D.base;
+        T* out = a.out + D.base;
+
+        T wsx = T(0), wsy = T(0), wsz = T(0);
+        auto idx = [&](int pyy, int pz) -> long long {
+            pz = pz < 0 ? 0 : (pz >= D.tz ? D.tz - 1 : pz);
+            pyy = pyy < 0 ? 0 : (pyy >= D.ty ? D.ty - 1 : pyy);
+            return (long long)pz * plane_stride + (long long)pyy * D.tx + px;
+        };
+
+        T zc[NV], zp[NV], Rz[NV], Hm[NV];
+        {
+            const int pz = tk.z0 + D.gz;
+            T qa[NV], zm[NV];
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz - 2), vs, qa);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz - 1), vs, zm);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz), vs, zc);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz + 1), vs, zp);
+            T Lt[NV], Rt[NV], L0[NV], R0[NV];
+            Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt);
+            Recon::template apply<T, NV>(zm, zc, zp, L0, R0);
+            const T s = face_flux<T, Eq, Flux, 2>(eq, Rt, L0, Hm);
+            if (interior) wsz = std_max(wsz, s);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Rz[v] = R0[v];
+        }
+
+        for (int zz = tk.z0; zz < tk.z1; ++zz) {
+            const int pz = zz + D.gz;
+            T qn[NV], ym2[NV], ym1[NV], yp1[NV], yp2[NV];
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz + 2), vs, qn);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py - 2, pz), vs, ym2);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py - 1, pz), vs, ym1);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py + 1, pz), vs, yp1);
+            load_cell<T, NV>(in, in2, a.lazy, c, idx(py + 2, pz), vs, yp2);
+            T out_v[NV];
+            {  // x
+                T xm[NV], xp[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    xm[v] = lane_from_lower(zc[v]);
+                    xp[v] = lane_from_upper(zc[v]);
+                }
+                T Lx[NV], Rx[NV];
+                Recon::template apply<T, NV>(xm, zc, xp, Lx, Rx);
+                T Ln[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
+                T Fp[NV];
+                const T s = face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp);
+                if (xface) wsx = std_max(wsx, s);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const T Fm = lane_from_lower(Fp[v]);
+                    out_v[v] = T(0) + ((T(-1.0) * Fp[v]) + Fm);
+                }
+            }
+            {  // y: faces y-1/2 and y+1/2 from rows y-2..y+2
+                T Lm[NV], Rm[NV], L0[NV], R0[NV], Lp[NV], Rp[NV];
+                Recon::template apply<T, NV>(ym2, ym1, zc, Lm, Rm);
+                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0);
+                Recon::template apply<T, NV>(zc, yp1, yp2, Lp, Rp);
+                T Gm[NV], Gp[NV];
+                const T s0 = face_flux<T, Eq, Flux, 1>(eq, Rm, L0, Gm);
+                const T s1 = face_flux<T, Eq, Flux, 1>(eq, R0, Lp, Gp);
+                if (interior) wsy = std_max(std_max(wsy, s0), s1);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+            }
+            {  // z
+                T Ln[NV], Rn[NV];
+                Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn);
+                T Hp[NV];
+                const T s = face_flux<T, Eq, Flux, 2>(eq, Rz, Ln, Hp);
+                if (interior) wsz = std_max(wsz, s);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    out_v[v] = out_v[v] + ((T(-1.0) * Hp[v]) + Hm[v]);
+                    Hm[v] = Hp[v];
+                    Rz[v] = Rn[v];
+                }
+            }
+            if (interior) {
+                const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
+                T u0[NV], res[NV];
+                if (a.mode != MODE_RHS) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                }
+                combine<T, NV>(a.mode, c, out_v, zc, u0, res);
+                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
+                store_cell<T, NV, 3>(out, vs, D, col, row, zz, res);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                zc[v] = zp[v];
+                zp[v] = qn[v];
+            }
+        }
+
+        if (a.mode == MODE_RHS && !a.lazy) {
+            const T mx = segment_max(wsx, seg);
+            const T my = segment_max(wsy, seg);
+            const T mz = segment_max(wsz, seg);
+            if (in_strip && li == 0 && Sp->active) {
+                atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
+                atomicMax(a.ws + 3 * sample + 1, (unsigned long long)__double_as_longlong(double(my)));
+                atomicMax(a.ws + 3 * sample + 2, (unsigned long long)__double_as_longlong(double(mz)));
             }
         }
     }

@@ -110,7 +110,7 @@ class Ensemble:
         r, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(_lib.load().alsvinn_hip_ensemble_get_schedule(self.h, ctypes.byref(r), ctypes.byref(v),
                                                             ctypes.byref(n)))
-        return {"rows": r.value, "variant": v.value, "waves_per_simd": 2 + v.value,
+        return {"rows": r.value, "variant": v.value, "waves_per_simd": 1 + v.value,
                 "tasks": n.value}
 
     def autotune(self):

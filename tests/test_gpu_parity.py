@@ -215,3 +215,80 @@ def test_packed_strips_bit_exact(hip, n):
         rc, ref, t, st, _ = O.run(s, u0s[i])
         assert rc == 0 and st == steps[i]
         _assert_bit_exact(ref, got[i])
+
+
+@pytest.mark.parametrize("recon,flux,integ", [
+    (O.RECON_WENO2, O.FLUX_HLL3, O.INT_RK3),
+    (O.RECON_NONE, O.FLUX_HLL, O.INT_FE),
+])
+def test_kh3d_bit_exact(hip, recon, flux, integ):
+    """3D Kelvin-Helmholtz (10x10 modes in x and z), 20^3, periodic."""
+    _print_case("kh3d 20^3 recon=%d" % recon)
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    g = 1 if recon == O.RECON_NONE else 2
+    n = 20
+    X = sampling.stl_mersenne_uniform(1, 400)[0]
+    u0 = ic.kelvin_helmholtz_3d(n, X, ghosts=g)
+    cfl = 0.9 if recon == O.RECON_NONE else 0.475
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER3, flux, recon, integ, O.BC_PERIODIC, (n, n, n), (0, 0, 0), (1, 1, 1),
+        cfl, 2.0, u0, max_steps=6)
+    assert info["steps"] == steps_ref == 6
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_euler3_neumann_bit_exact(hip):
+    """3D with Neumann boundaries on a non-cubic grid (ghost images on all axes)."""
+    _print_case("euler3 neumann 22x12x9")
+    rng = np.random.default_rng(5)
+    n = (22, 12, 9)
+    from alsvinn_amd import initial_data as ic
+    s = O.make_setup(O.EQ_EULER3, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_NEUMANN, n,
+                     (0, 0, 0), (1, 1, 1), cfl=0.475, end_time=1.0, max_steps=4)
+    u = np.zeros(O.volume_shape(s))
+    shp = (n[2], n[1], n[0])
+    cons = ic.conserved_euler(1 + rng.random(shp), [0.3 * rng.standard_normal(shp) for _ in range(3)],
+                              1 + rng.random(shp), 1.4)
+    for k in range(5):
+        u[k, 2:-2, 2:-2, 2:-2] = cons[k]
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER3, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_NEUMANN, n, (0, 0, 0),
+        (1, 1, 1), 0.475, 1.0, u, max_steps=4)
+    assert info["steps"] == 4
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_fp32_state_error_vs_fp64_oracle(hip):
+    """FP32 state (FP32 reconstruction and fluxes, FP64 CFL): the relative L1
+    error of rho against the FP64 oracle after 50 steps of KH 64^2 stays below
+    1e-4 (the float rounding level amplified by the flow), and the dt sequence
+    agrees to 1e-5 relative."""
+    _print_case("fp32 kh 64x64 50 steps")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(1, 40)[0]
+    n = 64
+    u0 = ic.kelvin_helmholtz_2d(n, X)
+    s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                     (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=2.0, max_steps=50)
+    rc, ref, t, steps, dts_ref = O.run(s, u0)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                             precision=hip.FP32)
+    with hip.Ensemble(scheme, [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), 2.0,
+                                               max_steps=50)]) as e:
+        e.record_dt(60)
+        e.upload(0, u0)
+        e.apply_boundary()
+        e.run(10)
+        got = e.download(0)
+        dts = e.dt_history(0)
+    g = 2
+    r = ref[0, 0, g:-g, g:-g]
+    q = got[0, 0, g:-g, g:-g]
+    l1 = np.abs(r - q).sum() / np.abs(r).sum()
+    print("fp32 rho rel L1 vs fp64 oracle: %.3e" % l1)
+    assert l1 < 1e-4
+    assert np.allclose(dts, dts_ref, rtol=1e-5, atol=0)
