@@ -44,6 +44,10 @@ EXPORTS = [
     "alsvinn_hip_ensemble_dt_history",
     "alsvinn_hip_ensemble_rhs",
     "alsvinn_hip_ensemble_last_elapsed_ms",
+    "alsvinn_hip_stats_meanvar",
+    "alsvinn_hip_stats_structure_cube",
+    "alsvinn_hip_stats_structure_basic",
+    "alsvinn_hip_ensemble_totals",
 ]
 
 
@@ -121,6 +125,12 @@ def load():
     L.alsvinn_hip_ensemble_dt_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_longlong]
     L.alsvinn_hip_ensemble_rhs.argtypes = [vp, ctypes.c_int, dp, dp]
     L.alsvinn_hip_ensemble_last_elapsed_ms.argtypes = [vp, dp]
+    L.alsvinn_hip_stats_meanvar.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, dp, dp]
+    L.alsvinn_hip_stats_structure_cube.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_double, dp]
+    L.alsvinn_hip_stats_structure_basic.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_double, ctypes.c_int, dp]
+    L.alsvinn_hip_ensemble_totals.argtypes = [vp, dp]
     _lib = L
     return L
 

@@ -21,6 +21,7 @@ SOURCES = [
     "kernels_euler1.hip",
     "kernels_euler2.hip",
     "kernels_euler3.hip",
+    "stats.hip",
 ]
 
 FLAGS = [

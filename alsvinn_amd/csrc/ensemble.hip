@@ -74,41 +74,6 @@ __global__ void count_active_kernel(const SampleState* __restrict__ state, int n
             return fail(ALSVINN_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-struct alsvinn_hip_ensemble {
-    alsvinn_hip_scheme scheme;
-    int nsamples = 0;
-    int nvar = 0;
-    int dim = 0;
-    int ghosts = 0;
-    size_t esize = 8;
-    std::vector<SampleDesc> desc;
-    std::vector<alsvinn_hip_sample> spec;
-    long long total = 0;  // elements per buffer
-    void* U = nullptr;
-    void* A = nullptr;
-    void* B = nullptr;
-    SampleDesc* d_desc = nullptr;
-    SampleState* d_state = nullptr;
-    unsigned long long* d_ws = nullptr;
-    WaveTask* d_tasks = nullptr;
-    int ntasks = 0;
-    int* d_count = nullptr;
-    int* h_count = nullptr;
-    double* d_dt_hist = nullptr;
-    long long dt_cap = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_steps = 0;
-    long long max_vstride = 0;
-    int variant = kDefaultVariant;  // stage kernel register budget
-    int band_rows = 0;  // rows per wave task; 0 = heuristic
-    bool tuned = false;
-    KernelSet<double> k64;
-    KernelSet<float> k32;
-};
 
 namespace {
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/alsvinn_hip.h"
 #include "ensemble_kernels.hpp"
@@ -85,3 +86,41 @@ int select_kernels(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
 }
 
 }  // namespace alsvinn_hip
+
+// The ensemble object behind the C ABI's opaque handle.
+struct alsvinn_hip_ensemble {
+    alsvinn_hip_scheme scheme;
+    int nsamples = 0;
+    int nvar = 0;
+    int dim = 0;
+    int ghosts = 0;
+    size_t esize = 8;
+    std::vector<alsvinn_hip::SampleDesc> desc;
+    std::vector<alsvinn_hip_sample> spec;
+    long long total = 0;  // elements per buffer
+    void* U = nullptr;
+    void* A = nullptr;
+    void* B = nullptr;
+    alsvinn_hip::SampleDesc* d_desc = nullptr;
+    alsvinn_hip::SampleState* d_state = nullptr;
+    unsigned long long* d_ws = nullptr;
+    alsvinn_hip::WaveTask* d_tasks = nullptr;
+    int ntasks = 0;
+    int* d_count = nullptr;
+    int* h_count = nullptr;
+    double* d_dt_hist = nullptr;
+    long long dt_cap = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    long long max_vstride = 0;
+    int variant = alsvinn_hip::kDefaultVariant;  // stage kernel register budget
+    int band_rows = 0;  // rows per wave task; 0 = heuristic
+    bool tuned = false;
+    alsvinn_hip::KernelSet<double> k64;
+    alsvinn_hip::KernelSet<float> k32;
+};
+

@@ -1,0 +1,385 @@
+// Sample statistics of the hip platform (alsuq::stats on the device) and the
+// FP64 conserved totals.
+//
+//   meanvar          MeanVariance.cpp:30-67 + StatisticsHelper.cpp:35-79: per
+//                    cell (ghosts included) S1 = sum u, S2 = sum pow(u,2) in
+//                    sample order; mean = S1/M, var = S2/M - pow(mean,2).
+//                    The Welford variant accumulates (mean, M2) in sample
+//                    order and finalises var = M2/M.
+//   structure_cube   structure_common.hpp:37-150 with ValueAtBoundary.hpp:20-98:
+//                    S_p(h) = sum over cells and the cube shell of radius h of
+//                    |u - u_h|^p / (nx*ny*nz), h = 1..H-1, summed over samples,
+//                    divided by M.
+//   structure_basic  StructureBasic.cpp:67-120: one direction, h = 0..H-1.
+// Accumulation is FP64 whatever the storage precision.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace alsvinn_hip;
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return fail(ALSVINN_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+namespace {
+
+template <class T>
+__global__ __launch_bounds__(256) void meanvar_kernel(const T* __restrict__ U,
+                                                     const long long* __restrict__ bases, int m,
+                                                     long long n, int welford,
+                                                     double* __restrict__ mean,
+                                                     double* __restrict__ var) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double a = 0, b = 0;
+        if (!welford) {
+            for (int k = 0; k < m; ++k) {
+                const double u = double(U[bases[k] + i]);
+                a += u;
+                b += u * u;
+            }
+            const double M = double(m);
+            const double mu = a / M;
+            mean[i] = mu;
+            var[i] = b / M - mu * mu;
+        } else {
+            for (int k = 0; k < m; ++k) {
+                const double u = double(U[bases[k] + i]);
+                const double d = u - a;
+                a += d / double(k + 1);
+                b += d * (u - a);
+            }
+            mean[i] = a;
+            var[i] = b / double(m);
+        }
+    }
+}
+
+// ValueAtBoundary<PERIODIC> (wrap) / <NEUMANN> (clamp) on one axis.
+__device__ __forceinline__ int wrap_index(int p, int N, int periodic) {
+    if (periodic) {
+        if (p < 0) p += N;
+        if (p >= N) p -= N;
+        if (p < 0 || p >= N) {
+            p %= N;
+            if (p < 0) p += N;
+        }
+        return p;
+    }
+    return p < 0 ? 0 : (p > N - 1 ? N - 1 : p);
+}
+
+__device__ __forceinline__ double pow_p(double x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    if (p == 3.0) return x * (x * x);
+    if (p == 4.0) return x * (x * (x * x));
+    if (p == 5.0) return x * (x * (x * (x * x)));
+    return pow(x, p);
+}
+
+struct StructGeom {
+    int nx, ny, nz, gx, gy, gz, tx, ty, dims, periodic;
+    long long vstride;
+};
+
+template <class T>
+__device__ __forceinline__ double value_at(const T* u, const StructGeom& g, int x, int y, int z) {
+    const int X = wrap_index(x, g.nx, g.periodic) + g.gx;
+    const int Y = wrap_index(y, g.ny, g.periodic) + g.gy;
+    const int Z = wrap_index(z, g.nz, g.periodic) + g.gz;
+    return double(u[((long long)Z * g.ty + Y) * g.tx + X]);
+}
+
+// Block-wide sum (256 threads); thread 0 gets the result.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// One block = 256 interior cells of one sample (blockIdx.z) and one variable
+// (blockIdx.y); writes partial[((k*nvar + var)*nblocks + block)*H + h].
+template <class T>
+__global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict__ U,
+                                                            const long long* __restrict__ bases,
+                                                            StructGeom g, int H, double p,
+                                                            int nvar, double* partial) {
+    __shared__ double sh[256];
+    const int k = blockIdx.z, var = blockIdx.y;
+    const long long ncell = (long long)g.nx * g.ny * g.nz;
+    const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = cell < ncell;
+    const int i = valid ? int(cell % g.nx) : 0;
+    const int j = valid ? int((cell / g.nx) % g.ny) : 0;
+    const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
+    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const double uc = value_at(u, g, i, j, l);
+    const int dims = g.dims;
+    for (int h = 1; h < H; ++h) {
+        double acc = 0;
+        if (valid) {
+            for (int d = 0; d < dims; d++) {
+                for (int side = 0; side < 2; side++) {
+                    const bool zD = d == 2, yD = d == 1, xD = d == 0;
+                    const int zs = zD ? (side == 0 ? l - h : l + h) : (dims > 2 ? l - h : 0);
+                    const int ze = zD ? zs + 1 : (dims > 2 ? l + h + 1 : 1);
+                    const int ys = yD ? (side == 0 ? j - h : j + h) : (dims >= 2 ? j - h + 1 : 0);
+                    const int ye = yD ? ys + 1 : (dims >= 2 ? j + h : 1);
+                    int xs = xD ? (side == 0 ? i - h : i + h) : i - h;
+                    int xe = xD ? xs + 1 : i + h + 1;
+                    if (zD) {
+                        xs += 1;
+                        xe -= 1;
+                    }
+                    for (int z = zs; z < ze; z++)
+                        for (int y = ys; y < ye; y++)
+                            for (int x = xs; x < xe; x++) {
+                                const double uh = value_at(u, g, x, y, z);
+                                acc += pow_p(fabs(uc - uh), p);
+                            }
+                }
+            }
+        }
+        const double tot = block_sum(acc, sh);
+        if (threadIdx.x == 0)
+            partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restrict__ U,
+                                                             const long long* __restrict__ bases,
+                                                             StructGeom g, int H, double p,
+                                                             int direction, int nvar,
+                                                             double* partial) {
+    __shared__ double sh[256];
+    const int k = blockIdx.z, var = blockIdx.y;
+    const long long ncell = (long long)g.nx * g.ny * g.nz;
+    const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = cell < ncell;
+    const int i = valid ? int(cell % g.nx) : 0;
+    const int j = valid ? int((cell / g.nx) % g.ny) : 0;
+    const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
+    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const double uc = value_at(u, g, i, j, l);
+    for (int h = 0; h < H; ++h) {
+        double acc = 0;
+        if (valid) {
+            const double uh = value_at(u, g, i + h * (direction == 0), j + h * (direction == 1),
+                                       l + h * (direction == 2));
+            acc = pow(fabs(uc - uh), p);
+        }
+        const double tot = block_sum(acc, sh);
+        if (threadIdx.x == 0)
+            partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+    }
+}
+
+// FP64 sum of every variable over the interior cells of each sample:
+// partial[(s*nvar + var)*gridDim.x + block].
+template <class T>
+__global__ __launch_bounds__(256) void totals_kernel(const T* __restrict__ U,
+                                                    const SampleDesc* __restrict__ desc, int nvar,
+                                                    double* partial) {
+    __shared__ double sh[256];
+    const int s = blockIdx.z, var = blockIdx.y;
+    const SampleDesc D = desc[s];
+    const long long ncell = (long long)D.nx * D.ny * D.nz;
+    double acc = 0;
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * 256) {
+        const int i = int(c % D.nx), j = int((c / D.nx) % D.ny), l = int(c / ((long long)D.nx * D.ny));
+        acc += double(U[D.base + var * D.vstride +
+                        ((long long)(l + D.gz) * D.ty + (j + D.gy)) * D.tx + (i + D.gx)]);
+    }
+    const double tot = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[((long long)s * nvar + var) * gridDim.x + blockIdx.x] = tot;
+}
+
+int group_bases(alsvinn_hip_ensemble* e, const int* samples, int m, std::vector<long long>& bases) {
+    if (!e || !samples || m < 1) return fail(ALSVINN_HIP_E_INVALID, "bad sample group");
+    const SampleDesc& D0 = e->desc[samples[0] >= 0 && samples[0] < e->nsamples ? samples[0] : 0];
+    for (int k = 0; k < m; ++k) {
+        const int s = samples[k];
+        if (s < 0 || s >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "sample index out of range");
+        const SampleDesc& D = e->desc[s];
+        if (D.nx != D0.nx || D.ny != D0.ny || D.nz != D0.nz)
+            return fail(ALSVINN_HIP_E_INVALID, "statistics need samples on identical grids");
+        bases.push_back(D.base);
+    }
+    return 0;
+}
+
+StructGeom geom_of(const alsvinn_hip_ensemble* e, const SampleDesc& D) {
+    StructGeom g;
+    g.nx = D.nx;
+    g.ny = D.ny;
+    g.nz = D.nz;
+    g.gx = D.gx;
+    g.gy = D.gy;
+    g.gz = D.gz;
+    g.tx = D.tx;
+    g.ty = D.ty;
+    g.dims = e->dim;
+    g.periodic = D.bc == 1;
+    g.vstride = D.vstride;
+    return g;
+}
+
+template <class Launch>
+int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, double* host_out,
+                  Launch launch) {
+    std::vector<long long> bases;
+    if (int rc = group_bases(e, samples, m, bases)) return rc;
+    const SampleDesc& D = e->desc[samples[0]];
+    const long long ncell = (long long)D.nx * D.ny * D.nz;
+    const int nblocks = int((ncell + 255) / 256);
+    long long* d_bases = nullptr;
+    double* d_part = nullptr;
+    const size_t npart = (size_t)m * e->nvar * nblocks * H;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
+    HIP_TRY(hipMalloc(&d_part, sizeof(double) * npart));
+    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d_part, 0, sizeof(double) * npart));
+    launch(dim3(nblocks, e->nvar, m), d_bases, geom_of(e, D), d_part);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    if (le == hipSuccess) le = se;
+    std::vector<double> part(npart);
+    hipError_t ce = hipMemcpy(part.data(), d_part, sizeof(double) * npart, hipMemcpyDeviceToHost);
+    hipFree(d_bases);
+    hipFree(d_part);
+    if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
+    if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
+    // fixed-order reduction: samples in list order, blocks in index order
+    const double N = double(ncell);
+    for (int var = 0; var < e->nvar; ++var)
+        for (int h = 0; h < H; ++h) {
+            double acc = 0;
+            for (int k = 0; k < m; ++k)
+                for (int b = 0; b < nblocks; ++b)
+                    acc += part[(((size_t)k * e->nvar + var) * nblocks + b) * H + h];
+            host_out[(size_t)var * H + h] = (acc / N) / double(m);
+        }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alsvinn_hip_stats_meanvar(alsvinn_hip_ensemble* e, const int* samples, int m, int welford,
+                              double* host_mean, double* host_var) {
+    std::vector<long long> bases;
+    if (int rc = group_bases(e, samples, m, bases)) return rc;
+    if (!host_mean || !host_var) return fail(ALSVINN_HIP_E_INVALID, "output is NULL");
+    const SampleDesc& D = e->desc[samples[0]];
+    const long long n = D.vstride * e->nvar;
+    long long* d_bases = nullptr;
+    double *d_mean = nullptr, *d_var = nullptr;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
+    HIP_TRY(hipMalloc(&d_mean, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&d_var, sizeof(double) * n));
+    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+    if (e->esize == 8)
+        hipLaunchKernelGGL(meanvar_kernel<double>, dim3(blocks), dim3(256), 0, e->stream,
+                           (const double*)e->U, d_bases, m, n, welford, d_mean, d_var);
+    else
+        hipLaunchKernelGGL(meanvar_kernel<float>, dim3(blocks), dim3(256), 0, e->stream,
+                           (const float*)e->U, d_bases, m, n, welford, d_mean, d_var);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    hipError_t c1 = hipMemcpy(host_mean, d_mean, sizeof(double) * n, hipMemcpyDeviceToHost);
+    hipError_t c2 = hipMemcpy(host_var, d_var, sizeof(double) * n, hipMemcpyDeviceToHost);
+    hipFree(d_bases);
+    hipFree(d_mean);
+    hipFree(d_var);
+    for (hipError_t x : {le, se, c1, c2})
+        if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
+    return 0;
+}
+
+int alsvinn_hip_stats_structure_cube(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                     int number_of_h, double p, double* host_out) {
+    if (!e || number_of_h < 1 || !host_out) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    return run_structure(e, samples, m, number_of_h, host_out,
+                         [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
+                             if (e->esize == 8)
+                                 hipLaunchKernelGGL(structure_cube_kernel<double>, grid, dim3(256), 0,
+                                                    e->stream, (const double*)e->U, bases, g,
+                                                    number_of_h, p, e->nvar, part);
+                             else
+                                 hipLaunchKernelGGL(structure_cube_kernel<float>, grid, dim3(256), 0,
+                                                    e->stream, (const float*)e->U, bases, g,
+                                                    number_of_h, p, e->nvar, part);
+                         });
+}
+
+int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                      int number_of_h, double p, int direction,
+                                      double* host_out) {
+    if (!e || number_of_h < 1 || !host_out || direction < 0 || direction >= e->dim)
+        return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    return run_structure(e, samples, m, number_of_h, host_out,
+                         [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
+                             if (e->esize == 8)
+                                 hipLaunchKernelGGL(structure_basic_kernel<double>, grid, dim3(256),
+                                                    0, e->stream, (const double*)e->U, bases, g,
+                                                    number_of_h, p, direction, e->nvar, part);
+                             else
+                                 hipLaunchKernelGGL(structure_basic_kernel<float>, grid, dim3(256),
+                                                    0, e->stream, (const float*)e->U, bases, g,
+                                                    number_of_h, p, direction, e->nvar, part);
+                         });
+}
+
+int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host) {
+    if (!e || !host) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const int blocks = 64;
+    const size_t np = (size_t)e->nsamples * e->nvar * blocks;
+    double* d_part = nullptr;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMalloc(&d_part, sizeof(double) * np));
+    if (e->esize == 8)
+        hipLaunchKernelGGL(totals_kernel<double>, dim3(blocks, e->nvar, e->nsamples), dim3(256), 0,
+                           e->stream, (const double*)e->U, e->d_desc, e->nvar, d_part);
+    else
+        hipLaunchKernelGGL(totals_kernel<float>, dim3(blocks, e->nvar, e->nsamples), dim3(256), 0,
+                           e->stream, (const float*)e->U, e->d_desc, e->nvar, d_part);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    if (le == hipSuccess) le = se;
+    std::vector<double> part(np);
+    hipError_t ce = hipMemcpy(part.data(), d_part, sizeof(double) * np, hipMemcpyDeviceToHost);
+    hipFree(d_part);
+    if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
+    if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
+    for (int s = 0; s < e->nsamples; ++s)
+        for (int v = 0; v < e->nvar; ++v) {
+            double acc = 0;
+            for (int b = 0; b < blocks; ++b) acc += part[((size_t)s * e->nvar + v) * blocks + b];
+            host[(size_t)s * e->nvar + v] = acc;
+        }
+    return 0;
+}
+
+}  // extern "C"

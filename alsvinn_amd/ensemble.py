@@ -168,3 +168,38 @@ class Ensemble:
         ms = ctypes.c_double()
         check(_lib.load().alsvinn_hip_ensemble_last_elapsed_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    # ---- statistics (alsuq::stats on the device)
+
+    def _group(self, samples):
+        if samples is None:
+            samples = range(len(self.samples))
+        arr = np.ascontiguousarray(list(samples), dtype=np.int32)
+        return arr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(arr)
+
+    def meanvar(self, samples=None, welford=False):
+        arr, ptr, m = self._group(samples)
+        shape = self.volume_shape(int(arr[0]))
+        mean = np.empty(shape)
+        var = np.empty(shape)
+        check(_lib.load().alsvinn_hip_stats_meanvar(self.h, ptr, m, int(welford), _dp(mean),
+                                                    _dp(var)))
+        return mean, var
+
+    def structure_cube(self, number_of_h, p=2.0, samples=None):
+        arr, ptr, m = self._group(samples)
+        out = np.zeros((self.nvar, number_of_h))
+        check(_lib.load().alsvinn_hip_stats_structure_cube(self.h, ptr, m, number_of_h, p, _dp(out)))
+        return out
+
+    def structure_basic(self, number_of_h, direction=0, p=2.0, samples=None):
+        arr, ptr, m = self._group(samples)
+        out = np.zeros((self.nvar, number_of_h))
+        check(_lib.load().alsvinn_hip_stats_structure_basic(self.h, ptr, m, number_of_h, p,
+                                                            direction, _dp(out)))
+        return out
+
+    def totals(self):
+        out = np.zeros((len(self.samples), self.nvar))
+        check(_lib.load().alsvinn_hip_ensemble_totals(self.h, _dp(out)))
+        return out

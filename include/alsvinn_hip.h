@@ -171,6 +171,34 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
  * call, from HIP events on the ensemble's stream (milliseconds). */
 int alsvinn_hip_ensemble_last_elapsed_ms(alsvinn_hip_ensemble* e, double* ms);
 
+/* ---- alsuq statistics on the device (accumulated in FP64) ----
+ * Each call takes a group of sample indices whose grids are identical and
+ * reads their current states (the states at the save time when called after
+ * alsvinn_hip_ensemble_run with one save at the end time). Synchronous. */
+
+/* meanvar (alsuq/src/stats/MeanVariance.cpp:30-67, StatisticsHelper.cpp:35-79):
+ * full padded volumes (nvar x tz x ty x tx). welford = 0 reproduces the
+ * reference (sum u and pow(u,2) in sample order; var = S2/M - mean^2);
+ * welford = 1 accumulates (mean, M2) in sample order and returns M2/M. */
+int alsvinn_hip_stats_meanvar(alsvinn_hip_ensemble* e, const int* samples, int m, int welford,
+                              double* host_mean, double* host_var);
+
+/* structure_cube (alsuq/src/stats/StructureCube.cpp:37-76,
+ * alsfvm/include/alsfvm/functional/structure_common.hpp:37-150): out[var*H + h]
+ * for h = 1..H-1 (out[var*H] = 0), combined over the group (divided by m). */
+int alsvinn_hip_stats_structure_cube(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                     int number_of_h, double p, double* host_out);
+
+/* structure_basic (alsuq/src/stats/StructureBasic.cpp:67-120): one direction,
+ * h = 0..H-1, out[var*H + h], divided by m. */
+int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                      int number_of_h, double p, int direction,
+                                      double* host_out);
+
+/* FP64 sum of every conserved variable over each sample's interior cells:
+ * host[sample*nvar + var]. */
+int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,150 @@
+"""Statistics, mixed-size ensembles and full-size invariants on the GPU.
+
+meanvar (reference formula) is compared bit for bit with the oracle's
+sequential sample loop; the Welford variant and the structure functions
+(parallel reductions) within a stated relative tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL_REDUCTION = 1e-12  # parallel vs sequential summation of O(1e5) terms
+
+
+def _case(name):
+    print("CASE", name, flush=True)
+    sys.stdout.flush()
+
+
+def _kh_ensemble(hip, sizes, T=0.05, seed_start=0):
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(len(sizes), 40, sample_start=seed_start)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    samples = [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), T) for n in sizes]
+    u0 = [ic.kelvin_helmholtz_2d(n, X[i]) for i, n in enumerate(sizes)]
+    return scheme, samples, u0
+
+
+def _oracle_final(n, u0, T):
+    s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                     (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=T)
+    rc, out, t, steps, _ = O.run(s, u0)
+    assert rc == 0 and t == T
+    return out, steps
+
+
+def test_mixed_size_ensemble_and_meanvar_bit_exact(hip):
+    """An ensemble mixing 16^2, 24^2 and 40^2 samples: every sample equals its
+    own oracle run, and meanvar per size group equals the oracle's sequential
+    accumulation (MeanVariance.cpp:30-67) bit for bit."""
+    _case("mixed sizes 16/24/40 x 3")
+    sizes = [16, 24, 40, 16, 24, 40, 16, 24, 40]
+    T = 0.05
+    scheme, samples, u0 = _kh_ensemble(hip, sizes, T)
+    with hip.Ensemble(scheme, samples) as e:
+        for i, u in enumerate(u0):
+            e.upload(i, u)
+        e.apply_boundary()
+        e.run(8)
+        finals = [e.download(i) for i in range(len(sizes))]
+        infos = [e.info(i) for i in range(len(sizes))]
+        stats = {n: e.meanvar([i for i, m in enumerate(sizes) if m == n]) for n in (16, 24, 40)}
+        welf = {n: e.meanvar([i for i, m in enumerate(sizes) if m == n], welford=True)
+                for n in (16, 24, 40)}
+    refs = []
+    for i, n in enumerate(sizes):
+        out, steps = _oracle_final(n, u0[i], T)
+        assert infos[i]["steps"] == steps and infos[i]["t"] == T
+        assert np.array_equal(out.view(np.uint64), finals[i].view(np.uint64)), "sample %d" % i
+        refs.append(out)
+    for n in (16, 24, 40):
+        group = [refs[i] for i, m in enumerate(sizes) if m == n]
+        mean_ref, var_ref = O.meanvar(np.stack(group))
+        mean, var = stats[n]
+        assert np.array_equal(mean.view(np.uint64), mean_ref.view(np.uint64))
+        assert np.array_equal(var.view(np.uint64), var_ref.view(np.uint64))
+        wm, wv = welf[n]
+        np.testing.assert_allclose(wm, mean_ref, rtol=1e-13, atol=1e-15)
+        scale = np.maximum(np.abs(mean_ref) ** 2, 1e-30)
+        assert np.all(np.abs(wv - var_ref) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("bc", [O.BC_PERIODIC, O.BC_NEUMANN])
+def test_structure_functions_match_oracle(hip, bc):
+    """structure_cube (h = 1..H-1) and structure_basic (h = 0..H-1, x and y)
+    over 4 samples against the oracle's per-sample loops, divided by M."""
+    _case("structure functions bc=%d" % bc)
+    from alsvinn_amd import initial_data as ic
+    rng = np.random.default_rng(2)
+    n, H, M = 20, 7, 4
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, bc)
+    samples = [hip.make_sample((n, n + 4, 1), (0, 0, 0), (1, 1, 0), 1.0) for _ in range(M)]
+    vols = []
+    with hip.Ensemble(scheme, samples) as e:
+        shape = e.volume_shape(0)
+        for i in range(M):
+            u = rng.random(shape) + 1.0
+            e.upload(i, u)
+        e.apply_boundary()
+        vols = [e.download(i) for i in range(M)]
+        cube = e.structure_cube(H, 2.0)
+        cube3 = e.structure_cube(H, 3.0)
+        basic_x = e.structure_basic(H, 0, 2.0)
+        basic_y = e.structure_basic(H, 1, 1.0)
+    per = lambda f: sum(f(v) for v in vols) / M
+    ref_cube = per(lambda v: O.structure_cube(v, (n, n + 4, 1), 2, H, 2.0, bc == O.BC_PERIODIC))
+    ref_cube3 = per(lambda v: O.structure_cube(v, (n, n + 4, 1), 2, H, 3.0, bc == O.BC_PERIODIC))
+    ref_bx = per(lambda v: O.structure_basic(v, (n, n + 4, 1), 2, H, 2.0, 0, bc == O.BC_PERIODIC))
+    ref_by = per(lambda v: O.structure_basic(v, (n, n + 4, 1), 2, H, 1.0, 1, bc == O.BC_PERIODIC))
+    np.testing.assert_allclose(cube, ref_cube, rtol=RTOL_REDUCTION, atol=0)
+    np.testing.assert_allclose(cube3, ref_cube3, rtol=RTOL_REDUCTION, atol=0)
+    np.testing.assert_allclose(basic_x, ref_bx, rtol=RTOL_REDUCTION, atol=0)
+    np.testing.assert_allclose(basic_y, ref_by, rtol=RTOL_REDUCTION, atol=0)
+
+
+def test_structure_cube_linear_closed_form(hip):
+    """StructureTestBoundary.cpp:255-290 on the GPU: u_i = i in 1D."""
+    _case("structure linear 1d")
+    N, H = 16, 8
+    scheme = hip.make_scheme(O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_ENO2, O.INT_RK2, O.BC_PERIODIC)
+    with hip.Ensemble(scheme, [hip.make_sample((N, 1, 1), (0, 0, 0), (1, 0, 0), 1.0)]) as e:
+        u = np.zeros(e.volume_shape(0))
+        u[0, 0, 0, 2:-2] = np.arange(N)
+        e.upload(0, u)
+        out = e.structure_cube(H, 2.0)
+    for h in range(1, H):
+        tot = 0.0
+        for i in range(N):
+            for j in (-h, h):
+                k = i + j
+                tot += float((i - (k % N)) ** 2)
+        assert out[0, h] == pytest.approx(tot / N, rel=1e-15)
+
+
+def test_full_size_kh_conserves_totals(hip):
+    """Full configuration size (256^2 x 64 samples, periodic): the FP64
+    conserved totals of every sample are unchanged after 20 steps up to
+    roundoff (relative 1e-12), and no sample leaves the constraint set."""
+    _case("full size 256^2 x 64 totals")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    M, n = 64, 256
+    X = sampling.stl_mersenne_uniform(M, 40)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    with hip.Ensemble(scheme, [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), 2.0,
+                                               max_steps=20) for _ in range(M)]) as e:
+        for i in range(M):
+            e.upload(i, ic.kelvin_helmholtz_2d(n, X[i]))
+        e.apply_boundary()
+        t0 = e.totals()
+        e.run(10)
+        t1 = e.totals()
+        assert all(e.info(i)["steps"] == 20 and e.info(i)["status"] == 0 for i in (0, M - 1))
+    scale = np.abs(t0).max(axis=0) + 1.0
+    assert np.all(np.abs(t1 - t0) <= 1e-12 * scale * n * n / 1000)

@@ -1,0 +1,107 @@
+"""Host-side logic of the hip platform and its C ABI, on the CPU (no GPU call
+is made): the library loads and exports every function the header declares,
+argument validation fails with the reference's error behaviour, the sampler
+and the initial data are bit-identical to the oracle's."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "alsvinn_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(alsvinn_hip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    import alsvinn_amd
+    L = alsvinn_amd._lib.load()
+    names = _header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(alsvinn_amd._lib.EXPORTS) == names
+
+
+def test_ghost_cells_and_validation():
+    """NumericalFluxCPU.cpp:226-231: ghosts = max(stencil-1, reconstruction);
+    unknown names are errors (NumericalFluxFactory.cpp:194-197 THROWs)."""
+    import alsvinn_amd as hip
+    L = hip._lib.load()
+    s = hip.make_scheme(hip.EQ_EULER2, hip.FLUX_HLL3, hip.RECON_WENO2, hip.INT_RK3, hip.BC_PERIODIC)
+    assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == 2
+    s.reconstruction = hip.RECON_NONE
+    assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == 1
+    s.reconstruction = 7
+    assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == -1
+    assert b"unknown reconstruction" in L.alsvinn_hip_last_error()
+    bad = hip.make_scheme(hip.EQ_BURGERS, hip.FLUX_HLL, hip.RECON_NONE, hip.INT_FE, hip.BC_NEUMANN)
+    with pytest.raises(hip.HipError, match="godunov"):
+        hip.Ensemble(bad, [hip.make_sample((8, 1, 1), (0, 0, 0), (1, 0, 0), 1.0)])
+
+
+def test_cfl_auto_rule():
+    """SimulatorSetup.cpp:306-320: cfl auto = 0.9 without reconstruction, else 0.475."""
+    import alsvinn_amd as hip
+    assert hip.make_scheme(1, 0, hip.RECON_NONE, 0, 0).cfl == 0.9
+    assert hip.make_scheme(2, 1, hip.RECON_WENO2, 2, 1).cfl == 0.475
+
+
+def test_product_sampler_matches_libstdcxx():
+    """The product's std::mt19937_64 + uniform_real_distribution restatement
+    equals libstdc++ (through the oracle) bit for bit, including the
+    skip-ahead of STLMersenne and the Uniform scaling."""
+    from alsvinn_amd import sampling
+    for (n, d, lo, hi, start) in [(4, 40, 0.0, 1.0, 0), (3, 7, -2.0, 3.0, 5), (2, 400, 0.0, 1.0, 1)]:
+        a = sampling.stl_mersenne_uniform(n, d, lo, hi, sample_start=start)
+        b = O.stl_mersenne_uniform(n, d, lo, hi, sample_start=start)
+        assert np.array_equal(a, b)
+    e = sampling.MT19937_64()
+    e.discard(9999)
+    assert int(e.raw(1)[0]) == 9981545732273789042
+
+
+def test_seeded_sampler_is_per_sample_engine():
+    from alsvinn_amd import sampling
+    a = sampling.seeded_uniform(100, 3, 5)
+    assert np.array_equal(a[1], sampling.seeded_uniform(101, 1, 5)[0])
+    assert not np.array_equal(a[0], a[1])
+
+
+def test_kh_initial_data_band_and_energy():
+    """KH: rho in {1, 2}, u = +-0.5, v = 0, E = 2.5/(gamma-1) + 0.5*rho*0.25; the
+    band edges are 0.25 + s0(x), 0.75 + s1(x) with |s| <= 0.01."""
+    from alsvinn_amd import initial_data as ic, sampling
+    X = sampling.stl_mersenne_uniform(1, 40)[0]
+    u = ic.kelvin_helmholtz_2d(64, X)
+    rho = u[0, 0, 2:-2, 2:-2]
+    assert set(np.unique(rho)) == {1.0, 2.0}
+    y = ic.midpoints(64, 0, 1)
+    assert np.all(rho[y < 0.24] == 1.0) and np.all(rho[(y > 0.26) & (y < 0.74)] == 2.0)
+    np.testing.assert_array_equal(u[1, 0, 2:-2, 2:-2], np.where(rho == 2.0, -1.0, 0.5))
+    assert np.all(u[2, 0, 2:-2, 2:-2] == 0.0)
+    np.testing.assert_array_equal(u[3, 0, 2:-2, 2:-2], 2.5 / (1.4 - 1) + (0.5 * rho) * 0.25)
+
+
+def test_riemann_and_fbm_initial_data():
+    from alsvinn_amd import initial_data as ic
+    prm = ic.riemann_parameters(np.linspace(0, 1, 16))
+    assert np.all((prm[:, 0] >= 0.1) & (prm[:, 0] <= 1.5)) and np.all(np.abs(prm[:, 1:3]) <= 1)
+    u = ic.riemann_four_quadrant(32, prm)
+    assert u[0, 0, 2, 2] == prm[0, 0] and u[0, 0, -3, -3] == prm[3, 0]
+    b = ic.fbm_rmd(4096, 0.5, seed=3)
+    assert b.shape == (4096,) and np.isfinite(b).all()
+    # Hurst scaling: increments over lag k have variance ~ k^(2H); H = 0.5 ->
+    # ratio 4 between lags 64 and 16 (sampling noise allowed)
+    for H, lo, hi in ((0.5, 2.5, 6.5), (0.75, 4.5, 13.0)):
+        b = np.mean([np.var(ic.fbm_rmd(4096, H, seed=s)[64:] - ic.fbm_rmd(4096, H, seed=s)[:-64])
+                     / np.var(ic.fbm_rmd(4096, H, seed=s)[16:] - ic.fbm_rmd(4096, H, seed=s)[:-16])
+                     for s in range(8)])
+        assert lo < b < hi, (H, b)

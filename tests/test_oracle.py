@@ -1,0 +1,463 @@
+"""Pins the CPU oracle (no GPU): the reference's own known-answer tests,
+closed forms, conservation/symmetry invariants, convergence rates and an
+exact Riemann solution. Each test names the reference test or the
+mathematics that fixes the expected value.
+"""
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+GAMMA = 1.4
+
+
+def euler_state(rho, vel, p, gamma=GAMMA):
+    vel = list(vel)
+    dot = sum(v * v for v in vel)
+    return np.array([rho] + [rho * v for v in vel] + [p / (gamma - 1) + 0.5 * rho * dot])
+
+
+# ---------------------------------------------------------------- sampler
+
+def test_mt19937_64_known_answer():
+    """C++ standard [rand.predef]: the 10000th invocation of a
+    default-constructed mt19937_64 produces 9981545732273789042."""
+    assert O.mt19937_64_nth(10000) == 9981545732273789042
+
+
+def test_sampler_sample_major_draw_order():
+    """STLMersenne.cpp:38-85: component c of sample s is the (s*D+c)-th draw;
+    a run starting at sample_start skips forward to exactly that draw."""
+    full = O.stl_mersenne_uniform(6, 5)
+    tail = O.stl_mersenne_uniform(3, 5, sample_start=3)
+    assert np.array_equal(full[3:], tail)
+    flat = O.stl_mersenne_uniform(1, 30)[0]
+    assert np.array_equal(full.ravel(), flat)
+    scaled = O.stl_mersenne_uniform(6, 5, lower=-2.0, upper=3.0)
+    assert np.array_equal(scaled, full * 5.0 + (-2.0))
+
+
+def test_sampler_moments_converge():
+    """MeanVarTest.cpp:85-149: sample mean/variance of U[0,1] draws converge
+    to 1/2 and 1/12 at rate ~ M^-1/2."""
+    x = O.stl_mersenne_uniform(1, 200000)[0]
+    assert abs(x.mean() - 0.5) < 5 / np.sqrt(12 * len(x))
+    assert abs(x.var() - 1 / 12) < 5e-3
+    assert x.min() >= 0 and x.max() < 1
+
+
+# ---------------------------------------------------------------- fluxes
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+@pytest.mark.parametrize("flux", [O.FLUX_HLL, O.FLUX_HLL3])
+def test_flux_consistency(nsd, flux):
+    """HLLTest.cpp:24-71: F(U, U) equals the point flux f(U) exactly, in
+    every direction (for states where the point-flux branch is taken and for
+    subsonic states where the HLL average of identical states reduces to f)."""
+    states = [euler_state(1.0, [2.0, 3.0, 4.0][:nsd], 6.0),
+              euler_state(0.7, [0.1, -0.2, 0.05][:nsd], 1.3),
+              euler_state(2.0, [-3.5, 0.5, 1.0][:nsd], 0.4)]
+    for q in states:
+        for d in range(nsd):
+            F, ws = O.face_flux(nsd, flux, d, GAMMA, q, q)
+            f = O.point_flux(nsd, d, GAMMA, q)
+            u = q[1 + d] / q[0]
+            p = (GAMMA - 1) * (q[-1] - 0.5 * np.dot(q[1:-1], q[1:-1]) / q[0])
+            c = np.sqrt(GAMMA * p / q[0])
+            np.testing.assert_allclose(F, f, rtol=1e-13, atol=1e-13)
+            assert ws == pytest.approx(abs(u) + c, rel=1e-14)
+
+
+def test_godunov_closed_form():
+    """burgers/Godunov.hpp:40-60: F = max(f(max(uL,0)), f(min(uR,0))) with
+    f(u) = u^2/2; wave speed max(|uL|, |uR|)."""
+    for uL, uR in [(1.0, 2.0), (-1.0, 0.5), (2.0, -3.0), (-0.5, -0.25), (0.0, 0.0)]:
+        F, ws = O.face_flux(O.EQ_BURGERS, O.FLUX_GODUNOV, 0, GAMMA, [uL], [uR])
+        expect = max(max(uL, 0) ** 2 / 2, min(uR, 0) ** 2 / 2)
+        assert F[0] == expect
+        assert ws == max(abs(uL), abs(uR))
+
+
+def test_hll_supersonic_upwinding():
+    """For a supersonic flow to the right (sL > 0) HLL/HLL3 return f(left);
+    to the left (sR < 0) f(right)."""
+    L = euler_state(1.0, [5.0], 1.0)
+    R = euler_state(0.5, [4.0], 0.8)
+    for flux in (O.FLUX_HLL, O.FLUX_HLL3):
+        F, _ = O.face_flux(1, flux, 0, GAMMA, L, R)
+        np.testing.assert_array_equal(F, O.point_flux(1, 0, GAMMA, L))
+        L2 = euler_state(1.0, [-5.0], 1.0)
+        R2 = euler_state(0.5, [-6.0], 0.8)
+        F2, _ = O.face_flux(1, flux, 0, GAMMA, L2, R2)
+        np.testing.assert_array_equal(F2, O.point_flux(1, 0, GAMMA, R2))
+
+
+def test_hll_textbook_formula():
+    """Subsonic HLL flux equals the Harten-Lax-van Leer formula (Toro 10.21)
+    with the reference's Roe-type wave-speed estimates, evaluated in
+    extended precision."""
+    L = euler_state(1.0, [0.3], 1.0)
+    R = euler_state(0.125, [-0.1], 0.1)
+    F, ws = O.face_flux(1, O.FLUX_HLL, 0, GAMMA, L, R)
+    ld = np.longdouble
+    rl, rr = ld(L[0]), ld(R[0])
+    ul, ur = ld(L[1]) / rl, ld(R[1]) / rr
+    pl = (ld(GAMMA) - 1) * (ld(L[2]) - ld(0.5) * ld(L[1]) ** 2 / rl)
+    pr = (ld(GAMMA) - 1) * (ld(R[2]) - ld(0.5) * ld(R[1]) ** 2 / rr)
+    wl, wr = np.sqrt(rl), np.sqrt(rr)
+    ub = (wl * ul + wr * ur) / (wl + wr)
+    pb = (wl * pl + wr * pr) / (wl + wr)
+    cb = np.sqrt(ld(GAMMA) * pb / ((rl + rr) / 2))
+    sl = min(ul - np.sqrt(ld(GAMMA) * pl / rl), ub - cb)
+    sr = max(ur + np.sqrt(ld(GAMMA) * pr / rr), ub + cb)
+    fl = np.array([rl * ul, rl * ul * ul + pl, (ld(L[2]) + pl) * ul])
+    fr = np.array([rr * ur, rr * ur * ur + pr, (ld(R[2]) + pr) * ur])
+    expect = (sr * fl - sl * fr + sr * sl * (R.astype(ld) - L.astype(ld))) / (sr - sl)
+    np.testing.assert_allclose(F, expect.astype(float), rtol=1e-13)
+    assert ws == pytest.approx(float(max(abs(sl), abs(sr))), rel=1e-14)
+
+
+# ---------------------------------------------------------------- reconstruction
+
+def _line(n, rho_overrides):
+    q = np.zeros((5, n))
+    q[0] = 1.0
+    q[1:4] = 1.0
+    q[4] = 10.0
+    for i, v in rho_overrides.items():
+        q[0, i] = v
+    return q
+
+
+def test_weno2_known_answer():
+    """WenoTest.cpp:228-299: rho = {.,2,0,1,.} around cell 2 gives
+    right = w0*0.5 + w1*(-1), left = w0~*(-0.5) + w1~*1 with
+    w ~ d/(beta+eps)^2, d = (2/3, 1/3), beta = (1, 4) (tolerance 1e-8 as there)."""
+    q = _line(14, {3: 2.0, 4: 0.0, 5: 1.0})  # 2 ghost cells: interior index 2 -> 4
+    left, right = O.reconstruct_1d(O.RECON_WENO2, q)
+    eps = 1e-8
+    d0, d1 = 2 / 3, 1 / 3
+    b0, b1 = 1.0, 4.0
+    a0, a1 = d0 / (b0 + eps) ** 2, d1 / (b1 + eps) ** 2
+    at0, at1 = d1 / (b0 + eps) ** 2, d0 / (b1 + eps) ** 2
+    assert right[0, 4] == pytest.approx((a0 * 0.5 + a1 * -1) / (a0 + a1), abs=1e-8)
+    assert left[0, 4] == pytest.approx((at0 * -0.5 + at1 * 1) / (at0 + at1), abs=1e-8)
+
+
+def test_eno2_known_answer():
+    """EnoTest.cpp:218-252: the same pattern gives right = 1/2, left = -1/2 exactly."""
+    q = _line(14, {3: 2.0, 4: 0.0, 5: 1.0})
+    left, right = O.reconstruct_1d(O.RECON_ENO2, q)
+    assert right[0, 4] == 0.5
+    assert left[0, 4] == -0.5
+
+
+@pytest.mark.parametrize("recon", [O.RECON_NONE, O.RECON_WENO2, O.RECON_ENO2])
+def test_reconstruction_of_constant(recon):
+    """A constant state is reproduced (to a few ulp: the WENO2 weights are
+    not exact fractions)."""
+    q = np.full((3, 20), 1.7)
+    left, right = O.reconstruct_1d(recon, q)
+    g = 1 if recon == O.RECON_NONE else 2
+    sl = slice(g - 1, 20 - (g - 1))
+    np.testing.assert_allclose(left[:, sl], 1.7, rtol=4e-16)
+    np.testing.assert_allclose(right[:, sl], 1.7, rtol=4e-16)
+
+
+@pytest.mark.parametrize("recon,rate", [(O.RECON_WENO2, 1.9), (O.RECON_ENO2, 1.9),
+                                        (O.RECON_NONE, 0.9)])
+def test_reconstruction_convergence(recon, rate):
+    """ReconstructionConvergenceTest.cpp:288-300: face values of cell averages
+    of a smooth function converge at rate >= 1.9 (weno2, eno2), 0.9 (none)."""
+    errs = []
+    Ns = [64, 128, 256, 512]
+    for N in Ns:
+        g = 1 if recon == O.RECON_NONE else 2
+        h = 1.0 / N
+        x0 = (np.arange(N + 2 * g) - g) * h
+        # cell averages of sin(2 pi x) + 0.5 sin(4 pi x) + 2 (positive)
+        F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi) - 0.25 * np.cos(4 * np.pi * x) / (2 * np.pi) + 2 * x
+        avg = (F(x0 + h) - F(x0)) / h
+        f = lambda x: np.sin(2 * np.pi * x) + 0.5 * np.sin(4 * np.pi * x) + 2
+        q = np.vstack([avg, avg])
+        left, right = O.reconstruct_1d(recon, q)
+        sl = slice(g, g + N)
+        err = np.mean(np.abs(left[0, sl] - f(x0[sl]))) + np.mean(np.abs(right[0, sl] - f(x0[sl] + h)))
+        errs.append(err)
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] >= rate, rates
+
+
+# ---------------------------------------------------------------- operator
+
+@pytest.mark.parametrize("eq,flux,recon,n", [
+    (O.EQ_EULER1, O.FLUX_HLL, O.RECON_NONE, (24, 1, 1)),
+    (O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, (16, 12, 1)),
+    (O.EQ_EULER3, O.FLUX_HLL3, O.RECON_ENO2, (8, 7, 6)),
+    (O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_ENO2, (30, 1, 1)),
+])
+def test_net_flux_zero_for_constant_state(eq, flux, recon, n):
+    """numericalfluxtest.cpp:65-126: the divergence of a constant state is zero."""
+    s = O.make_setup(eq, flux, recon, O.INT_FE, O.BC_PERIODIC, n, (0, 0, 0), (1, 1, 1))
+    shape = O.volume_shape(s)
+    u = np.zeros(shape)
+    if eq == O.EQ_BURGERS:
+        u[0] = 0.7
+    else:
+        st = euler_state(1.3, [0.2, -0.1, 0.3][:eq], 2.0)
+        for k in range(shape[0]):
+            u[k] = st[k]
+    out, ws = O.system_rhs(s, u)
+    g = O.ghosts(s)
+    sl = tuple([slice(None)] + [slice(g, -g) if m > 1 else slice(None) for m in (n[2], n[1], n[0])])
+    assert np.all(out[sl] == 0.0)
+    assert ws[0] > 0
+
+
+def _fill_distinct(s):
+    shape = O.volume_shape(s)
+    u = np.full(shape, np.nan)
+    g = O.ghosts(s)
+    nx, ny, nz = s.nx, s.ny, s.nz
+    gy = g if ny > 1 else 0
+    gz = g if nz > 1 else 0
+    vals = np.arange(shape[0] * nx * ny * nz, dtype=float).reshape(shape[0], nz, ny, nx) + 1
+    u[:, gz:gz + nz, gy:gy + ny, g:g + nx] = vals
+    return u, vals, (g, gy, gz)
+
+
+@pytest.mark.parametrize("bc", [O.BC_NEUMANN, O.BC_PERIODIC])
+@pytest.mark.parametrize("n,recon", [((9, 1, 1), O.RECON_WENO2), ((7, 6, 1), O.RECON_WENO2),
+                                     ((5, 4, 3), O.RECON_NONE)])
+def test_boundary_ghost_values(bc, n, recon):
+    """BoundaryTest.cpp: periodic ghost = interior at (p-G) mod N; Neumann
+    ghost = mirror image across the boundary face; corners compose per axis."""
+    s = O.make_setup(O.EQ_BURGERS, O.FLUX_GODUNOV, recon, O.INT_FE, bc, n, (0, 0, 0), (1, 1, 1))
+    u, vals, gs = _fill_distinct(s)
+    out = O.apply_boundary(s, u)
+
+    def src(p, N, G):
+        if G == 0:
+            return 0
+        if bc == O.BC_PERIODIC:
+            return (p - G) % N
+        i = p - G
+        if i < 0:
+            return -i - 1
+        if i >= N:
+            return 2 * N - 1 - i
+        return i
+
+    g, gy, gz = gs
+    for z in range(out.shape[1]):
+        for y in range(out.shape[2]):
+            for x in range(out.shape[3]):
+                assert out[0, z, y, x] == vals[0, src(z, n[2], gz), src(y, n[1], gy), src(x, n[0], g)]
+
+
+def test_timestep_closed_form_and_adjusters():
+    """Integrator.cpp:20-46: dt = cfl / sum_d(ws_d/dx_d); FixedInterval
+    adjusters clamp to saved*interval - t (FixedIntervalWriter.cpp:45-53)."""
+    ws = [2.0, 3.0, 0.0]
+    dx = [0.5, 0.25, 1.0]
+    dt = O.compute_timestep(ws, dx, 0.475, 0.0)
+    assert dt == 0.475 / (2.0 / 0.5 + 3.0 / 0.25)
+    assert O.compute_timestep(ws, dx, 0.475, 0.99, [1.0], [1]) == pytest.approx(0.01, abs=1e-15)
+    assert O.compute_timestep(ws, dx, 0.475, 0.5, [0.25], [3]) == dt
+
+
+# ---------------------------------------------------------------- whole solves
+
+def exact_riemann(x, t, L, R, gamma=GAMMA):
+    """Exact solution of the 1D Euler Riemann problem (Toro, ch. 4): density."""
+    rl, ul, pl = L
+    rr, ur, pr = R
+    cl, cr = np.sqrt(gamma * pl / rl), np.sqrt(gamma * pr / rr)
+
+    def f(p, r, pk, c):
+        if p > pk:
+            A, B = 2 / ((gamma + 1) * r), (gamma - 1) / (gamma + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * c / (gamma - 1) * ((p / pk) ** ((gamma - 1) / (2 * gamma)) - 1)
+
+    lo, hi = 1e-8, 10.0
+    for _ in range(200):
+        pm = 0.5 * (lo + hi)
+        if f(pm, rl, pl, cl) + f(pm, rr, pr, cr) + ur - ul > 0:
+            hi = pm
+        else:
+            lo = pm
+    p = 0.5 * (lo + hi)
+    um = 0.5 * (ul + ur) + 0.5 * (f(p, rr, pr, cr) - f(p, rl, pl, cl))
+    rho = np.empty_like(x)
+    g1 = (gamma - 1) / (gamma + 1)
+    for i, xi in enumerate(x):
+        s = xi / t
+        if s < um:  # left of contact
+            if p > pl:
+                S = ul - cl * np.sqrt((gamma + 1) / (2 * gamma) * p / pl + (gamma - 1) / (2 * gamma))
+                rho[i] = rl if s < S else rl * (p / pl + g1) / (g1 * p / pl + 1)
+            else:
+                shl, stl = ul - cl, um - cl * (p / pl) ** ((gamma - 1) / (2 * gamma))
+                if s < shl:
+                    rho[i] = rl
+                elif s > stl:
+                    rho[i] = rl * (p / pl) ** (1 / gamma)
+                else:
+                    rho[i] = rl * (2 / (gamma + 1) + g1 / cl * (ul - s)) ** (2 / (gamma - 1))
+        else:
+            if p > pr:
+                S = ur + cr * np.sqrt((gamma + 1) / (2 * gamma) * p / pr + (gamma - 1) / (2 * gamma))
+                rho[i] = rr if s > S else rr * (p / pr + g1) / (g1 * p / pr + 1)
+            else:
+                shr, str_ = ur + cr, um + cr * (p / pr) ** ((gamma - 1) / (2 * gamma))
+                if s > shr:
+                    rho[i] = rr
+                elif s < str_:
+                    rho[i] = rr * (p / pr) ** (1 / gamma)
+                else:
+                    rho[i] = rr * (2 / (gamma + 1) - g1 / cr * (ur - s)) ** (2 / (gamma - 1))
+    return rho
+
+
+def test_sod_converges_to_exact_riemann_solution():
+    """Sod (HLL, no reconstruction, forward Euler, CFL 0.9, T = 0.2): the L1
+    density error against the exact Riemann solution decreases with N
+    (first-order method: rate ~0.5-1 at discontinuities), and the run ends
+    exactly at T."""
+    from alsvinn_amd import initial_data as ic
+    errs = []
+    for N in (256, 512, 1024):
+        s = O.make_setup(O.EQ_EULER1, O.FLUX_HLL, O.RECON_NONE, O.INT_FE, O.BC_NEUMANN,
+                         (N, 1, 1), (-1, 0, 0), (1, 0, 0), cfl=0.9, end_time=0.2)
+        rc, out, t, steps, dts = O.run(s, ic.sod(N))
+        assert rc == 0 and t == 0.2
+        x = ic.midpoints(N, -1.0, 1.0)
+        rho = out[0, 0, 0, 1:-1]
+        errs.append(np.mean(np.abs(rho - exact_riemann(x, 0.2, (1, 0, 1), (0.125, 0, 0.1)))) * 2)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 0.012
+    assert np.log2(errs[1] / errs[2]) > 0.4
+
+
+@pytest.mark.parametrize("integ,order", [(O.INT_FE, 1), (O.INT_RK2, 2), (O.INT_RK3, 3)])
+def test_integrator_temporal_order(integ, order):
+    """IntegratorConvergenceTest.cpp:158-170 (FE/RK2/RK3 >= 0.9/1.9/2.9): on a
+    fixed grid, the difference to a run with a 16x smaller CFL number shrinks
+    like cfl^order."""
+    from alsvinn_amd import initial_data as ic
+    N = 64
+    x = ic.midpoints(N, 0.0, 1.0)
+    rho = 1 + 0.2 * np.sin(2 * np.pi * x)
+    u0 = np.zeros((3, 1, 1, N + 4))
+    st = ic.conserved_euler(rho, [0.5 + 0 * x], 1.0 + 0 * x, GAMMA)
+    for k in range(3):
+        u0[k, 0, 0, 2:-2] = st[k]
+
+    def run(cfl):
+        s = O.make_setup(O.EQ_EULER1, O.FLUX_HLL3, O.RECON_WENO2, integ, O.BC_PERIODIC,
+                         (N, 1, 1), (0, 0, 0), (1, 0, 0), cfl=cfl, end_time=0.1)
+        rc, out, t, steps, _ = O.run(s, u0)
+        assert rc == 0 and t == pytest.approx(0.1, abs=1e-15)
+        return out[0, 0, 0, 2:-2]
+
+    ref = run(0.4 / 64)
+    e1 = np.abs(run(0.4) - ref).mean()
+    e2 = np.abs(run(0.2) - ref).mean()
+    assert np.log2(e1 / e2) > order - 0.25
+
+
+def test_kh_conserves_totals_and_x_invariance():
+    """Periodic KH: sum(rho), sum(m), sum(E) are conserved to roundoff; with
+    all perturbation amplitudes zero the flow stays x-invariant."""
+    from alsvinn_amd import initial_data as ic
+    n = 32
+    X = O.stl_mersenne_uniform(1, 40)[0]
+    s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                     (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=1.0, max_steps=30)
+    u0 = ic.kelvin_helmholtz_2d(n, X)
+    rc, out, t, steps, _ = O.run(s, u0)
+    assert rc == 0 and steps == 30
+    tot0 = u0[:, 0, 2:-2, 2:-2].sum(axis=(1, 2))
+    tot1 = out[:, 0, 2:-2, 2:-2].sum(axis=(1, 2))
+    np.testing.assert_allclose(tot1, tot0, rtol=1e-12, atol=1e-12 * n * n)
+    Xz = X.copy()
+    Xz[:20] = 0.0  # a = 0: no perturbation
+    rc, out, t, steps, _ = O.run(s, ic.kelvin_helmholtz_2d(n, Xz))
+    inner = out[:, 0, 2:-2, 2:-2]
+    assert np.all(inner == inner[:, :, :1])
+
+
+def test_constraint_violation_is_reported():
+    """Simulator.cpp:180-189: a state with negative pressure THROWs (status 2)."""
+    n = 16
+    s = O.make_setup(O.EQ_EULER1, O.FLUX_HLL, O.RECON_NONE, O.INT_FE, O.BC_NEUMANN,
+                     (n, 1, 1), (0, 0, 0), (1, 0, 0), cfl=0.9, end_time=1.0)
+    u = np.zeros(O.volume_shape(s))
+    u[0] = 1.0
+    u[2] = 2.5
+    u[2, 0, 0, 4:9] = -100.0  # negative energy -> negative pressure
+    assert O.obeys_constraints(s, u) == 0
+    rc, *_ = O.run(s, u)
+    assert rc == 2
+
+
+# ---------------------------------------------------------------- statistics
+
+def test_meanvar_closed_form():
+    """MeanVariance.cpp:30-67: samples c + k, k = 0..M-1 -> mean c + (M-1)/2,
+    variance (1/M normalisation) (M^2-1)/12."""
+    M = 9
+    base = np.linspace(-1, 1, 12).reshape(3, 4)
+    samples = np.stack([base + k for k in range(M)])
+    mean, var = O.meanvar(samples)
+    np.testing.assert_allclose(mean, base + (M - 1) / 2, rtol=1e-14)
+    np.testing.assert_allclose(var, np.full_like(base, (M * M - 1) / 12), rtol=1e-10)
+
+
+def _structure_linear_expect(N, h, p, periodic):
+    """StructureTestBoundary.cpp:255-290: u_i = i in 1D."""
+    tot = 0.0
+    for i in range(N):
+        for j in (-h, h):
+            if 0 <= i + j < N:
+                tot += abs(-j) ** p
+            elif i + j < 0:
+                tot += abs(i - (N + (i + j))) ** p if periodic else abs(i) ** p
+            else:
+                tot += abs(i - (i + j - N)) ** p if periodic else abs(i - (N - 1)) ** p
+    return tot / N
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("p", [1.0, 2.0, 3.0])
+def test_structure_cube_linear_1d(periodic, p):
+    N, H, g = 16, 8, 2
+    u = np.zeros((1, 1, 1, N + 2 * g))
+    u[0, 0, 0, g:-g] = np.arange(N)
+    out = O.structure_cube(u, (N, 1, 1), g, H, p, periodic)
+    for h in range(1, H):
+        assert out[0, h] == pytest.approx(_structure_linear_expect(N, h, p, periodic), rel=4e-16 * 4)
+
+
+def test_structure_constant_is_zero():
+    u = np.full((2, 1, 10, 10), 3.25)
+    assert np.all(O.structure_cube(u, (6, 6, 1), 2, 4, 2.0, True) == 0)
+    assert np.all(O.structure_basic(u, (6, 6, 1), 2, 4, 2.0, 1, False) == 0)
+
+
+def test_structure_cube_shell_counts():
+    """The radius-h shell of the 2D (3D) cube holds 8h (24h^2+2) points: with
+    u = 1 at one cell and 0 elsewhere on a large periodic grid, S_1(h) =
+    2*shell(h)/N."""
+    for n, dims in (((21, 21, 1), 2), ((11, 11, 11), 3)):
+        g = 2
+        shape = (1, n[2] + (2 * g if dims == 3 else 0), n[1] + 2 * g, n[0] + 2 * g)
+        u = np.zeros(shape)
+        zc = g + n[2] // 2 if dims == 3 else 0
+        u[0, zc, g + n[1] // 2, g + n[0] // 2] = 1.0
+        out = O.structure_cube(u, n, g, 4, 1.0, True)
+        N = n[0] * n[1] * n[2]
+        for h in range(1, 4):
+            shell = 8 * h if dims == 2 else 24 * h * h + 2
+            assert out[0, h] == pytest.approx(2 * shell / N, rel=1e-14)
