@@ -192,21 +192,23 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
 // the ensemble's stream (no host synchronisation between launches); the
 // durations are summed once the stream has drained.
 struct KernelTimer {
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;  // created before any launch is enqueued
     std::vector<int> kind;
+    size_t next = 0;
     hipStream_t s = nullptr;
+    void reserve(size_t launches) {
+        ev.resize(2 * launches);
+        for (auto& x : ev) hipEventCreate(&x);
+    }
     void begin() {
-        hipEvent_t a;
-        hipEventCreate(&a);
-        hipEventRecord(a, s);
-        ev.push_back(a);
+        if (next + 1 < ev.size()) hipEventRecord(ev[next], s);
     }
     void end(int k) {
-        hipEvent_t b;
-        hipEventCreate(&b);
-        hipEventRecord(b, s);
-        ev.push_back(b);
-        kind.push_back(k);
+        if (next + 1 < ev.size()) {
+            hipEventRecord(ev[next + 1], s);
+            kind.push_back(k);
+            next += 2;
+        }
     }
     void collect(double ms[3], long long count[3]) {
         hipStreamSynchronize(s);
@@ -689,6 +691,7 @@ int alsvinn_hip_ensemble_step_timed(alsvinn_hip_ensemble* e, int nsteps, double 
     if (!e || nsteps < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
     KernelTimer tm;
     tm.s = e->stream;
+    tm.reserve((size_t)nsteps * 5);
     HIP_TRY(hipStreamSynchronize(e->stream));
     int rc = enqueue_steps(e, nsteps, &tm);
     double ms[3];

@@ -1,18 +1,21 @@
 """Benchmark: batched Monte Carlo Kelvin-Helmholtz ensemble on one MI355X.
 
-Workload (one "step" = one SSP-RK3 timestep of every sample of the batch):
+Headline workload (one "step" = one SSP-RK3 timestep of every sample):
   2D Euler Kelvin-Helmholtz, 256^2 cells per sample on [0,1]^2, periodic,
   gamma 1.4, WENO2 + HLL3, SSP-RK3, CFL 0.475, FP64 state, 1024 samples
   advanced together; each sample's 40 perturbation parameters are the
-  reference sampler's (global std::mt19937_64, uniform[0,1]) draws for that
+  reference sampler's draws (global std::mt19937_64, uniform[0,1]) for that
   sample. Inputs are resident in HBM before the timed region starts.
 
-Reported: cell-updates/s of the batch (value), the same configuration run one
-sample at a time on the GPU (plain path), the roofline of the dominant kernel
-(the fused stage kernel, timed with HIP events on the ensemble's stream) and
-a CPU baseline (the oracle on the host cores, a bounded sample).
+Reported beside it (extra fields, one GPU): the same batch run one sample at a
+time (plain path), the FP32-state variant with its error against the FP64
+result, a mixed-size ensemble (128^2/256^2/512^2), the 1024^2 single solve,
+the 3D FP32 256^3 x 64 ensemble, the statistics pass and the Sod shock tube;
+the roofline of the dominant kernel (the fused stage kernel, timed with HIP
+events on the ensemble's stream) and a CPU baseline (the oracle on the host
+cores, a bounded sample).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--only main]
 """
 import argparse
 import json
@@ -20,12 +23,13 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-NVAR = 4
-STAGES = 3
+PER_GRAPH = 10
 
 
 def parse():
@@ -37,8 +41,10 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--plain-steps", type=int, default=100)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra-steps", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["all", "main", "fe-calibration"], default="all")
+    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
 
@@ -47,31 +53,52 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_batch(hip, nsamples, n, precision, X):
+def rounded(k):
+    return max(PER_GRAPH, (k // PER_GRAPH) * PER_GRAPH)
+
+
+def kh_scheme(hip, precision, eq=None):
+    return hip.make_scheme(eq or hip.EQ_EULER2, hip.FLUX_HLL3, hip.RECON_WENO2, hip.INT_RK3,
+                           hip.BC_PERIODIC, gamma=1.4, cfl=0.475,
+                           precision=hip.FP64 if precision == "fp64" else hip.FP32)
+
+
+def make_batch(hip, sizes, precision, X, dim=2):
     from alsvinn_amd import initial_data as ic
-    scheme = hip.make_scheme(hip.EQ_EULER2, hip.FLUX_HLL3, hip.RECON_WENO2, hip.INT_RK3,
-                             hip.BC_PERIODIC, gamma=1.4, cfl=0.475,
-                             precision=hip.FP64 if precision == "fp64" else hip.FP32)
-    samples = [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), 2.0) for _ in range(nsamples)]
+    if dim == 2:
+        scheme = kh_scheme(hip, precision)
+        samples = [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), 2.0) for n in sizes]
+    else:
+        scheme = kh_scheme(hip, precision, hip.EQ_EULER3)
+        samples = [hip.make_sample((n, n, n), (0, 0, 0), (1, 1, 1), 2.0) for n in sizes]
     e = hip.Ensemble(scheme, samples)
-    for i in range(nsamples):
-        e.upload(i, ic.kelvin_helmholtz_2d(n, X[i]))
+    for i, n in enumerate(sizes):
+        u = ic.kelvin_helmholtz_2d(n, X[i]) if dim == 2 else ic.kelvin_helmholtz_3d(n, X[i])
+        e.upload(i, u)
     e.apply_boundary()
     e.synchronize()
     return e
 
 
-def timed_steps(e, steps, per_graph):
+def timed_steps(e, steps):
     e.synchronize()
     t0 = time.perf_counter()
-    e.run_steps(steps, per_graph)
+    e.run_steps(steps, PER_GRAPH)
     e.synchronize()
     return time.perf_counter() - t0
 
 
+def throughput(e, cells, steps, warm=PER_GRAPH):
+    e.run_steps(rounded(warm), PER_GRAPH)
+    e.synchronize()
+    steps = rounded(steps)
+    secs = timed_steps(e, steps)
+    return cells * steps / secs, 1e3 * secs / steps, steps
+
+
 def cpu_baseline(n, X0, steps):
-    """The oracle (clean-room restatement of the reference CPU path: OpenMP
-    over rows, separate passes) on the host cores, one sample, `steps` steps."""
+    """The oracle (restatement of the reference CPU path: OpenMP over rows,
+    separate passes) on the host cores, one sample, `steps` steps."""
     from tests import oracle_lib as O
     from alsvinn_amd import initial_data as ic
     s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
@@ -84,9 +111,43 @@ def cpu_baseline(n, X0, steps):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": "oracle (C++ restatement of the reference cpu platform, OpenMP, -O3 "
-                      "-ffp-contract=off), 1 sample of the same 256^2 KH WENO2+HLL3 SSP-RK3 "
-                      "FP64 workload, %d steps, %.1f s" % (steps, dt)}
+            "sample": "oracle (C++ restatement of the reference cpu platform, OpenMP over rows, "
+                      "-O3 -ffp-contract=off), 1 sample of the same %d^2 KH WENO2+HLL3 SSP-RK3 "
+                      "FP64 workload, %d steps in %.1f s" % (n, steps, dt)}
+
+
+def fp32_error(hip, M, n, X, steps):
+    """FP32-state batch vs the FP64 batch (bit-identical to the oracle) after
+    the same number of steps: relative L1 of rho (first 8 samples) and of the
+    ensemble mean / variance of rho."""
+    e64 = make_batch(hip, [n] * M, "fp64", X)
+    e32 = make_batch(hip, [n] * M, "fp32", X)
+    e64.autotune()
+    sch = e32.autotune()
+    v32, ms32, k = throughput(e32, M * n * n, steps)
+    e64.run_steps(rounded(steps) + PER_GRAPH, PER_GRAPH)
+    g = 2
+    errs = []
+    for i in range(min(8, M)):
+        a = e64.download(i)[0, 0, g:-g, g:-g]
+        b = e32.download(i)[0, 0, g:-g, g:-g]
+        errs.append(np.abs(a - b).sum() / np.abs(a).sum())
+    m64, v64 = e64.meanvar()
+    m32, v32s = e32.meanvar()
+    sl = (0, 0, slice(g, -g), slice(g, -g))
+    out = {"value": v32, "unit": "cell-updates/s", "ms_per_step": ms32, "steps": k,
+           "dtype": "f32 state, f64 CFL/statistics/totals", "schedule": sch,
+           "bytes_per_cell_update": 144,
+           "hbm_fraction": v32 * 144 / HBM_PEAK,
+           "rho_rel_l1_vs_fp64": float(np.mean(errs)),
+           "mean_rho_rel_l1_vs_fp64": float(np.abs(m64[sl] - m32[sl]).sum() / np.abs(m64[sl]).sum()),
+           "var_rho_rel_l1_vs_fp64": float(np.abs(v64[sl] - v32s[sl]).sum()
+                                           / max(np.abs(v64[sl]).sum(), 1e-300)),
+           "after_steps": k + PER_GRAPH * 2,
+           "workload": "KH 256^2 x %d, FP32 state" % M}
+    e64.close()
+    e32.close()
+    return out
 
 
 def main():
@@ -99,20 +160,40 @@ def main():
     import alsvinn_amd as hip
     from alsvinn_amd import sampling
 
+    if args.only == "fe-calibration":
+        # forward-Euler batch: its elementwise kernel reads two volumes and
+        # writes one (known bytes), used to calibrate the HBM byte counters
+        from alsvinn_amd import initial_data as ic
+        X = sampling.stl_mersenne_uniform(64, 40)
+        sc = hip.make_scheme(hip.EQ_EULER2, hip.FLUX_HLL3, hip.RECON_NONE, hip.INT_FE,
+                             hip.BC_PERIODIC)
+        e = hip.Ensemble(sc, [hip.make_sample((256, 256, 1), (0, 0, 0), (1, 1, 0), 2.0)
+                              for _ in range(64)])
+        for i in range(64):
+            u = ic.kelvin_helmholtz_2d(256, X[i], ghosts=1)
+            e.upload(i, u)
+        e.apply_boundary()
+        e.run_steps(10, 10)
+        e.synchronize()
+        print(json.dumps({"calibration": "axpy_kernel reads 2 and writes 1 volume of "
+                          "64 x 258^2 x 4 doubles per launch",
+                          "bytes_read": 2 * 64 * 258 * 258 * 4 * 8,
+                          "bytes_written": 64 * 258 * 258 * 4 * 8}))
+        return
+
     M, n = args.samples, args.n
-    per_graph = 10
-    steps = max(per_graph, (args.steps // per_graph) * per_graph)
-    warm = max(per_graph, (args.warmup // per_graph) * per_graph)
+    esize = 8 if args.precision == "fp64" else 4
+    steps = rounded(args.steps)
+    warm = rounded(args.warmup)
     log("generating %d samples of %d^2 (reference sampler draws)" % (M, n))
     X = sampling.stl_mersenne_uniform(M, 40)
-    e = make_batch(hip, M, n, args.precision, X)
+    e = make_batch(hip, [n] * M, args.precision, X)
     sched = e.autotune()
     log("autotuned schedule: %s" % sched)
-    log("warmup %d steps" % warm)
-    e.run_steps(warm, per_graph)
+    e.run_steps(warm, PER_GRAPH)
     e.synchronize()
     log("timed %d steps" % steps)
-    secs = timed_steps(e, steps, per_graph)
+    secs = timed_steps(e, steps)
     cells = M * n * n
     value = cells * steps / secs
     ms_per_step = 1e3 * secs / steps
@@ -121,37 +202,101 @@ def main():
     # dominant kernel: fused stage kernel, HIP events around each launch
     kms, launches = e.step_timed(6)
     stage_ms = kms[0] / launches[0]
-    esize = 8 if args.precision == "fp64" else 4
-    bytes_per_stage = 3 * NVAR * esize * cells  # read stage input, read U^n, write result
+    bytes_per_stage = 3 * 4 * esize * cells  # read stage input, read U^n, write result
     achieved = bytes_per_stage / (stage_ms * 1e-3)
     log("stage kernel avg %.3f ms, %.1f GB/s algorithmic" % (stage_ms, achieved / 1e9))
-    info0 = e.info(0)
+
+    extra = {}
+    if args.only == "all":
+        t0 = time.perf_counter()
+        mean, var = e.meanvar()
+        t1 = time.perf_counter()
+        sf = e.structure_cube(33, 2.0)
+        t2 = time.perf_counter()
+        extra["statistics"] = {"meanvar_ms": 1e3 * (t1 - t0), "structure_cube_h32_ms": 1e3 * (t2 - t1),
+                               "samples": M, "note": "device kernels + host copy of the result"}
+        log("statistics: meanvar %.1f ms, structure_cube(H=33) %.1f ms" % (1e3 * (t1 - t0), 1e3 * (t2 - t1)))
     e.close()
 
     plain = None
     if not args.no_plain:
-        ep = make_batch(hip, 1, n, args.precision, X[:1])
+        ep = make_batch(hip, [n], args.precision, X[:1])
         psched = ep.autotune()
-        log("plain schedule: %s" % psched)
-        ep.run_steps(per_graph, per_graph)
-        ep.synchronize()
-        ps = max(per_graph, (args.plain_steps // per_graph) * per_graph)
-        psecs = timed_steps(ep, ps, per_graph)
+        pval, pms, ps = throughput(ep, n * n, args.plain_steps)
         ep.close()
-        pval = n * n * ps / psecs
-        plain = {"value": pval, "unit": "cell-updates/s", "ms_per_step": 1e3 * psecs / ps,
+        plain = {"value": pval, "unit": "cell-updates/s", "ms_per_step": pms,
                  "workload": "one sample at a time (batch of 1), same scheme and grid",
-                 "batched_over_plain": value / pval}
+                 "schedule": psched, "batched_over_plain": value / pval}
         log("plain: %.3e cell-updates/s, batched/plain = %.1fx" % (pval, value / pval))
 
+    if args.only == "all":
+        K = args.extra_steps
+        log("fp32 variant")
+        extra["fp32_state"] = fp32_error(hip, M, n, X, K)
+        log("  %s" % extra["fp32_state"])
+        log("mixed sizes")
+        sizes = [128] * 171 + [256] * 171 + [512] * 170
+        Xm = sampling.stl_mersenne_uniform(len(sizes), 40)
+        em = make_batch(hip, sizes, "fp64", Xm)
+        msch = em.autotune()
+        cm = sum(s * s for s in sizes)
+        mv, mms, mk = throughput(em, cm, K)
+        em.close()
+        extra["mixed_sizes"] = {"value": mv, "unit": "cell-updates/s", "ms_per_step": mms,
+                                "steps": mk, "schedule": msch, "hbm_fraction": mv * 288 / HBM_PEAK,
+                                "workload": "KH 2D FP64, 512 samples: 171x128^2, 171x256^2, 170x512^2"}
+        log("  mixed: %.3e cell-updates/s" % mv)
+        log("single 1024^2")
+        es = make_batch(hip, [1024], "fp64", X[:1])
+        ssch = es.autotune()
+        sv, sms, sk = throughput(es, 1024 * 1024, max(K, 60))
+        es.close()
+        extra["single_1024"] = {"value": sv, "unit": "cell-updates/s", "ms_per_step": sms,
+                                "steps": sk, "schedule": ssch, "hbm_fraction": sv * 288 / HBM_PEAK,
+                                "workload": "KH 2D 1024^2 single solve, FP64 (plain GPU path)"}
+        log("  single 1024^2: %.3e cell-updates/s" % sv)
+        log("3D fp32 256^3 x 64")
+        X3 = sampling.stl_mersenne_uniform(64, 400)
+        e3 = make_batch(hip, [256] * 64, "fp32", X3, dim=3)
+        s3 = e3.autotune()
+        v3, ms3, k3 = throughput(e3, 64 * 256 ** 3, 10, warm=10)
+        e3.close()
+        extra["kh3d_fp32"] = {"value": v3, "unit": "cell-updates/s", "ms_per_step": ms3,
+                              "steps": k3, "schedule": s3, "bytes_per_cell_update": 180,
+                              "hbm_fraction": v3 * 180 / HBM_PEAK,
+                              "workload": "KH 3D 256^3 x 64 samples, FP32 state"}
+        log("  3d: %.3e cell-updates/s" % v3)
+        log("sod")
+        from alsvinn_amd import initial_data as ic
+        sod = hip.Ensemble(hip.make_scheme(hip.EQ_EULER1, hip.FLUX_HLL, hip.RECON_NONE, hip.INT_FE,
+                                           hip.BC_NEUMANN, cfl=0.9),
+                           [hip.make_sample((1024, 1, 1), (-1, 0, 0), (1, 0, 0), 0.2)])
+        sod.upload(0, ic.sod(1024))
+        sod.apply_boundary()
+        t0 = time.perf_counter()
+        sod.run(50)
+        t1 = time.perf_counter()
+        info = sod.info(0)
+        sod.close()
+        extra["sod"] = {"solve_ms": 1e3 * (t1 - t0), "steps": info["steps"], "t": info["t"],
+                        "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
+        log("  sod: %d steps in %.2f ms" % (info["steps"], 1e3 * (t1 - t0)))
+
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.only == "all":
         probe = cpu_baseline(n, X[0], 4)
         steps_cpu = int(max(4, min(4000, args.cpu_seconds * probe["value"] / (n * n))))
         cpu = cpu_baseline(n, X[0], steps_cpu)
         log("cpu baseline: %.3e cell-updates/s on %d threads" % (cpu["value"], cpu["cores"]))
 
-    B = STAGES * 3 * NVAR * esize
+    B = 3 * 3 * 4 * esize
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "stage_kernel_kh256x1024_fp64.md")
+    if args.precision == "fp64" and M == 1024 and n == 256 and os.path.exists(prof):
+        # HBM bytes per stage launch from the calibrated FETCH_SIZE/WRITE_SIZE runs
+        for ln in open(prof):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                traffic = float(ln.split(":")[1])
     line = {
         "metric": "cell-updates/s + % HBM peak, batched vs plain (1 GPU); p50/p99 UQ-request latency",
         "value": value,
@@ -172,10 +317,12 @@ def main():
                    "hbm_fraction": value * B / HBM_PEAK, "schedule": sched},
         "plain": plain,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "stage_kernel_12<double, Euler2, WENO2, HLL3>",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_per_stage,
+                     "kernel": "stage_kernel_12<double, EulerEq<double,2>, ReconWENO2, FluxHLL3, 2>",
                      "avg_launch_ms": stage_ms, "launches": launches[0]},
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 
