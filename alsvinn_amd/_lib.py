@@ -25,6 +25,10 @@ EXPORTS = [
     "alsvinn_hip_ghost_cells",
     "alsvinn_hip_ensemble_create",
     "alsvinn_hip_ensemble_destroy",
+    "alsvinn_hip_pool_create",
+    "alsvinn_hip_pool_assign",
+    "alsvinn_hip_pool_release",
+    "alsvinn_hip_ensemble_states",
     "alsvinn_hip_ensemble_layout",
     "alsvinn_hip_ensemble_upload",
     "alsvinn_hip_ensemble_download",
@@ -106,6 +110,11 @@ def load():
     L.alsvinn_hip_ensemble_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
                                               ctypes.POINTER(Sample), vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_ensemble_destroy.argtypes = [vp]
+    L.alsvinn_hip_pool_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_longlong, vp, ctypes.POINTER(vp)]
+    L.alsvinn_hip_pool_assign.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Sample), ctypes.c_longlong]
+    L.alsvinn_hip_pool_release.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_ensemble_states.argtypes = [vp, dp, llp, ip, ip]
     L.alsvinn_hip_ensemble_layout.argtypes = [vp, ctypes.c_int, ip, llp, llp]
     L.alsvinn_hip_ensemble_upload.argtypes = [vp, ctypes.c_int, dp]
     L.alsvinn_hip_ensemble_download.argtypes = [vp, ctypes.c_int, dp]

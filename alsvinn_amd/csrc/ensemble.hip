@@ -106,6 +106,45 @@ void* buf_at(void* base, long long off) {
     return (void*)((T*)base + off);
 }
 
+SampleDesc make_desc(const alsvinn_hip_scheme& sc, int G, const alsvinn_hip_sample& p,
+                     long long base) {
+    SampleDesc D;
+    std::memset(&D, 0, sizeof(D));
+    D.nx = p.nx;
+    D.ny = p.ny;
+    D.nz = p.nz;
+    D.gx = G;
+    D.gy = p.ny > 1 ? G : 0;
+    D.gz = p.nz > 1 ? G : 0;
+    D.tx = p.nx + 2 * D.gx;
+    D.ty = p.ny + 2 * D.gy;
+    D.tz = p.nz + 2 * D.gz;
+    D.vstride = (long long)D.tx * D.ty * D.tz;
+    D.base = base;
+    D.bc = sc.boundary;
+    // Grid.cpp:24-41: (top - origin) / N, inactive axes 1
+    const int n[3] = {p.nx, p.ny, p.nz};
+    for (int k = 0; k < 3; ++k) D.dx[k] = (p.top[k] - p.origin[k]) / double(n[k]);
+    if (p.ny == 1) D.dx[1] = 1;
+    if (p.nz == 1) D.dx[2] = 1;
+    D.cfl = sc.cfl;
+    D.end_time = p.end_time;
+    D.nadj = p.num_adjusters;
+    D.interval[0] = p.interval[0];
+    D.interval[1] = p.interval[1];
+    D.max_steps = p.max_steps;
+    return D;
+}
+
+SampleState fresh_state(const alsvinn_hip_sample& p) {
+    SampleState S;
+    std::memset(&S, 0, sizeof(S));
+    S.saved[0] = p.saved0[0];
+    S.saved[1] = p.saved0[1];
+    S.active = (0 < p.end_time && p.max_steps != 0) ? 1 : 0;
+    return S;
+}
+
 // Band height (rows per wave task) for a sample of ny rows: whole columns
 // unless the ensemble would then launch fewer than ~4 waves per SIMD.
 int band_rows(long long tasks_per_band, int ny) {
@@ -123,7 +162,8 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
     const int W = 64 - 2 * G;
     long long waves = 0;
     for (int s = 0; s < e->nsamples; ++s)
-        waves += (long long)((e->desc[s].nx + W - 1) / W) * (e->dim == 3 ? e->desc[s].ny : 1);
+        if (e->assigned[s])
+            waves += (long long)((e->desc[s].nx + W - 1) / W) * (e->dim == 3 ? e->desc[s].ny : 1);
     // (ny, nz, y0, y1, z0, z1) -> index of the task still open for packing
     std::map<std::tuple<int, int, int, int, int, int>, size_t> open;
     auto add_strips = [&](int s, int y0, int y1, int z0, int z1) {
@@ -174,6 +214,7 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
         }
     };
     for (int s = 0; s < e->nsamples; ++s) {
+        if (!e->assigned[s]) continue;
         const SampleDesc& D = e->desc[s];
         if (e->dim == 3) {
             // one row per task, bands of planes marched in z
@@ -309,14 +350,8 @@ int enqueue_steps(alsvinn_hip_ensemble* e, int nsteps, KernelTimer* tm = nullptr
 int init_state(alsvinn_hip_ensemble* e) {
     std::vector<SampleState> st(e->nsamples);
     for (int s = 0; s < e->nsamples; ++s) {
-        SampleState& S = st[s];
-        std::memset(&S, 0, sizeof(S));
-        S.t = 0;
-        S.steps = 0;
-        S.saved[0] = e->spec[s].saved0[0];
-        S.saved[1] = e->spec[s].saved0[1];
-        S.active = (0 < e->spec[s].end_time && e->spec[s].max_steps != 0) ? 1 : 0;
-        S.status = 0;
+        st[s] = fresh_state(e->spec[s]);
+        if (!e->assigned[s]) st[s].active = 0;
     }
     HIP_TRY(hipMemcpy(e->d_state, st.data(), sizeof(SampleState) * st.size(), hipMemcpyHostToDevice));
     std::vector<unsigned long long> z(3 * (size_t)e->nsamples, 0ull);
@@ -338,6 +373,7 @@ int upload_tasks(alsvinn_hip_ensemble* e) {
     hipFree(e->d_tasks);
     e->d_tasks = nullptr;
     e->ntasks = (int)tasks.size();
+    if (tasks.empty()) return 0;
     HIP_TRY(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
     HIP_TRY(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
     return 0;
@@ -373,75 +409,9 @@ double time_first_stage(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps
 
 }  // namespace
 
-extern "C" {
-
-const char* alsvinn_hip_last_error(void) { return g_last_error.c_str(); }
-
-int alsvinn_hip_ghost_cells(const alsvinn_hip_scheme* scheme) {
-    if (validate(scheme)) return -1;
-    return ghost_cells(*scheme);
-}
-
-int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
-                                const alsvinn_hip_sample* samples, void* stream,
-                                alsvinn_hip_ensemble** out) {
-    if (int rc = validate(scheme)) return rc;
-    if (!out || !samples || nsamples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
-    *out = nullptr;
-    auto* e = new alsvinn_hip_ensemble();
-    e->scheme = *scheme;
-    e->nsamples = nsamples;
-    e->nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
-    e->ghosts = ghost_cells(*scheme);
-    e->esize = scheme->precision == ALSVINN_HIP_FP64 ? 8 : 4;
-    const int G = e->ghosts;
-    long long off = 0;
-    for (int s = 0; s < nsamples; ++s) {
-        const alsvinn_hip_sample& p = samples[s];
-        const int dim = 1 + (p.ny > 1) + (p.nz > 1);
-        if (s == 0) e->dim = dim;
-        if (dim != e->dim || p.nx < 1 || p.ny < 1 || p.nz < 1) {
-            delete e;
-            return fail(ALSVINN_HIP_E_INVALID, "all samples of an ensemble must have the same dimension");
-        }
-        if (p.nx < G || (p.ny > 1 && p.ny < G) || (p.nz > 1 && p.nz < G)) {
-            delete e;
-            return fail(ALSVINN_HIP_E_INVALID, "grid smaller than the number of ghost cells");
-        }
-        if (p.num_adjusters < 0 || p.num_adjusters > 2) {
-            delete e;
-            return fail(ALSVINN_HIP_E_INVALID, "num_adjusters must be 0..2");
-        }
-        SampleDesc D;
-        std::memset(&D, 0, sizeof(D));
-        D.nx = p.nx;
-        D.ny = p.ny;
-        D.nz = p.nz;
-        D.gx = G;
-        D.gy = p.ny > 1 ? G : 0;
-        D.gz = p.nz > 1 ? G : 0;
-        D.tx = p.nx + 2 * D.gx;
-        D.ty = p.ny + 2 * D.gy;
-        D.tz = p.nz + 2 * D.gz;
-        D.vstride = (long long)D.tx * D.ty * D.tz;
-        D.base = off;
-        D.bc = scheme->boundary;
-        const int n[3] = {p.nx, p.ny, p.nz};
-        for (int k = 0; k < 3; ++k) D.dx[k] = (p.top[k] - p.origin[k]) / double(n[k]);
-        if (p.ny == 1) D.dx[1] = 1;
-        if (p.nz == 1) D.dx[2] = 1;
-        D.cfl = scheme->cfl;
-        D.end_time = p.end_time;
-        D.nadj = p.num_adjusters;
-        D.interval[0] = p.interval[0];
-        D.interval[1] = p.interval[1];
-        D.max_steps = p.max_steps;
-        off += D.vstride * e->nvar;
-        e->max_vstride = std::max(e->max_vstride, D.vstride);
-        e->desc.push_back(D);
-        e->spec.push_back(p);
-    }
-    e->total = off;
+static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* scheme, void* stream,
+                         alsvinn_hip_ensemble** out) {
+    const int nsamples = e->nsamples;
     int rc = 0;
     if (e->esize == 8)
         rc = select_kernels<double>(*scheme, e->dim, e->k64);
@@ -483,14 +453,68 @@ int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
     std::vector<WaveTask> tasks;
     build_tasks(e, tasks);
     e->ntasks = (int)tasks.size();
-    HIP_TRY_C(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
-    HIP_TRY_C(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
+    if (!tasks.empty()) {
+        HIP_TRY_C(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
+        HIP_TRY_C(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(),
+                            hipMemcpyHostToDevice));
+    }
     HIP_TRY_C(hipEventCreate(&e->ev0));
     HIP_TRY_C(hipEventCreate(&e->ev1));
 #undef HIP_TRY_C
     if (int r = init_state(e)) return cleanup(r);
     *out = e;
     return 0;
+}
+
+
+extern "C" {
+
+const char* alsvinn_hip_last_error(void) { return g_last_error.c_str(); }
+
+int alsvinn_hip_ghost_cells(const alsvinn_hip_scheme* scheme) {
+    if (validate(scheme)) return -1;
+    return ghost_cells(*scheme);
+}
+
+int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
+                                const alsvinn_hip_sample* samples, void* stream,
+                                alsvinn_hip_ensemble** out) {
+    if (int rc = validate(scheme)) return rc;
+    if (!out || !samples || nsamples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
+    *out = nullptr;
+    auto* e = new alsvinn_hip_ensemble();
+    e->scheme = *scheme;
+    e->nsamples = nsamples;
+    e->nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
+    e->ghosts = ghost_cells(*scheme);
+    e->esize = scheme->precision == ALSVINN_HIP_FP64 ? 8 : 4;
+    const int G = e->ghosts;
+    long long off = 0;
+    for (int s = 0; s < nsamples; ++s) {
+        const alsvinn_hip_sample& p = samples[s];
+        const int dim = 1 + (p.ny > 1) + (p.nz > 1);
+        if (s == 0) e->dim = dim;
+        if (dim != e->dim || p.nx < 1 || p.ny < 1 || p.nz < 1) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "all samples of an ensemble must have the same dimension");
+        }
+        if (p.nx < G || (p.ny > 1 && p.ny < G) || (p.nz > 1 && p.nz < G)) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "grid smaller than the number of ghost cells");
+        }
+        if (p.num_adjusters < 0 || p.num_adjusters > 2) {
+            delete e;
+            return fail(ALSVINN_HIP_E_INVALID, "num_adjusters must be 0..2");
+        }
+        const SampleDesc D = make_desc(*scheme, G, p, off);
+        off += D.vstride * e->nvar;
+        e->max_vstride = std::max(e->max_vstride, D.vstride);
+        e->desc.push_back(D);
+        e->spec.push_back(p);
+        e->assigned.push_back(1);
+    }
+    e->total = off;
+    return finish_create(e, scheme, stream, out);
 }
 
 int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
@@ -511,6 +535,99 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     if (e->ev1) hipEventDestroy(e->ev1);
     if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
     delete e;
+    return 0;
+}
+
+int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int dim,
+                            long long arena_elements, void* stream, alsvinn_hip_ensemble** out) {
+    if (int rc = validate(scheme)) return rc;
+    if (!out || nslots < 1 || dim < 1 || dim > 3 || arena_elements < 1)
+        return fail(ALSVINN_HIP_E_INVALID, "bad pool arguments");
+    *out = nullptr;
+    auto* e = new alsvinn_hip_ensemble();
+    e->scheme = *scheme;
+    e->nsamples = nslots;
+    e->nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
+    e->ghosts = ghost_cells(*scheme);
+    e->esize = scheme->precision == ALSVINN_HIP_FP64 ? 8 : 4;
+    e->dim = dim;
+    alsvinn_hip_sample blank;
+    std::memset(&blank, 0, sizeof(blank));
+    SampleDesc D;
+    std::memset(&D, 0, sizeof(D));
+    for (int s = 0; s < nslots; ++s) {
+        e->desc.push_back(D);
+        e->spec.push_back(blank);
+        e->assigned.push_back(0);
+    }
+    e->total = arena_elements;
+    return finish_create(e, scheme, stream, out);
+}
+
+int alsvinn_hip_pool_assign(alsvinn_hip_ensemble* e, int slot, const alsvinn_hip_sample* p,
+                            long long offset) {
+    if (!e || !p || slot < 0 || slot >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad slot");
+    const int G = e->ghosts;
+    const int dim = 1 + (p->ny > 1) + (p->nz > 1);
+    if (dim != e->dim || p->nx < G || (p->ny > 1 && p->ny < G) || (p->nz > 1 && p->nz < G))
+        return fail(ALSVINN_HIP_E_INVALID, "grid does not fit the pool's dimension or ghost cells");
+    if (p->num_adjusters < 0 || p->num_adjusters > 2)
+        return fail(ALSVINN_HIP_E_INVALID, "num_adjusters must be 0..2");
+    const SampleDesc D = make_desc(e->scheme, G, *p, offset);
+    if (offset < 0 || offset + D.vstride * e->nvar > e->total)
+        return fail(ALSVINN_HIP_E_INVALID, "sample does not fit in the pool arena at this offset");
+    for (int s = 0; s < e->nsamples; ++s) {
+        if (s == slot || !e->assigned[s]) continue;
+        const long long a0 = e->desc[s].base, a1 = a0 + e->desc[s].vstride * e->nvar;
+        if (offset < a1 && offset + D.vstride * e->nvar > a0)
+            return fail(ALSVINN_HIP_E_INVALID, "arena range overlaps slot " + std::to_string(s));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    e->desc[slot] = D;
+    e->spec[slot] = *p;
+    e->assigned[slot] = 1;
+    e->max_vstride = 0;
+    for (int s = 0; s < e->nsamples; ++s)
+        if (e->assigned[s]) e->max_vstride = std::max(e->max_vstride, e->desc[s].vstride);
+    const SampleState S = fresh_state(*p);
+    const unsigned long long z[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpy(e->d_desc + slot, &D, sizeof(D), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_state + slot, &S, sizeof(S), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_ws + 3 * slot, z, sizeof(z), hipMemcpyHostToDevice));
+    return upload_tasks(e);
+}
+
+int alsvinn_hip_pool_release(alsvinn_hip_ensemble* e, int slot) {
+    if (!e || slot < 0 || slot >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad slot");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    e->assigned[slot] = 0;
+    SampleState S;
+    std::memset(&S, 0, sizeof(S));
+    SampleDesc D;
+    std::memset(&D, 0, sizeof(D));
+    e->desc[slot] = D;
+    HIP_TRY(hipMemcpy(e->d_state + slot, &S, sizeof(S), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_desc + slot, &D, sizeof(D), hipMemcpyHostToDevice));
+    e->max_vstride = 0;
+    for (int s = 0; s < e->nsamples; ++s)
+        if (e->assigned[s]) e->max_vstride = std::max(e->max_vstride, e->desc[s].vstride);
+    return upload_tasks(e);
+}
+
+int alsvinn_hip_ensemble_states(alsvinn_hip_ensemble* e, double* t, long long* steps, int* active,
+                                int* status) {
+    if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<SampleState> st(e->nsamples);
+    HIP_TRY(hipMemcpy(st.data(), e->d_state, sizeof(SampleState) * st.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->nsamples; ++s) {
+        if (t) t[s] = st[s].t;
+        if (steps) steps[s] = st[s].steps;
+        if (active) active[s] = st[s].active;
+        if (status) status[s] = st[s].status;
+    }
     return 0;
 }
 

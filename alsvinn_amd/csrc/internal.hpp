@@ -97,6 +97,7 @@ struct alsvinn_hip_ensemble {
     size_t esize = 8;
     std::vector<alsvinn_hip::SampleDesc> desc;
     std::vector<alsvinn_hip_sample> spec;
+    std::vector<char> assigned;  // pool slots: 0 = free
     long long total = 0;  // elements per buffer
     void* U = nullptr;
     void* A = nullptr;

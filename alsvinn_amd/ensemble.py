@@ -203,3 +203,80 @@ class Ensemble:
         out = np.zeros((len(self.samples), self.nvar))
         check(_lib.load().alsvinn_hip_ensemble_totals(self.h, _dp(out)))
         return out
+
+
+def first_fit(allocations, arena, size):
+    """Lowest offset where `size` elements fit between the (offset, size)
+    allocations inside [0, arena), or None when no gap is large enough."""
+    pos = 0
+    for off, sz in sorted(allocations):
+        if off - pos >= size:
+            return pos
+        pos = max(pos, off + sz)
+    return pos if arena - pos >= size else None
+
+
+class Pool(Ensemble):
+    """Slots over an HBM arena for continuous batching (C ABI pool).
+
+    Solves are placed first-fit by footprint (elements per state buffer) and
+    released when their result has been read; the other slots keep stepping.
+    """
+
+    def __init__(self, scheme, nslots, dim, arena_elements, stream=None):
+        L = _lib.load()
+        self.scheme = scheme
+        self.samples = [None] * nslots
+        h = ctypes.c_void_p()
+        check(L.alsvinn_hip_pool_create(ctypes.byref(scheme), nslots, dim, arena_elements, stream,
+                                        ctypes.byref(h)))
+        self.h = h
+        self.nvar = 1 if scheme.equation == _lib.EQ_BURGERS else scheme.equation + 2
+        self.ghosts = L.alsvinn_hip_ghost_cells(ctypes.byref(scheme))
+        self.arena = arena_elements
+        self.allocs = {}  # slot -> (offset, size)
+
+    def footprint(self, n):
+        g = self.ghosts
+        nx, ny, nz = n
+        return self.nvar * (nx + 2 * g) * (ny + (2 * g if ny > 1 else 0)) * (nz + (2 * g if nz > 1 else 0))
+
+    def place(self, size):
+        return first_fit(self.allocs.values(), self.arena, size)
+
+    def free_slot(self):
+        for s in range(len(self.samples)):
+            if s not in self.allocs:
+                return s
+        return None
+
+    def assign(self, sample):
+        """Places a solve; returns its slot or None (no slot / no memory)."""
+        slot = self.free_slot()
+        if slot is None:
+            return None
+        size = self.footprint((sample.nx, sample.ny, sample.nz))
+        off = self.place(size)
+        if off is None:
+            return None
+        check(_lib.load().alsvinn_hip_pool_assign(self.h, slot, ctypes.byref(sample), off))
+        self.allocs[slot] = (off, size)
+        self.samples[slot] = sample
+        return slot
+
+    def release(self, slot):
+        check(_lib.load().alsvinn_hip_pool_release(self.h, slot))
+        self.allocs.pop(slot, None)
+        self.samples[slot] = None
+
+    def states(self):
+        n = len(self.samples)
+        t = np.zeros(n)
+        steps = np.zeros(n, dtype=np.int64)
+        act = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        check(_lib.load().alsvinn_hip_ensemble_states(
+            self.h, _dp(t), steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+            act.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return t, steps, act, st

@@ -92,6 +92,23 @@ int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
                                 alsvinn_hip_ensemble** out);
 int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e);
 
+/* Pools for continuous batching (no counterpart in the reference, whose
+ * Runner creates one Simulator per sample): nslots sample slots over an
+ * arena of arena_elements state elements (per buffer). A slot is assigned a
+ * solve at an arena offset (the caller places solves, e.g. first-fit by
+ * footprint nvar*(nx+2g)*(ny+2g)*(nz+2g)) and released when its result was
+ * read; other slots keep stepping. Unassigned slots cost nothing. */
+int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int dim,
+                            long long arena_elements, void* stream, alsvinn_hip_ensemble** out);
+int alsvinn_hip_pool_assign(alsvinn_hip_ensemble* e, int slot, const alsvinn_hip_sample* sample,
+                            long long offset);
+int alsvinn_hip_pool_release(alsvinn_hip_ensemble* e, int slot);
+
+/* Per-sample t, steps, active flag and status of every sample/slot in one
+ * copy. Synchronous. */
+int alsvinn_hip_ensemble_states(alsvinn_hip_ensemble* e, double* t, long long* steps, int* active,
+                                int* status);
+
 /* Padded extents (tx,ty,tz), ghost layers (gx,gy,gz) and the element offset of
  * a sample's volume in the state buffer. */
 int alsvinn_hip_ensemble_layout(const alsvinn_hip_ensemble* e, int sample, int dims6[6],

@@ -105,3 +105,29 @@ def test_riemann_and_fbm_initial_data():
                      / np.var(ic.fbm_rmd(4096, H, seed=s)[16:] - ic.fbm_rmd(4096, H, seed=s)[:-16])
                      for s in range(8)])
         assert lo < b < hi, (H, b)
+
+
+def test_first_fit_arena_placement():
+    """Pool placement: lowest gap that fits, None when the arena is full."""
+    from alsvinn_amd.ensemble import first_fit
+    assert first_fit([], 100, 30) == 0
+    assert first_fit([(0, 30)], 100, 30) == 30
+    assert first_fit([(0, 30), (60, 40)], 100, 30) == 30
+    assert first_fit([(0, 30), (60, 40)], 100, 31) is None
+    assert first_fit([(10, 10)], 100, 10) == 0
+    assert first_fit([(0, 50), (50, 50)], 100, 1) is None
+
+
+def test_poisson_request_stream():
+    from alsvinn_amd import serving
+    rs = serving.make_requests(200, rate=10.0, seed=1, t_range=(0.1, 1.0))
+    arr = np.array([r.arrival for r in rs])
+    assert np.all(np.diff(arr) > 0)
+    gaps = np.diff(np.concatenate([[0.0], arr]))
+    assert abs(gaps.mean() - 0.1) < 0.03
+    for r in rs:
+        assert 0.1 <= r.end_time <= 1.0
+        if r.kind == "burgers":
+            assert 4096 <= r.n <= 65536 and 0.25 <= r.hurst <= 0.75
+        else:
+            assert 256 <= r.n <= 1024 and r.states.shape == (4, 4)
