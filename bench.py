@@ -43,7 +43,10 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--extra-steps", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["all", "main", "fe-calibration"], default="all")
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--serving-tmin", type=float, default=0.1)
+    ap.add_argument("--serving-tmax", type=float, default=1.0)
+    ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving"], default="all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -150,6 +153,34 @@ def fp32_error(hip, M, n, X, steps):
     return out
 
 
+def serving_bench(args):
+    """Capacity from a closed batch (every request at t = 0), then a Poisson
+    stream at 80% of it; p50/p99 latency (arrival -> result on the host)."""
+    from alsvinn_amd import serving
+    t_range = (args.serving_tmin, args.serving_tmax)
+    cfg = serving.ServerConfig()
+    cfg.steps_per_tick = 50
+    cfg.keep_results = False
+    closed = serving.make_requests(args.serving_requests // 2, rate=0.0, seed=11, t_range=t_range)
+    srv = serving.Server(cfg)
+    srv.serve(closed)
+    cap = len(closed) / srv.elapsed
+    srv.close()
+    stream = serving.make_requests(args.serving_requests, rate=0.8 * cap, seed=12, t_range=t_range)
+    srv = serving.Server(cfg)
+    srv.serve(stream)
+    lat = serving.latency_stats(stream)
+    out = {"capacity_req_per_s": cap, "arrival_rate_req_per_s": 0.8 * cap,
+           "requests": len(stream), "latency": lat,
+           "cell_updates_per_s": srv.cell_updates / srv.elapsed, "elapsed_s": srv.elapsed,
+           "end_time_range": list(t_range),
+           "workload": "Poisson stream at 80% of measured capacity: 1D periodic Burgers "
+                       "4096-65536 cells (fBm, ENO2+Godunov, SSP-RK2) and 2D Euler four-quadrant "
+                       "Riemann 256^2-1024^2 (WENO2+HLL3, SSP-RK3, Neumann), FP64"}
+    srv.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -159,6 +190,10 @@ def main():
         sys.exit(2)
     import alsvinn_amd as hip
     from alsvinn_amd import sampling
+
+    if args.only == "serving":
+        print(json.dumps(serving_bench(args)), flush=True)
+        return
 
     if args.only == "fe-calibration":
         # forward-Euler batch: its elementwise kernel reads two volumes and
@@ -281,6 +316,11 @@ def main():
         extra["sod"] = {"solve_ms": 1e3 * (t1 - t0), "steps": info["steps"], "t": info["t"],
                         "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
         log("  sod: %d steps in %.2f ms" % (info["steps"], 1e3 * (t1 - t0)))
+
+    if args.only == "all" and args.serving_requests > 0:
+        log("serving")
+        extra["serving"] = serving_bench(args)
+        log("  %s" % extra["serving"])
 
     cpu = None
     if not args.no_cpu and args.only == "all":
