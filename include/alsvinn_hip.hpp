@@ -1,0 +1,320 @@
+// C++ host layer of the hip platform, over the C ABI of alsvinn_hip.h.
+//
+// The classes mirror the reference interfaces the platform stands behind, with
+// the same names, argument meaning and error behaviour (a failing call throws
+// std::runtime_error, as the reference's THROW / CUDA_SAFE_CALL do):
+//
+//   alsfvm::hip::Simulator        alsfvm/include/alsfvm/simulator/Simulator.hpp
+//                                 (setInitialValue, performStep, atEnd,
+//                                 getCurrentTime, getEndTime; Simulator.cpp:77-214)
+//   alsfvm::hip::NumericalFlux    numflux::NumericalFlux::computeFlux over all
+//                                 regions (NumericalFlux.hpp:44-52, ConservedSystem.cpp:31-103)
+//   alsfvm::hip::Boundary         boundary::Boundary::applyBoundaryConditions (Boundary.hpp:30-31)
+//   alsuq::hip::Runner            alsuq::run::Runner::run (Runner.cpp:39-90): all samples
+//                                 of the rank advance together
+//   alsuq::hip::MeanVariance, StructureCube, StructureBasic
+//                                 alsuq::stats (MeanVariance.cpp:30-67, StructureCube.cpp:37-76,
+//                                 StructureBasic.cpp:67-120)
+//
+// Volumes cross the boundary as host vectors in the reference's layout
+// (variable-major, z*tx*ty + y*tx + x over padded extents).
+#pragma once
+
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "alsvinn_hip.h"
+
+namespace alsvinn_hip_cxx {
+
+inline void check(int rc, const char* what) {
+    if (rc != ALSVINN_HIP_OK)
+        throw std::runtime_error(std::string(what) + ": " + alsvinn_hip_last_error());
+}
+
+struct EnsembleHandle {
+    alsvinn_hip_ensemble* e = nullptr;
+    EnsembleHandle() = default;
+    EnsembleHandle(const EnsembleHandle&) = delete;
+    EnsembleHandle& operator=(const EnsembleHandle&) = delete;
+    ~EnsembleHandle() {
+        if (e) alsvinn_hip_ensemble_destroy(e);
+    }
+};
+
+// Grid of the reference (Grid.cpp:24-41): origin, top, cells per axis.
+struct Grid {
+    double origin[3] = {0, 0, 0};
+    double top[3] = {1, 1, 1};
+    int n[3] = {1, 1, 1};
+};
+
+inline alsvinn_hip_sample make_sample(const Grid& g, double end_time, int number_of_saves,
+                                      long long max_steps = -1) {
+    alsvinn_hip_sample p;
+    std::memset(&p, 0, sizeof(p));
+    p.nx = g.n[0];
+    p.ny = g.n[1];
+    p.nz = g.n[2];
+    for (int k = 0; k < 3; ++k) {
+        p.origin[k] = g.origin[k];
+        p.top[k] = g.top[k];
+    }
+    p.end_time = end_time;
+    // FixedIntervalWriter / FixedIntervalStatistics without the initial
+    // timestep: interval T / numberOfSaves, first save at one interval.
+    p.num_adjusters = number_of_saves > 0 ? 1 : 0;
+    p.interval[0] = number_of_saves > 0 ? end_time / number_of_saves : 0.0;
+    p.saved0[0] = 1;
+    p.max_steps = max_steps;
+    return p;
+}
+
+inline int number_of_variables(const alsvinn_hip_scheme& s) {
+    return s.equation == ALSVINN_HIP_EQ_BURGERS ? 1 : s.equation + 2;
+}
+
+}  // namespace alsvinn_hip_cxx
+
+namespace alsfvm {
+namespace hip {
+
+using alsvinn_hip_cxx::check;
+using alsvinn_hip_cxx::Grid;
+
+// One solve on the device: the hip counterpart of simulator::Simulator.
+class Simulator {
+public:
+    Simulator(const alsvinn_hip_scheme& scheme, const Grid& grid, double endTime,
+              int numberOfSaves = 1)
+        : scheme(scheme), grid(grid), endTime(endTime) {
+        auto p = alsvinn_hip_cxx::make_sample(grid, endTime, numberOfSaves);
+        check(alsvinn_hip_ensemble_create(&scheme, 1, &p, nullptr, &h.e), "Simulator");
+    }
+
+    // Simulator::setInitialValue: conserved volume (padded, reference layout);
+    // applies the boundary conditions (Simulator.cpp:160).
+    void setInitialValue(const std::vector<double>& conserved) {
+        if (conserved.size() != volumeSize())
+            throw std::runtime_error("setInitialValue: wrong volume size");
+        check(alsvinn_hip_ensemble_upload(h.e, 0, conserved.data()), "setInitialValue");
+        check(alsvinn_hip_ensemble_apply_boundary(h.e), "setInitialValue");
+    }
+
+    // Simulator::performStep: one timestep; THROWs on a constraint violation.
+    void performStep() {
+        check(alsvinn_hip_ensemble_step(h.e, 1), "performStep");
+        int status = 0;
+        check(alsvinn_hip_ensemble_sample_info(h.e, 0, &t, &steps, &lastDt, &status, &active),
+              "performStep");
+        if (status != 0)
+            throw std::runtime_error("Simulation state does not obey constraints! At time " +
+                                     std::to_string(t) + ", number of timesteps performed " +
+                                     std::to_string(steps));
+    }
+
+    // Runs to the end time inside the device loop (HIP graph replays).
+    void runToEnd(int stepsPerSync = 32) {
+        check(alsvinn_hip_ensemble_run(h.e, stepsPerSync), "runToEnd");
+        int status = 0;
+        check(alsvinn_hip_ensemble_sample_info(h.e, 0, &t, &steps, &lastDt, &status, &active),
+              "runToEnd");
+    }
+
+    bool atEnd() const { return t >= endTime; }
+    double getCurrentTime() const { return t; }
+    double getEndTime() const { return endTime; }
+    long long getTimestepsPerformed() const { return steps; }
+
+    std::vector<double> getConservedVolume() {
+        std::vector<double> out(volumeSize());
+        check(alsvinn_hip_ensemble_download(h.e, 0, out.data()), "getConservedVolume");
+        return out;
+    }
+
+    size_t volumeSize() const {
+        int d[6];
+        long long off = 0, n = 0;
+        check(alsvinn_hip_ensemble_layout(h.e, 0, d, &off, &n), "layout");
+        return (size_t)n * alsvinn_hip_cxx::number_of_variables(scheme);
+    }
+
+    alsvinn_hip_ensemble* handle() { return h.e; }
+
+private:
+    alsvinn_hip_scheme scheme;
+    Grid grid;
+    double endTime;
+    double t = 0, lastDt = 0;
+    long long steps = 0;
+    int active = 1;
+    alsvinn_hip_cxx::EnsembleHandle h;
+};
+
+// numflux::NumericalFlux on the device: computeFlux over the inner and side
+// regions of a conserved volume (the reference's ConservedSystem), returning
+// the divergence and the per-direction max wave speeds.
+class NumericalFlux {
+public:
+    NumericalFlux(const alsvinn_hip_scheme& scheme, const Grid& grid) : sim(scheme, grid, 1.0, 1) {}
+
+    void computeFlux(const std::vector<double>& conservedVariables, double waveSpeed[3],
+                     bool computeWaveSpeed, std::vector<double>& output) {
+        check(alsvinn_hip_ensemble_upload(sim.handle(), 0, conservedVariables.data()), "computeFlux");
+        output.resize(conservedVariables.size());
+        double ws[3];
+        check(alsvinn_hip_ensemble_rhs(sim.handle(), 0, output.data(), ws), "computeFlux");
+        if (computeWaveSpeed)
+            for (int k = 0; k < 3; ++k) waveSpeed[k] = ws[k];
+    }
+
+private:
+    Simulator sim;
+};
+
+// boundary::Boundary on the device.
+class Boundary {
+public:
+    Boundary(const alsvinn_hip_scheme& scheme, const Grid& grid) : sim(scheme, grid, 1.0, 1) {}
+    void applyBoundaryConditions(std::vector<double>& volume) {
+        sim.setInitialValue(volume);
+        volume = sim.getConservedVolume();
+    }
+
+private:
+    Simulator sim;
+};
+
+}  // namespace hip
+}  // namespace alsfvm
+
+namespace alsuq {
+namespace hip {
+
+using alsvinn_hip_cxx::check;
+using alsvinn_hip_cxx::Grid;
+
+// generator::STLMersenne + distribution::Uniform (STLMersenne.cpp:25-85,
+// Uniform.cpp:27-34): one engine shared by the process, draws addressed by
+// (sample, component) and fast-forwarded with discard.
+class STLMersenneUniform {
+public:
+    STLMersenneUniform(long dimension, double lower = 0, double upper = 1)
+        : dimension(dimension), lower(lower), upper(upper) {}
+    double generate(long component, long sample) {
+        if (component >= dimension) throw std::runtime_error("Component given higher than dimension");
+        if (count / dimension < sample) {
+            const long add = sample * dimension - count;
+            engine.discard(add);
+            count += add;
+        }
+        if (count % dimension < component) {
+            const long add = component - count % dimension;
+            engine.discard(add);
+            count += add;
+        }
+        count++;
+        return dist(engine) * (upper - lower) + lower;
+    }
+
+private:
+    std::mt19937_64 engine;
+    long count = 0;
+    std::uniform_real_distribution<double> dist{0.0, 1.0};
+    long dimension;
+    double lower, upper;
+};
+
+// run::Runner: every sample of [sampleStart, sampleStart + samples) is set up
+// from its parameter vector and all advance together to the end time.
+class Runner {
+public:
+    using InitialData = std::function<std::vector<double>(const std::vector<double>& parameters,
+                                                          long sample)>;
+
+    Runner(const alsvinn_hip_scheme& scheme, const Grid& grid, double endTime, long samples,
+           long parameterDimension, InitialData init, long sampleStart = 0)
+        : scheme(scheme), grid(grid), endTime(endTime), samples(samples) {
+        std::vector<alsvinn_hip_sample> specs(samples, alsvinn_hip_cxx::make_sample(grid, endTime, 1));
+        check(alsvinn_hip_ensemble_create(&scheme, (int)samples, specs.data(), nullptr, &h.e),
+              "Runner");
+        STLMersenneUniform gen(parameterDimension);
+        for (long i = 0; i < samples; ++i) {
+            std::vector<double> prm(parameterDimension);
+            for (long c = 0; c < parameterDimension; ++c) prm[c] = gen.generate(c, sampleStart + i);
+            const auto u = init(prm, sampleStart + i);
+            check(alsvinn_hip_ensemble_upload(h.e, (int)i, u.data()), "Runner");
+        }
+        check(alsvinn_hip_ensemble_apply_boundary(h.e), "Runner");
+    }
+
+    void run(int stepsPerSync = 32) { check(alsvinn_hip_ensemble_run(h.e, stepsPerSync), "run"); }
+
+    alsvinn_hip_ensemble* handle() { return h.e; }
+    long getNumberOfSamples() const { return samples; }
+
+    std::vector<double> getSample(int i) {
+        int d[6];
+        long long off = 0, n = 0;
+        check(alsvinn_hip_ensemble_layout(h.e, i, d, &off, &n), "layout");
+        std::vector<double> out((size_t)n * alsvinn_hip_cxx::number_of_variables(scheme));
+        check(alsvinn_hip_ensemble_download(h.e, i, out.data()), "getSample");
+        return out;
+    }
+
+private:
+    alsvinn_hip_scheme scheme;
+    Grid grid;
+    double endTime;
+    long samples;
+    alsvinn_hip_cxx::EnsembleHandle h;
+};
+
+// stats::MeanVariance over all samples of a Runner (combine + finalize:
+// mean = S1/M, variance = S2/M - mean^2).
+class MeanVariance {
+public:
+    explicit MeanVariance(Runner& r) : runner(r) {}
+    void computeStatistics(std::vector<double>& mean, std::vector<double>& variance,
+                           bool welford = false) {
+        std::vector<int> idx(runner.getNumberOfSamples());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        const size_t n = runner.getSample(0).size();
+        mean.resize(n);
+        variance.resize(n);
+        check(alsvinn_hip_stats_meanvar(runner.handle(), idx.data(), (int)idx.size(), welford ? 1 : 0,
+                                        mean.data(), variance.data()),
+              "MeanVariance");
+    }
+
+private:
+    Runner& runner;
+};
+
+// stats::StructureCube (p, numberOfH) over all samples of a Runner.
+class StructureCube {
+public:
+    StructureCube(Runner& r, double p, int numberOfH) : runner(r), p(p), numberOfH(numberOfH) {}
+    std::vector<double> computeStatistics(int nvar) {
+        std::vector<int> idx(runner.getNumberOfSamples());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::vector<double> out((size_t)nvar * numberOfH);
+        check(alsvinn_hip_stats_structure_cube(runner.handle(), idx.data(), (int)idx.size(), numberOfH,
+                                               p, out.data()),
+              "StructureCube");
+        return out;
+    }
+
+private:
+    Runner& runner;
+    double p;
+    int numberOfH;
+};
+
+}  // namespace hip
+}  // namespace alsuq

@@ -1,0 +1,31 @@
+/* C declarations of the CPU oracle (test infrastructure; see alsvinn_oracle.cpp). */
+#ifndef ALSVINN_ORACLE_H
+#define ALSVINN_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_setup {
+    int equation, flux, recon, integrator, boundary;
+    int nx, ny, nz;
+    double origin[3], top[3];
+    double gamma, cfl, end_time;
+    int num_adjusters;
+    double adj_interval[2];
+    long adj_saved0[2];
+    long max_steps;
+} oracle_setup;
+
+int oracle_ghost_cells(const oracle_setup* s);
+int oracle_run(const oracle_setup* setup, const double* u0, double* uout, double* t_out,
+               long* steps_out, double* dt_hist, long dt_cap);
+int oracle_system_rhs(const oracle_setup* setup, const double* u, double* out, double* ws3);
+int oracle_apply_boundary(const oracle_setup* setup, double* u);
+void oracle_meanvar(long nsamples, long len, const double* samples, double* mean, double* var);
+void oracle_stl_mersenne_uniform(long sample_start, long nsamples, long dim, double lower,
+                                 double upper, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
