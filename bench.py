@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--serving-tmin", type=float, default=0.1)
     ap.add_argument("--serving-tmax", type=float, default=1.0)
-    ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving"], default="all")
+    ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving", "full-uq"], default="all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -153,6 +153,69 @@ def fp32_error(hip, M, n, X, steps):
     return out
 
 
+def _pair_error(hip, scheme_kw, samples, u0s, steps=None):
+    """Runs the same solves with FP64 and FP32 state; relative L1 of rho (var 0)
+    of the FP32 result against the FP64 one (which is bit-identical to the
+    oracle), and of the ensemble mean/variance when there are several samples."""
+    out = {}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        sc = hip.make_scheme(**scheme_kw, precision=hip.FP64 if prec == "fp64" else hip.FP32)
+        e = hip.Ensemble(sc, samples)
+        for i, u in enumerate(u0s):
+            e.upload(i, u)
+        e.apply_boundary()
+        if steps:
+            e.run_steps(steps, PER_GRAPH)
+        else:
+            e.run(32)
+        res[prec] = ([e.download(i) for i in range(len(samples))],
+                     e.meanvar() if len(samples) > 1 else None, e.info(0))
+        e.close()
+    a, b = res["fp64"][0], res["fp32"][0]
+    out["rho_rel_l1"] = float(np.mean([np.abs(x[0] - y[0]).sum() / np.abs(x[0]).sum()
+                                       for x, y in zip(a, b)]))
+    if res["fp64"][1] is not None:
+        (m64, v64), (m32, v32) = res["fp64"][1], res["fp32"][1]
+        out["mean_rho_rel_l1"] = float(np.abs(m64[0] - m32[0]).sum() / np.abs(m64[0]).sum())
+        out["var_rho_rel_l1"] = float(np.abs(v64[0] - v32[0]).sum() / max(np.abs(v64[0]).sum(), 1e-300))
+    out["steps_fp64"] = res["fp64"][2]["steps"]
+    out["steps_fp32"] = res["fp32"][2]["steps"]
+    return out
+
+
+def fp32_errors(hip):
+    """FP32-state error against FP64 for each configuration (sizes reduced
+    where the full one would take long; stated per entry)."""
+    from alsvinn_amd import initial_data as ic, sampling
+    out = {}
+    kw = dict(equation=hip.EQ_EULER1, flux=hip.FLUX_HLL, reconstruction=hip.RECON_NONE,
+              integrator=hip.INT_FE, boundary=hip.BC_NEUMANN, cfl=0.9)
+    out["sod_1024_T0.2"] = _pair_error(hip, kw, [hip.make_sample((1024, 1, 1), (-1, 0, 0), (1, 0, 0), 0.2)],
+                                       [ic.sod(1024)])
+    kw2 = dict(equation=hip.EQ_EULER2, flux=hip.FLUX_HLL3, reconstruction=hip.RECON_WENO2,
+               integrator=hip.INT_RK3, boundary=hip.BC_PERIODIC, cfl=0.475)
+    X = sampling.stl_mersenne_uniform(8, 40)
+    out["kh_1024_100steps"] = _pair_error(hip, kw2, [hip.make_sample((1024, 1024, 1), (0, 0, 0), (1, 1, 0), 2.0)],
+                                          [ic.kelvin_helmholtz_2d(1024, X[0])], steps=100)
+    out["kh_128x8_T2"] = _pair_error(hip, kw2, [hip.make_sample((128, 128, 1), (0, 0, 0), (1, 1, 0), 2.0)
+                                                for _ in range(8)],
+                                     [ic.kelvin_helmholtz_2d(128, X[i]) for i in range(8)])
+    kw3 = dict(kw2, equation=hip.EQ_EULER3)
+    X3 = sampling.stl_mersenne_uniform(2, 400)
+    out["kh3d_64x2_40steps"] = _pair_error(hip, kw3, [hip.make_sample((64, 64, 64), (0, 0, 0), (1, 1, 1), 2.0)
+                                                      for _ in range(2)],
+                                           [ic.kelvin_helmholtz_3d(64, X3[i]) for i in range(2)], steps=40)
+    kwb = dict(equation=hip.EQ_BURGERS, flux=hip.FLUX_GODUNOV, reconstruction=hip.RECON_ENO2,
+               integrator=hip.INT_RK2, boundary=hip.BC_PERIODIC, cfl=0.475)
+    out["burgers_16384_T0.1"] = _pair_error(hip, kwb, [hip.make_sample((16384, 1, 1), (0, 0, 0), (1, 0, 0), 0.1)],
+                                            [ic.burgers_fbm(16384, 0.5, 3)])
+    kwr = dict(kw2, boundary=hip.BC_NEUMANN)
+    out["riemann4_256_T0.2"] = _pair_error(hip, kwr, [hip.make_sample((256, 256, 1), (0, 0, 0), (1, 1, 0), 0.2)],
+                                           [ic.riemann_four_quadrant(256, ic.riemann_parameters(X[1][:16]))])
+    return out
+
+
 def serving_bench(args):
     """Capacity from a closed batch (every request at t = 0), then a Poisson
     stream at 80% of it; p50/p99 latency (arrival -> result on the host)."""
@@ -193,6 +256,34 @@ def main():
 
     if args.only == "serving":
         print(json.dumps(serving_bench(args)), flush=True)
+        return
+
+    if args.only == "full-uq":
+        # the whole Monte Carlo run: every sample to T = 2, then the statistics
+        M, n = args.samples, args.n
+        X = sampling.stl_mersenne_uniform(M, 40)
+        t0 = time.perf_counter()
+        e = make_batch(hip, [n] * M, args.precision, X)
+        t1 = time.perf_counter()
+        sched = e.autotune()
+        t2 = time.perf_counter()
+        e.run(50)
+        t3 = time.perf_counter()
+        mean, var = e.meanvar()
+        sf = e.structure_cube(33, 2.0)
+        t4 = time.perf_counter()
+        steps = np.array([e.info(i)["steps"] for i in range(M)])
+        e.close()
+        updates = float(n * n * steps.sum())
+        print(json.dumps({"workload": "full UQ run KH %d^2 x %d to T=2, %s" % (n, M, args.precision),
+                          "setup_s": t1 - t0, "autotune_s": t2 - t1, "solve_s": t3 - t2,
+                          "statistics_s": t4 - t3,
+                          "steps_min_max": [int(steps.min()), int(steps.max())],
+                          "cell_updates_per_s": updates / (t3 - t2),
+                          "mean_rho_range": [float(mean[0].min()), float(mean[0].max())],
+                          "var_rho_max": float(var[0].max()),
+                          "structure_cube_rho_h1_h32": [float(sf[0, 1]), float(sf[0, 32])],
+                          "schedule": sched}), flush=True)
         return
 
     if args.only == "fe-calibration":
@@ -316,6 +407,11 @@ def main():
         extra["sod"] = {"solve_ms": 1e3 * (t1 - t0), "steps": info["steps"], "t": info["t"],
                         "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
         log("  sod: %d steps in %.2f ms" % (info["steps"], 1e3 * (t1 - t0)))
+
+    if args.only == "all":
+        log("fp32 errors per configuration")
+        extra["fp32_vs_fp64"] = fp32_errors(hip)
+        log("  %s" % extra["fp32_vs_fp64"])
 
     if args.only == "all" and args.serving_requests > 0:
         log("serving")

@@ -148,3 +148,39 @@ def test_full_size_kh_conserves_totals(hip):
         assert all(e.info(i)["steps"] == 20 and e.info(i)["status"] == 0 for i in (0, M - 1))
     scale = np.abs(t0).max(axis=0) + 1.0
     assert np.all(np.abs(t1 - t0) <= 1e-12 * scale * n * n / 1000)
+
+
+def test_full_time_uq_run_bit_exact(hip):
+    """The whole UQ run to T = 2 (reference sampler, KH WENO2+HLL3 SSP-RK3,
+    periodic) on 48^2 x 4 samples: every final state, the step counts, meanvar
+    (bit for bit) and structure_cube(H=9, p=2) (1e-12) against the oracle's
+    sequential sample loop."""
+    _case("full T=2 uq 48^2 x 4")
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    n, M, T = 48, 4, 2.0
+    X = sampling.stl_mersenne_uniform(M, 40)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    u0 = [ic.kelvin_helmholtz_2d(n, X[i]) for i in range(M)]
+    with hip.Ensemble(scheme, [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), T)
+                               for _ in range(M)]) as e:
+        for i in range(M):
+            e.upload(i, u0[i])
+        e.apply_boundary()
+        e.autotune()
+        e.run(32)
+        finals = [e.download(i) for i in range(M)]
+        steps = [e.info(i)["steps"] for i in range(M)]
+        mean, var = e.meanvar()
+        sf = e.structure_cube(9, 2.0)
+    refs = []
+    for i in range(M):
+        out, st = _oracle_final(n, u0[i], T)
+        assert st == steps[i]
+        assert np.array_equal(out.view(np.uint64), finals[i].view(np.uint64)), i
+        refs.append(out)
+    mr, vr = O.meanvar(np.stack(refs))
+    assert np.array_equal(mean.view(np.uint64), mr.view(np.uint64))
+    assert np.array_equal(var.view(np.uint64), vr.view(np.uint64))
+    sref = sum(O.structure_cube(r, (n, n, 1), 2, 9, 2.0, True) for r in refs) / M
+    np.testing.assert_allclose(sf, sref, rtol=RTOL_REDUCTION, atol=0)
