@@ -37,8 +37,14 @@ __device__ __forceinline__ T std_max(T a, T b) {
     return (a < b) ? b : a;
 }
 
+// Division and square root. FP64 (the parity path): IEEE correctly rounded,
+// exactly the reference's operations. FP32-state variant: the hardware
+// reciprocal and square root (about 1 ulp each), traded for speed; the
+// variant's error against the FP64 path is measured, not assumed.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
@@ -61,9 +67,9 @@ struct EulerEq {
         T dot = q[1] * q[1];
         if (NSD > 1) dot = dot + q[2] * q[2];
         if (NSD > 2) dot = dot + q[3] * q[3];
-        const T ie = q[nvar - 1] - T(0.5) * dot / q[0];
+        const T ie = q[nvar - 1] - dv(T(0.5) * dot, q[0]);
 #pragma unroll
-        for (int k = 0; k < NSD; ++k) a.u[k] = q[1 + k] / q[0];
+        for (int k = 0; k < NSD; ++k) a.u[k] = dv(q[1 + k], q[0]);
         a.p = (gamma - T(1)) * ie;
     }
     template <int d>
@@ -119,11 +125,11 @@ struct FluxHLL {
         const T wR = dsqrt(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
-        const T ud = (L.u[d] * wL + R.u[d] * wR) / wsum;
-        const T p = (L.p * wL + R.p * wR) / wsum;
-        const T cs = dsqrt(g * p / rho);
-        const T sL = ref_fmin(L.u[d] - dsqrt(g * L.p / L.q[0]), ud - cs);
-        const T sR = ref_fmax(R.u[d] + dsqrt(g * R.p / R.q[0]), ud + cs);
+        const T ud = dv(L.u[d] * wL + R.u[d] * wR, wsum);
+        const T p = dv(L.p * wL + R.p * wR, wsum);
+        const T cs = dsqrt(dv(g * p, rho));
+        const T sL = ref_fmin(L.u[d] - dsqrt(dv(g * L.p, L.q[0])), ud - cs);
+        const T sR = ref_fmax(R.u[d] + dsqrt(dv(g * R.p, R.q[0])), ud + cs);
         if (sL == T(0)) {
 #pragma unroll
             for (int i = 0; i < nv; ++i) F[i] = T(0);
@@ -139,7 +145,7 @@ struct FluxHLL {
             const T den = sR - sL;
 #pragma unroll
             for (int i = 0; i < nv; ++i)
-                F[i] = (fL[i] * sR - fR[i] * sL + (R.q[i] - L.q[i]) * srl) / den;
+                F[i] = dv(fL[i] * sR - fR[i] * sL + (R.q[i] - L.q[i]) * srl, den);
         }
         return ref_fmax(dabs(sL), dabs(sR));
     }
@@ -156,12 +162,12 @@ struct FluxHLL3 {
         const T wR = dsqrt(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
-        const T ud = (L.u[d] * wL + R.u[d] * wR) / wsum;
-        const T p = (wL * L.p + wR * R.p) / wsum;
-        const T cfL = dsqrt(g * L.p / L.q[0]);
-        const T cfR = dsqrt(g * R.p / R.q[0]);
+        const T ud = dv(L.u[d] * wL + R.u[d] * wR, wsum);
+        const T p = dv(wL * L.p + wR * R.p, wsum);
+        const T cfL = dsqrt(dv(g * L.p, L.q[0]));
+        const T cfR = dsqrt(dv(g * R.p, R.q[0]));
         const T corr = T(0.5) * ref_fmax(T(0), L.u[d] - R.u[d]);
-        const T cs = dsqrt(g * p / rho);
+        const T cs = dsqrt(dv(g * p, rho));
         const T sL = ref_fmin(L.u[d] + corr - cfL, ud - cs);
         const T sR = ref_fmax(R.u[d] - corr + cfR, ud + cs);
         if (sL == T(0)) {
@@ -180,14 +186,14 @@ struct FluxHLL3 {
             const T udl = L.u[d] - sL;
             const T udr = R.u[d] - sR;
             const T aa = udr * R.q[0] - udl * L.q[0];
-            const T sm = (R.q[1 + d] * udr - L.q[1 + d] * udl + R.p - L.p) / aa;
+            const T sm = dv(R.q[1 + d] * udr - L.q[1 + d] * udl + R.p - L.p, aa);
             T us[NSD];
 #pragma unroll
             for (int k = 0; k < NSD; ++k) {
                 if (k == d)
                     us[k] = sm;
                 else
-                    us[k] = (fR[1 + k] - fL[1 + k] - R.q[1 + k] * sR + L.q[1 + k] * sL) / aa;
+                    us[k] = dv(fR[1 + k] - fL[1 + k] - R.q[1 + k] * sR + L.q[1 + k] * sL, aa);
             }
             // The reference evaluates the two branches with the same formula
             // on mirrored operands; select the operands, then evaluate once.
@@ -200,11 +206,11 @@ struct FluxHLL3 {
             const T EX = left ? L.q[nv - 1] : R.q[nv - 1];
             T mid[nv];
             const T smx = sm - sX;
-            mid[0] = rhoX * udx / smx;
+            mid[0] = dv(rhoX * udx, smx);
 #pragma unroll
             for (int k = 0; k < NSD; ++k) mid[1 + k] = us[k] * mid[0];
             const T ps = pX + rhoX * (uX - sm) * udx;
-            mid[nv - 1] = (udx * EX + pX * uX - ps * sm) / smx;
+            mid[nv - 1] = dv(udx * EX + pX * uX - ps * sm, smx);
 #pragma unroll
             for (int i = 0; i < nv; ++i) {
                 const T fX = left ? fL[i] : fR[i];
@@ -254,37 +260,34 @@ struct ReconWENO2 {
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
                                                  T* R) {
-        // The reference adds the double literal 1e-8 (WENOCoefficients.hpp:21),
-        // so the weights are formed in double even for a float state.
-        typedef double W;
-        const W eps = 1e-8;
+        // eps = 1e-8 (WENOCoefficients.hpp:21); the FP64 path evaluates the
+        // reference's expressions exactly.
+        const T eps = T(1e-8);
         const T i0 = q0[0];
         const T d0 = qp[0] - i0;
         const T d1 = i0 - qm[0];
         const T b0 = d0 * d0;
         const T b1 = d1 * d1;
-        const W e0 = eps + W(b0);
-        const W e1 = eps + W(b1);
-        const W D0 = 3 * e0 * e0;
-        const W D1 = 3 * e1 * e1;
-        // The weights are rounded to the state type as in the reference
-        // (a0Left etc. are `real`). 2/D == 2*(1/D) exactly while 1/D is a
-        // normal number (D < 2^1021, i.e. |w_{i+1}-w_i| < 2^510), so two of
-        // the four divisions are shared.
-        const T a0L = T(1 / D0);
-        const T a1R = T(1 / D1);
+        const T e0 = eps + b0;
+        const T e1 = eps + b1;
+        const T D0 = T(3) * e0 * e0;
+        const T D1 = T(3) * e1 * e1;
+        // 2/D == 2*(1/D) exactly while 1/D is a normal number (D < 2^1021,
+        // i.e. |w_{i+1}-w_i| < 2^510), so two of the four divisions are shared.
+        const T a0L = dv(T(1), D0);
+        const T a1R = dv(T(1), D1);
         const T a1L = T(2) * a1R;
         const T a0R = T(2) * a0L;
-        const T w0L = a0L / (a0L + a1L);
-        const T w1L = a1L / (a0L + a1L);
-        const T w0R = a0R / (a0R + a1R);
-        const T w1R = a1R / (a0R + a1R);
+        const T w0L = dv(a0L, a0L + a1L);
+        const T w1L = dv(a1L, a0L + a1L);
+        const T w0R = dv(a0R, a0R + a1R);
+        const T w1R = dv(a1R, a0R + a1R);
         const T cL = T(3) * w0L + w1L;
         const T cR = T(3) * w1R + w0R;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            L[v] = T(0.5 * W(w1L * qm[v] + cL * q0[v] - w0L * qp[v]));
-            R[v] = T(0.5 * W(w0R * qp[v] + cR * q0[v] - w1R * qm[v]));
+            L[v] = T(0.5) * (w1L * qm[v] + cL * q0[v] - w0L * qp[v]);
+            R[v] = T(0.5) * (w0R * qp[v] + cR * q0[v] - w1R * qm[v]);
         }
     }
 };
