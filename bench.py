@@ -203,9 +203,9 @@ def fp32_errors(hip):
                                      [ic.kelvin_helmholtz_2d(128, X[i]) for i in range(8)])
     kw3 = dict(kw2, equation=hip.EQ_EULER3)
     X3 = sampling.stl_mersenne_uniform(2, 400)
-    out["kh3d_64x2_40steps"] = _pair_error(hip, kw3, [hip.make_sample((64, 64, 64), (0, 0, 0), (1, 1, 1), 2.0)
-                                                      for _ in range(2)],
-                                           [ic.kelvin_helmholtz_3d(64, X3[i]) for i in range(2)], steps=40)
+    out["kh3d_128x2_40steps"] = _pair_error(hip, kw3, [hip.make_sample((128, 128, 128), (0, 0, 0), (1, 1, 1), 2.0)
+                                                       for _ in range(2)],
+                                            [ic.kelvin_helmholtz_3d(128, X3[i]) for i in range(2)], steps=40)
     kwb = dict(equation=hip.EQ_BURGERS, flux=hip.FLUX_GODUNOV, reconstruction=hip.RECON_ENO2,
                integrator=hip.INT_RK2, boundary=hip.BC_PERIODIC, cfl=0.475)
     out["burgers_16384_T0.1"] = _pair_error(hip, kwb, [hip.make_sample((16384, 1, 1), (0, 0, 0), (1, 0, 0), 0.1)],
