@@ -318,6 +318,14 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             T qn[NV];
             // next row first: its latency overlaps the x sweep
             if constexpr (DIM >= 2) load_cell<T, NV>(in, in2, a.lazy, c, row_index(pr + 2), vs, qn);
+            // U^n of this row for the stage combination, also issued before
+            // the sweep (loaded after it, its latency would stall the wave)
+            const long long i0 = (long long)pr * D.tx + px;
+            T u0[NV];
+            if (a.mode != MODE_RHS && interior) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+            }
             T out_v[NV];
             // ---- x direction on row r (first: out = 0 + x divergence)
             {
@@ -357,12 +365,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 }
             }
             if (interior) {
-                const long long i0 = (long long)pr * D.tx + px;
-                T u0[NV], res[NV];
-                if (a.mode != MODE_RHS) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
-                }
+                T res[NV];
                 combine<T, NV>(a.mode, c, out_v, yc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
                 store_cell<T, NV, DIM>(out, vs, D, col, r, 0, res);
@@ -461,6 +464,12 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             const int pz = zz + D.gz;
             T qn[NV], ym2[NV], ym1[NV], yp1[NV], yp2[NV];
             load_cell<T, NV>(in, in2, a.lazy, c, idx(py, pz + 2), vs, qn);
+            const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
+            T u0[NV];
+            if (a.mode != MODE_RHS && interior) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+            }
             load_cell<T, NV>(in, in2, a.lazy, c, idx(py - 2, pz), vs, ym2);
             load_cell<T, NV>(in, in2, a.lazy, c, idx(py - 1, pz), vs, ym1);
             load_cell<T, NV>(in, in2, a.lazy, c, idx(py + 1, pz), vs, yp1);
@@ -513,12 +522,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
                 }
             }
             if (interior) {
-                const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
-                T u0[NV], res[NV];
-                if (a.mode != MODE_RHS) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
-                }
+                T res[NV];
                 combine<T, NV>(a.mode, c, out_v, zc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
                 store_cell<T, NV, 3>(out, vs, D, col, row, zz, res);

@@ -136,9 +136,10 @@ int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int var
 int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
                                       int* ntasks);
 
-/* Autotuner: times the first-stage kernel on the current state for every
- * variant and a ladder of band heights, keeps the fastest schedule.
- * Synchronous; leaves the conserved state untouched. */
+/* Autotuner: times a dry run of the step's stage kernels (all three for
+ * SSP-RK3, the first otherwise; outputs into scratch volumes) on the current
+ * state for every variant and a ladder of band heights, keeps the fastest
+ * schedule. Synchronous; leaves the conserved state untouched. */
 int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* variant);
 
 /* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
