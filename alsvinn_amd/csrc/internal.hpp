@@ -27,14 +27,26 @@ struct KernelSet {
                  double gamma) = nullptr;
 };
 
+// The lazy stage input (second SSP-RK3 stage) is a separate instantiation, so
+// that the other stages carry no code for it.
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
 void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
-    if constexpr (DIM == 3)
-        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS>), dim3(blocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS>), dim3(blocks), dim3(256), 0,
-                           s, a);
+    if constexpr (DIM == 3) {
+        if (a.lazy)
+            hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, true>), dim3(blocks), dim3(256), 0,
+                               s, a);
+        else
+            hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, false>), dim3(blocks), dim3(256),
+                               0, s, a);
+    } else {
+        if (a.lazy)
+            hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, true>), dim3(blocks),
+                               dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, false>), dim3(blocks),
+                               dim3(256), 0, s, a);
+    }
 }
 
 template <class T, class Eq, class Recon, class Flux, int DIM>
