@@ -287,7 +287,6 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
 
     // stage 1: A = L(U), wave speeds
     a.mode = MODE_RHS;
-    a.lazy = 0;
     a.check = 0;
     a.in = U;
     a.in2 = U;
@@ -315,13 +314,11 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
         stage();
     } else {
         a.mode = MODE_RK3_1;
-        a.lazy = 1;
         a.in = A;
         a.in2 = U;
         a.out = B;
         stage();
         a.mode = MODE_RK3_2;
-        a.lazy = 0;
         a.in = B;
         a.in2 = U;
         a.out = U;
@@ -405,20 +402,17 @@ double time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps) {
     const bool rk3 = e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK3;
     auto launch_all = [&]() {
         a.mode = MODE_RHS;
-        a.lazy = 0;
         a.in = U;
         a.in2 = U;
         a.out = A;
         k.stage[e->variant](a, blocks, e->stream);
         if (!rk3) return;
         a.mode = MODE_RK3_1;
-        a.lazy = 1;
         a.in = A;
         a.in2 = U;
         a.out = B;
         k.stage[e->variant](a, blocks, e->stream);
         a.mode = MODE_RK3_2;
-        a.lazy = 0;
         a.in = B;
         a.in2 = U;
         a.out = A;

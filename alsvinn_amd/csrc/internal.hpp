@@ -27,25 +27,28 @@ struct KernelSet {
                  double gamma) = nullptr;
 };
 
-// The lazy stage input (second SSP-RK3 stage) is a separate instantiation, so
-// that the other stages carry no code for it.
+// Each stage kind (first stage, RK2 second stage, SSP-RK3 second and third
+// stages) is its own instantiation: the combination, the lazy stage input of
+// the second SSP-RK3 stage and the wave-speed reduction are resolved at
+// compile time instead of by branches in the marching loop.
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
+void launch_stage_mode(const StageArgs<T>& a, int blocks, hipStream_t s) {
+    if constexpr (DIM == 3)
+        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, MODE>), dim3(blocks), dim3(256), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, MODE>), dim3(blocks), dim3(256),
+                           0, s, a);
+}
+
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
 void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
-    if constexpr (DIM == 3) {
-        if (a.lazy)
-            hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, true>), dim3(blocks), dim3(256), 0,
-                               s, a);
-        else
-            hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, false>), dim3(blocks), dim3(256),
-                               0, s, a);
-    } else {
-        if (a.lazy)
-            hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, true>), dim3(blocks),
-                               dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, false>), dim3(blocks),
-                               dim3(256), 0, s, a);
+    switch (a.mode) {
+        case MODE_RHS: launch_stage_mode<T, Eq, Recon, Flux, DIM, WPS, MODE_RHS>(a, blocks, s); break;
+        case MODE_RK2_1: launch_stage_mode<T, Eq, Recon, Flux, DIM, WPS, MODE_RK2_1>(a, blocks, s); break;
+        case MODE_RK3_1: launch_stage_mode<T, Eq, Recon, Flux, DIM, WPS, MODE_RK3_1>(a, blocks, s); break;
+        case MODE_RK3_2: launch_stage_mode<T, Eq, Recon, Flux, DIM, WPS, MODE_RK3_2>(a, blocks, s); break;
     }
 }
 

@@ -77,8 +77,7 @@ struct StageArgs {
     unsigned long long* ws;  // 3 per sample: bit patterns of non-negative doubles
     const WaveTask* tasks;
     int ntasks;
-    int mode;
-    int lazy;   // stage input = in*c + in2 (formed on the fly)
+    int mode;   // StageMode; MODE_RK3_1 forms its stage input as in*c + in2
     int check;  // evaluate the constraint check on the result
     const T* in;
     const T* in2;  // U0
@@ -265,8 +264,9 @@ __device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDes
 // 1D and 2D stage kernel (DIM = active grid dimensions). WPS is the
 // register budget expressed as waves per SIMD (2: no spills for FP64 Euler
 // 2D; 3 and 4 trade spills for occupancy); the autotuner picks one.
-template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, bool LAZY>
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
+    constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     const int lane = threadIdx.x & 63;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         SampleState* Sp = a.state + sample;
         const bool active = in_strip && Sp->active;
         if (!__any(active)) continue;  // wave-uniform
-        const bool need_dt = LAZY || a.mode != MODE_RHS;
+        const bool need_dt = LAZY || MODE != MODE_RHS;
         T c = T(0);
         if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
 
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             // the sweep (loaded after it, its latency would stall the wave)
             const long long i0 = (long long)pr * D.tx + px;
             T u0[NV];
-            if (a.mode != MODE_RHS && interior) {
+            if (MODE != MODE_RHS && interior) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
             }
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             }
             if (interior) {
                 T res[NV];
-                combine<T, NV>(a.mode, c, out_v, yc, u0, res);
+                combine<T, NV>(MODE, c, out_v, yc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
                 store_cell<T, NV, DIM>(out, vs, D, col, r, 0, res);
             }
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             }
         }
 
-        if (a.mode == MODE_RHS && !LAZY) {
+        if (MODE == MODE_RHS && !LAZY) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             if (in_strip && li == 0 && Sp->active) {
@@ -420,8 +420,9 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
 // with a register window (z neighbours); x neighbours come from adjacent
 // lanes, y neighbours from loads of rows y-2..y+2 of the current plane (the
 // y reconstructions of rows y-1 and y+1 are recomputed per row task).
-template <class T, class Eq, class Recon, class Flux, int WPS, bool LAZY>
+template <class T, class Eq, class Recon, class Flux, int WPS, int MODE>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
+    constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     const int lane = threadIdx.x & 63;
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
         SampleState* Sp = a.state + sample;
         const bool active = in_strip && Sp->active;
         if (!__any(active)) continue;
-        const bool need_dt = LAZY || a.mode != MODE_RHS;
+        const bool need_dt = LAZY || MODE != MODE_RHS;
         T c = T(0);
         if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
 
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             pp2.issue(in, in2, LAZY, idx(py + 2, pz), vs);
             const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
             T u0[NV];
-            if (a.mode != MODE_RHS && interior) {
+            if (MODE != MODE_RHS && interior) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
             }
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             }
             if (interior) {
                 T res[NV];
-                combine<T, NV>(a.mode, c, out_v, zc, u0, res);
+                combine<T, NV>(MODE, c, out_v, zc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
                 store_cell<T, NV, 3>(out, vs, D, col, row, zz, res);
             }
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             }
         }
 
-        if (a.mode == MODE_RHS && !LAZY) {
+        if (MODE == MODE_RHS && !LAZY) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             const T mz = segment_max(wsz, seg);
