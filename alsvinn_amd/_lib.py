@@ -25,6 +25,8 @@ EXPORTS = [
     "alsvinn_hip_ghost_cells",
     "alsvinn_hip_ensemble_create",
     "alsvinn_hip_ensemble_destroy",
+    "alsvinn_hip_state_elements",
+    "alsvinn_hip_ensemble_create_bound",
     "alsvinn_hip_pool_create",
     "alsvinn_hip_pool_assign",
     "alsvinn_hip_pool_release",
@@ -110,6 +112,10 @@ def load():
     L.alsvinn_hip_ensemble_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
                                               ctypes.POINTER(Sample), vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_ensemble_destroy.argtypes = [vp]
+    L.alsvinn_hip_state_elements.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
+                                             ctypes.POINTER(Sample), llp]
+    L.alsvinn_hip_ensemble_create_bound.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
+                                                    ctypes.POINTER(Sample), vp, vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_pool_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int,
                                           ctypes.c_longlong, vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_pool_assign.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Sample), ctypes.c_longlong]

@@ -459,10 +459,10 @@ static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* sche
         e->own_stream = true;
     }
     const size_t bytes = (size_t)e->total * e->esize;
-    HIP_TRY_C(hipMalloc(&e->U, bytes));
+    if (e->own_U) HIP_TRY_C(hipMalloc(&e->U, bytes));
     HIP_TRY_C(hipMalloc(&e->A, bytes));
     if (scheme->integrator == ALSVINN_HIP_INTEGRATOR_RK3) HIP_TRY_C(hipMalloc(&e->B, bytes));
-    HIP_TRY_C(hipMemset(e->U, 0, bytes));
+    if (e->own_U) HIP_TRY_C(hipMemset(e->U, 0, bytes));
     HIP_TRY_C(hipMemset(e->A, 0, bytes));
     if (e->B) HIP_TRY_C(hipMemset(e->B, 0, bytes));
     HIP_TRY_C(hipMalloc(&e->d_desc, sizeof(SampleDesc) * nsamples));
@@ -497,12 +497,12 @@ int alsvinn_hip_ghost_cells(const alsvinn_hip_scheme* scheme) {
     return ghost_cells(*scheme);
 }
 
-int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
-                                const alsvinn_hip_sample* samples, void* stream,
-                                alsvinn_hip_ensemble** out) {
+// Descriptors of an ensemble's samples (layout of the concatenated volumes);
+// the caller owns the returned object until finish_create takes it.
+static int describe_samples(const alsvinn_hip_scheme* scheme, int nsamples,
+                            const alsvinn_hip_sample* samples, alsvinn_hip_ensemble** out) {
     if (int rc = validate(scheme)) return rc;
-    if (!out || !samples || nsamples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
-    *out = nullptr;
+    if (!samples || nsamples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
     auto* e = new alsvinn_hip_ensemble();
     e->scheme = *scheme;
     e->nsamples = nsamples;
@@ -535,6 +535,37 @@ int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
         e->assigned.push_back(1);
     }
     e->total = off;
+    *out = e;
+    return 0;
+}
+
+int alsvinn_hip_state_elements(const alsvinn_hip_scheme* scheme, int nsamples,
+                               const alsvinn_hip_sample* samples, long long* elements) {
+    if (!elements) return fail(ALSVINN_HIP_E_INVALID, "elements is NULL");
+    alsvinn_hip_ensemble* e = nullptr;
+    if (int rc = describe_samples(scheme, nsamples, samples, &e)) return rc;
+    *elements = e->total;
+    delete e;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
+                                const alsvinn_hip_sample* samples, void* stream,
+                                alsvinn_hip_ensemble** out) {
+    return alsvinn_hip_ensemble_create_bound(scheme, nsamples, samples, stream, nullptr, out);
+}
+
+int alsvinn_hip_ensemble_create_bound(const alsvinn_hip_scheme* scheme, int nsamples,
+                                      const alsvinn_hip_sample* samples, void* stream, void* state,
+                                      alsvinn_hip_ensemble** out) {
+    if (!out) return fail(ALSVINN_HIP_E_INVALID, "bad ensemble arguments");
+    *out = nullptr;
+    alsvinn_hip_ensemble* e = nullptr;
+    if (int rc = describe_samples(scheme, nsamples, samples, &e)) return rc;
+    if (state) {
+        e->U = state;
+        e->own_U = false;
+    }
     return finish_create(e, scheme, stream, out);
 }
 
@@ -542,7 +573,7 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     if (!e) return 0;
     if (e->stream) hipStreamSynchronize(e->stream);
     destroy_graph(e);
-    hipFree(e->U);
+    if (e->own_U) hipFree(e->U);
     hipFree(e->A);
     hipFree(e->B);
     hipFree(e->d_desc);

@@ -114,7 +114,8 @@ struct alsvinn_hip_ensemble {
     std::vector<alsvinn_hip_sample> spec;
     std::vector<char> assigned;  // pool slots: 0 = free
     long long total = 0;  // elements per buffer
-    void* U = nullptr;
+    void* U = nullptr;  // conserved state (the caller's buffer when own_U is false)
+    bool own_U = true;
     void* A = nullptr;
     void* B = nullptr;
     alsvinn_hip::SampleDesc* d_desc = nullptr;

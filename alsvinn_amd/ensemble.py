@@ -46,15 +46,30 @@ def make_sample(n, origin, top, end_time, saves=(1,), max_steps=-1):
     return p
 
 
+def state_elements(scheme, samples):
+    """Reals of the concatenated sample volumes (the state buffer an
+    Ensemble(..., state=ptr) advances in place)."""
+    samples = list(samples)
+    arr = (_lib.Sample * len(samples))(*samples)
+    n = ctypes.c_longlong()
+    check(_lib.load().alsvinn_hip_state_elements(ctypes.byref(scheme), len(samples), arr,
+                                                 ctypes.byref(n)))
+    return n.value
+
+
 class Ensemble:
-    def __init__(self, scheme, samples, stream=None):
+    """Samples of one scheme advanced together. state: optional device pointer
+    (int) of a caller-owned buffer of state_elements(scheme, samples) reals of
+    the scheme's precision, advanced in place (the library never frees it)."""
+
+    def __init__(self, scheme, samples, stream=None, state=None):
         L = _lib.load()
         self.scheme = scheme
         self.samples = list(samples)
         arr = (_lib.Sample * len(self.samples))(*self.samples)
         h = ctypes.c_void_p()
-        check(L.alsvinn_hip_ensemble_create(ctypes.byref(scheme), len(self.samples), arr,
-                                            stream, ctypes.byref(h)))
+        check(L.alsvinn_hip_ensemble_create_bound(ctypes.byref(scheme), len(self.samples), arr,
+                                                  stream, state, ctypes.byref(h)))
         self.h = h
         self.nvar = 1 if scheme.equation == _lib.EQ_BURGERS else scheme.equation + 2
 

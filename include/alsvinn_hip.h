@@ -92,6 +92,20 @@ int alsvinn_hip_ensemble_create(const alsvinn_hip_scheme* scheme, int nsamples,
                                 alsvinn_hip_ensemble** out);
 int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e);
 
+/* Caller-owned state (the reference's Simulator owns its conserved volumes,
+ * Simulator.cpp:59-63, and hands device pointers to its platform kernels):
+ * alsvinn_hip_state_elements gives the number of reals of the concatenated
+ * sample volumes (each in the reference's SoA layout with ghost layers, in
+ * sample order; offsets from alsvinn_hip_ensemble_layout), and
+ * alsvinn_hip_ensemble_create_bound advances the samples in that device buffer
+ * in place. The buffer must outlive the ensemble; the library never frees it
+ * and does not clear it. Scratch volumes stay library-owned. */
+int alsvinn_hip_state_elements(const alsvinn_hip_scheme* scheme, int nsamples,
+                               const alsvinn_hip_sample* samples, long long* elements);
+int alsvinn_hip_ensemble_create_bound(const alsvinn_hip_scheme* scheme, int nsamples,
+                                      const alsvinn_hip_sample* samples, void* stream, void* state,
+                                      alsvinn_hip_ensemble** out);
+
 /* Pools for continuous batching (no counterpart in the reference, whose
  * Runner creates one Simulator per sample): nslots sample slots over an
  * arena of arena_elements state elements (per buffer). A slot is assigned a

@@ -292,3 +292,46 @@ def test_fp32_state_error_vs_fp64_oracle(hip):
     print("fp32 rho rel L1 vs fp64 oracle: %.3e" % l1)
     assert l1 < 1e-4
     assert np.allclose(dts, dts_ref, rtol=1e-5, atol=0)
+
+
+def test_caller_owned_state_advanced_in_place(hip):
+    """A state buffer owned by the caller (hipMalloc'd through the HIP runtime
+    the library links) is advanced in place and gives the same bits as the
+    oracle and as the library's own download; the buffer stays the caller's."""
+    _print_case("caller-owned state KH 32^2 x 2")
+    import ctypes
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    rt = ctypes.CDLL("libamdhip64.so.7")  # the runtime instance libalsvinn_hip.so loaded
+    rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rt.hipFree.argtypes = [ctypes.c_void_p]
+    n, T = 32, 0.05
+    X = sampling.stl_mersenne_uniform(2, 40)
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC)
+    samples = [hip.make_sample((n, n, 1), (0, 0, 0), (1, 1, 0), T) for _ in range(2)]
+    u0 = [ic.kelvin_helmholtz_2d(n, X[i]) for i in range(2)]
+    total = hip.state_elements(scheme, samples)
+    assert total == 2 * u0[0].size
+    host = np.ascontiguousarray(np.concatenate([u.ravel() for u in u0]))
+    nbytes = host.nbytes
+    ptr = ctypes.c_void_p()
+    assert rt.hipMalloc(ctypes.byref(ptr), nbytes) == 0
+    try:
+        assert rt.hipMemcpy(ptr, host.ctypes.data, nbytes, 1) == 0  # host to device
+        with hip.Ensemble(scheme, samples, state=ptr.value) as e:
+            e.apply_boundary()
+            e.run(8)
+            via_api = [e.download(i) for i in range(2)]
+        got = np.empty_like(host)
+        assert rt.hipMemcpy(got.ctypes.data, ptr, nbytes, 2) == 0  # device to host
+    finally:
+        rt.hipFree(ptr)
+    got = got.reshape(2, *u0[0].shape)
+    for i in range(2):
+        s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                         (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=T)
+        rc, ref, t, steps, _ = O.run(s, u0[i])
+        assert rc == 0
+        assert np.array_equal(ref.view(np.uint64), got[i].view(np.uint64))
+        assert np.array_equal(ref.view(np.uint64), via_api[i].view(np.uint64))
