@@ -217,10 +217,11 @@ void build_tasks(alsvinn_hip_ensemble* e, std::vector<WaveTask>& tasks) {
         if (!e->assigned[s]) continue;
         const SampleDesc& D = e->desc[s];
         if (e->dim == 3) {
-            // one row per task, bands of planes marched in z
+            // kGroupRows rows per task (one workgroup), bands of planes marched in z
             const int tz = e->band_rows > 0 ? std::min(e->band_rows, D.nz) : band_rows(waves, D.nz);
             for (int z0 = 0; z0 < D.nz; z0 += tz)
-                for (int y = 0; y < D.ny; ++y) add_strips(s, y, y + 1, z0, std::min(D.nz, z0 + tz));
+                for (int y = 0; y < D.ny; y += kGroupRows)
+                    add_strips(s, y, std::min(D.ny, y + kGroupRows), z0, std::min(D.nz, z0 + tz));
         } else {
             const int ty = e->dim < 2 ? 1
                            : (e->band_rows > 0 ? std::min(e->band_rows, D.ny) : band_rows(waves, D.ny));

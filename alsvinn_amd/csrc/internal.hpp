@@ -33,9 +33,9 @@ struct KernelSet {
 // compile time instead of by branches in the marching loop.
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
 void launch_stage_mode(const StageArgs<T>& a, int blocks, hipStream_t s) {
-    if constexpr (DIM == 3)
-        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, MODE>), dim3(blocks), dim3(256), 0,
-                           s, a);
+    if constexpr (DIM == 3)  // one workgroup of kGroupRows wavefronts per task
+        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, WPS, MODE>), dim3(a.ntasks),
+                           dim3(64 * kGroupRows), 0, s, a);
     else
         hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, WPS, MODE>), dim3(blocks), dim3(256),
                            0, s, a);

@@ -416,23 +416,34 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     }
 }
 
-// 3D stage kernel. A wavefront owns x-strips of one row y and marches in z
-// with a register window (z neighbours); x neighbours come from adjacent
-// lanes, y neighbours from loads of rows y-2..y+2 of the current plane (the
-// y reconstructions of rows y-1 and y+1 are recomputed per row task).
+// 3D stage kernel. A workgroup of kGroupRows wavefronts owns the x-strips of
+// rows y0 .. y0+kGroupRows-1 (one row per wavefront) and marches in z with a
+// register window (z neighbours); x neighbours come from adjacent lanes. The
+// y direction is shared through LDS, plane by plane: every wavefront writes
+// its row of the stage input (the first and last also the two halo rows on
+// their side), reconstructs its own row in y and publishes the left face
+// value, forms the flux of the face above its row from the published value of
+// the row above and publishes it for that row. Each y face flux and each
+// reconstruction is thus evaluated once (the halo rows' once more per group)
+// instead of per row.
+constexpr int kGroupRows = 4;
+
 template <class T, class Eq, class Recon, class Flux, int WPS, int MODE>
-__global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
+__global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
+    constexpr int NW = kGroupRows;
+    __shared__ T s_rows[NW + 4][NV][64];  // rows y0-2 .. y0+NW+1 of the plane's stage input
+    __shared__ T s_L[NW + 1][NV][64];     // y-left face values of rows y0 .. y0+NW
+    __shared__ T s_G[NW + 1][NV][64];     // y fluxes of the faces above rows y0-1 .. y0+NW-1
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int w = threadIdx.x >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
 
-    for (int w = wave; w < a.ntasks; w += nwaves) {
-        const WaveTask& tk = a.tasks[w];
+    for (int t = blockIdx.x; t < a.ntasks; t += gridDim.x) {  // uniform over the workgroup
+        const WaveTask& tk = a.tasks[t];
         int seg = -1;
         for (int k = 0; k < tk.nseg; ++k)
             if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
@@ -444,16 +455,19 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
         const SampleDesc D = a.desc[sample];
         SampleState* Sp = a.state + sample;
         const bool active = in_strip && Sp->active;
-        if (!__any(active)) continue;
+        // the samples of a task are the same for every wavefront of the group
+        if (!__syncthreads_or(active)) continue;
         const bool need_dt = LAZY || MODE != MODE_RHS;
         T c = T(0);
         if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
 
+        const int row = tk.y0 + w;            // this wavefront's row
+        const bool row_in = row < tk.y1;      // rows past the group end only feed neighbours
         const int col = tk.x0[sk] - G + li;
-        const bool interior = active && li >= G && li < G + ncols;
-        const bool xface = active && li >= G - 1 && li < G + ncols;
+        const bool lanes_in = active && li >= G && li < G + ncols;
+        const bool interior = lanes_in && row_in;
+        const bool xface = active && row_in && li >= G - 1 && li < G + ncols;
         const int px = in_strip ? col + D.gx : D.gx;
-        const int row = tk.y0;  // one interior row per task
         const int py = row + D.gy;
         const long long vs = D.vstride;
         const long long plane_stride = (long long)D.tx * D.ty;
@@ -467,6 +481,10 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             pyy = pyy < 0 ? 0 : (pyy >= D.ty ? D.ty - 1 : pyy);
             return (long long)pz * plane_stride + (long long)pyy * D.tx + px;
         };
+        // halo rows: the first wavefront loads y0-2, y0-1, the last y0+NW, y0+NW+1
+        const bool lo_halo = w == 0;
+        const bool hi_halo = w == NW - 1;
+        const int h0 = lo_halo ? tk.y0 - 2 + D.gy : tk.y0 + NW + D.gy;
 
         T zc[NV], zp[NV], Rz[NV], Hm[NV];
         {
@@ -487,13 +505,13 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
 
         for (int zz = tk.z0; zz < tk.z1; ++zz) {
             const int pz = zz + D.gz;
-            T qn[NV], ym2[NV], ym1[NV], yp1[NV], yp2[NV];
-            PendingCell<T, NV> pq, pm2, pm1, pp1, pp2;
+            T qn[NV];
+            PendingCell<T, NV> pq, ph0, ph1;
             pq.issue(in, in2, LAZY, idx(py, pz + 2), vs);
-            pm2.issue(in, in2, LAZY, idx(py - 2, pz), vs);
-            pm1.issue(in, in2, LAZY, idx(py - 1, pz), vs);
-            pp1.issue(in, in2, LAZY, idx(py + 1, pz), vs);
-            pp2.issue(in, in2, LAZY, idx(py + 2, pz), vs);
+            if (lo_halo || hi_halo) {
+                ph0.issue(in, in2, LAZY, idx(h0, pz), vs);
+                ph1.issue(in, in2, LAZY, idx(h0 + 1, pz), vs);
+            }
             const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
             T u0[NV];
             if (MODE != MODE_RHS && interior) {
@@ -522,23 +540,51 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
                     out_v[v] = T(0) + ((T(-1.0) * Fp[v]) + Fm);
                 }
             }
-            {  // y: faces y-1/2 and y+1/2 from rows y-2..y+2
-                pm2.finish(LAZY, c, ym2);
-                pm1.finish(LAZY, c, ym1);
-                pp1.finish(LAZY, c, yp1);
-                pp2.finish(LAZY, c, yp2);
-                T Lm[NV], Rm[NV], L0[NV], R0[NV], Lp[NV], Rp[NV];
-                Recon::template apply<T, NV>(ym2, ym1, zc, Lm, Rm);
-                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0);
-                Recon::template apply<T, NV>(zc, yp1, yp2, Lp, Rp);
-                T Gm[NV], Gp[NV];
-                const T s0 = face_flux<T, Eq, Flux, 1>(eq, Rm, L0, Gm);
-                const T s1 = face_flux<T, Eq, Flux, 1>(eq, R0, Lp, Gp);
-                if (interior) wsy = std_max(std_max(wsy, s0), s1);
+            // ---- y, phase 1: publish the plane's rows
 #pragma unroll
-                for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+            for (int v = 0; v < NV; ++v) s_rows[w + 2][v][lane] = zc[v];
+            if (lo_halo || hi_halo) {
+                T q0[NV], q1[NV];
+                ph0.finish(LAZY, c, q0);
+                ph1.finish(LAZY, c, q1);
+                const int k0 = lo_halo ? 0 : NW + 2;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    s_rows[k0][v][lane] = q0[v];
+                    s_rows[k0 + 1][v][lane] = q1[v];
+                }
             }
-            {  // z
+            __syncthreads();
+            // ---- y, phase 2: reconstruct the own row (and the halo row on the edge)
+            T L0[NV], R0[NV], Rb[NV];
+            {
+                T ym1[NV], yp1[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    ym1[v] = s_rows[w + 1][v][lane];
+                    yp1[v] = s_rows[w + 3][v][lane];
+                }
+                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s_L[w][v][lane] = L0[v];
+                if (lo_halo) {  // right value of row y0-1
+                    T q0[NV], Lb[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) q0[v] = s_rows[0][v][lane];
+                    Recon::template apply<T, NV>(q0, ym1, zc, Lb, Rb);
+                }
+                if (hi_halo) {  // left value of row y0+NW
+                    T q1[NV], La[NV], Ra[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) q1[v] = s_rows[NW + 3][v][lane];
+                    Recon::template apply<T, NV>(zc, yp1, q1, La, Ra);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) s_L[NW][v][lane] = La[v];
+                }
+            }
+            // ---- z (own row only; placed here to overlap the exchange)
+            T zpart[NV];
+            {
                 pq.finish(LAZY, c, qn);
                 T Ln[NV], Rn[NV];
                 Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn);
@@ -547,10 +593,36 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
                 if (interior) wsz = std_max(wsz, s);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    out_v[v] = out_v[v] + ((T(-1.0) * Hp[v]) + Hm[v]);
+                    zpart[v] = (T(-1.0) * Hp[v]) + Hm[v];
                     Hm[v] = Hp[v];
                     Rz[v] = Rn[v];
                 }
+            }
+            __syncthreads();
+            // ---- y, phase 3: the flux of the face above the own row
+            T Gp[NV], Gm[NV];
+            {
+                T La[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) La[v] = s_L[w + 1][v][lane];
+                const T s1 = face_flux<T, Eq, Flux, 1>(eq, R0, La, Gp);
+                if (interior) wsy = std_max(wsy, s1);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s_G[w + 1][v][lane] = Gp[v];
+                if (lo_halo) {  // the face below row y0
+                    const T s0 = face_flux<T, Eq, Flux, 1>(eq, Rb, L0, Gm);
+                    if (interior) wsy = std_max(wsy, s0);
+                }
+            }
+            __syncthreads();
+            if (!lo_halo) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Gm[v] = s_G[w][v][lane];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+                out_v[v] = out_v[v] + zpart[v];
             }
             if (interior) {
                 T res[NV];
@@ -569,12 +641,13 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_3(StageArgs<T> a) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             const T mz = segment_max(wsz, seg);
-            if (in_strip && li == 0 && Sp->active) {
+            if (in_strip && li == 0 && Sp->active && row_in) {
                 atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
                 atomicMax(a.ws + 3 * sample + 1, (unsigned long long)__double_as_longlong(double(my)));
                 atomicMax(a.ws + 3 * sample + 2, (unsigned long long)__double_as_longlong(double(mz)));
             }
         }
+        __syncthreads();  // the next task reuses the LDS rows
     }
 }
 
