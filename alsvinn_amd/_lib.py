@@ -42,6 +42,8 @@ EXPORTS = [
     "alsvinn_hip_ensemble_autotune",
     "alsvinn_hip_ensemble_step",
     "alsvinn_hip_ensemble_run",
+    "alsvinn_hip_ensemble_set_hold",
+    "alsvinn_hip_ensemble_saves",
     "alsvinn_hip_ensemble_run_steps",
     "alsvinn_hip_ensemble_step_timed",
     "alsvinn_hip_ensemble_synchronize",
@@ -112,6 +114,8 @@ def load():
     L.alsvinn_hip_ensemble_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
                                               ctypes.POINTER(Sample), vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_ensemble_destroy.argtypes = [vp]
+    L.alsvinn_hip_ensemble_set_hold.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.alsvinn_hip_ensemble_saves.argtypes = [vp, ctypes.c_int, llp]
     L.alsvinn_hip_state_elements.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
                                              ctypes.POINTER(Sample), llp]
     L.alsvinn_hip_ensemble_create_bound.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <string>
@@ -36,7 +37,7 @@ __global__ void bookkeeping_kernel(const SampleDesc* __restrict__ desc, SampleSt
     if (s >= nsamples) return;
     const SampleDesc D = desc[s];
     SampleState S = state[s];
-    if (!S.active) return;
+    if (!runnable(S)) return;
     const double dt = sample_dt(D, S, ws + 3 * s);
     if (dt_hist && S.steps < dt_cap) dt_hist[(long long)s * dt_cap + S.steps] = dt;
     S.t = S.t + dt;
@@ -52,13 +53,18 @@ __global__ void bookkeeping_kernel(const SampleDesc* __restrict__ desc, SampleSt
     state[s] = S;
 }
 
+__global__ void set_hold_kernel(SampleState* state, int nsamples, int k, long long limit) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsamples) state[s].hold[k] = limit;
+}
+
 __global__ void count_active_kernel(const SampleState* __restrict__ state, int nsamples,
                                     int* count) {
     __shared__ int acc;
     if (threadIdx.x == 0) acc = 0;
     __syncthreads();
     int local = 0;
-    for (int s = threadIdx.x; s < nsamples; s += blockDim.x) local += state[s].active ? 1 : 0;
+    for (int s = threadIdx.x; s < nsamples; s += blockDim.x) local += runnable(state[s]) ? 1 : 0;
     atomicAdd(&acc, local);
     __syncthreads();
     if (threadIdx.x == 0) *count = acc;
@@ -142,6 +148,7 @@ SampleState fresh_state(const alsvinn_hip_sample& p) {
     S.saved[0] = p.saved0[0];
     S.saved[1] = p.saved0[1];
     S.active = (0 < p.end_time && p.max_steps != 0) ? 1 : 0;
+    S.hold[0] = S.hold[1] = LLONG_MAX;
     return S;
 }
 
@@ -843,6 +850,26 @@ static int ensure_graph(alsvinn_hip_ensemble* e, int steps) {
     e->graph = g;
     HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
     e->graph_steps = steps;
+    return 0;
+}
+
+int alsvinn_hip_ensemble_set_hold(alsvinn_hip_ensemble* e, int adjuster, long long saved_limit) {
+    if (!e || adjuster < 0 || adjuster > 1) return fail(ALSVINN_HIP_E_INVALID, "adjuster must be 0 or 1");
+    const long long limit = saved_limit < 0 ? LLONG_MAX : saved_limit;
+    hipLaunchKernelGGL(set_hold_kernel, dim3((e->nsamples + 255) / 256), dim3(256), 0, e->stream,
+                       e->d_state, e->nsamples, adjuster, limit);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_ensemble_saves(alsvinn_hip_ensemble* e, int sample, long long saved[2]) {
+    if (!e || sample < 0 || sample >= e->nsamples || !saved)
+        return fail(ALSVINN_HIP_E_INVALID, "bad sample index");
+    SampleState S;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(&S, e->d_state + sample, sizeof(S), hipMemcpyDeviceToHost));
+    saved[0] = S.saved[0];
+    saved[1] = S.saved[1];
     return 0;
 }
 

@@ -19,7 +19,7 @@ __global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict_
     const int s = blockIdx.y;
     const SampleDesc D = desc[s];
     SampleState* Sp = state + s;
-    if (!Sp->active) return;
+    if (!runnable(*Sp)) return;
     __shared__ double cs;
     if (threadIdx.x == 0) cs = sample_dt(D, *Sp, ws + 3 * s) / D.dx[0];
     __syncthreads();

@@ -49,9 +49,15 @@ struct SampleState {
     double last_dt;
     long long steps;
     long long saved[2];
-    int active;  // 1 while the sample still steps
+    int active;  // 1 until the sample reaches its end (or fails)
     int status;  // 0 ok, 2 constraint violation
+    long long hold[2];  // the sample pauses while saved[k] >= hold[k] (a host-side writer's turn)
 };
+
+// A sample advances while it is active and not held at a save.
+__device__ __forceinline__ bool runnable(const SampleState& S) {
+    return S.active && S.saved[0] < S.hold[0] && S.saved[1] < S.hold[1];
+}
 
 // A wavefront's work: up to four segments, each a strip of interior
 // columns [x0, x0+ncols) of one sample occupying lanes
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         const int ncols = tk.ncols[sk];
         const SampleDesc D = a.desc[sample];
         SampleState* Sp = a.state + sample;
-        const bool active = in_strip && Sp->active;
+        const bool active = in_strip && runnable(*Sp);
         if (!__any(active)) continue;  // wave-uniform
         const bool need_dt = LAZY || MODE != MODE_RHS;
         T c = T(0);
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         if (MODE == MODE_RHS && !LAZY) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
-            if (in_strip && li == 0 && Sp->active) {
+            if (in_strip && li == 0 && runnable(*Sp)) {
                 atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
                 if (DIM >= 2)
                     atomicMax(a.ws + 3 * sample + 1,
@@ -454,7 +460,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         const int ncols = tk.ncols[sk];
         const SampleDesc D = a.desc[sample];
         SampleState* Sp = a.state + sample;
-        const bool active = in_strip && Sp->active;
+        const bool active = in_strip && runnable(*Sp);
         // the samples of a task are the same for every wavefront of the group
         if (!__syncthreads_or(active)) continue;
         const bool need_dt = LAZY || MODE != MODE_RHS;
@@ -641,7 +647,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             const T mz = segment_max(wsz, seg);
-            if (in_strip && li == 0 && Sp->active && row_in) {
+            if (in_strip && li == 0 && runnable(*Sp) && row_in) {
                 atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
                 atomicMax(a.ws + 3 * sample + 1, (unsigned long long)__double_as_longlong(double(my)));
                 atomicMax(a.ws + 3 * sample + 2, (unsigned long long)__double_as_longlong(double(mz)));

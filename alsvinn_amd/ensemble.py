@@ -163,6 +163,18 @@ class Ensemble:
         return {"t": t.value, "steps": steps.value, "last_dt": dt.value, "status": st.value,
                 "active": act.value}
 
+    def set_hold(self, adjuster, saved_limit):
+        """Samples pause once adjuster's save counter reaches saved_limit
+        (None removes the hold); run() returns when every sample is paused
+        or finished."""
+        check(_lib.load().alsvinn_hip_ensemble_set_hold(self.h, adjuster,
+                                                        -1 if saved_limit is None else saved_limit))
+
+    def saves(self, sample):
+        out = (ctypes.c_longlong * 2)()
+        check(_lib.load().alsvinn_hip_ensemble_saves(self.h, sample, out))
+        return list(out)
+
     def record_dt(self, capacity):
         check(_lib.load().alsvinn_hip_ensemble_record_dt(self.h, capacity))
         self._dt_cap = capacity

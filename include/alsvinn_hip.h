@@ -165,6 +165,18 @@ int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps);
  * Returns ALSVINN_HIP_E_CONSTRAINT if a sample violated the constraints. */
 int alsvinn_hip_ensemble_run(alsvinn_hip_ensemble* e, int steps_per_sync);
 
+/* Pause at saves (FixedIntervalWriter::write and FixedIntervalStatistics::write,
+ * FixedIntervalWriter.cpp:31-43, FixedIntervalStatistics.cpp:44-62, run after
+ * the step that reaches a save time): every sample holds (stays unchanged and
+ * is not counted as running) once its save counter of adjuster k (0 or 1)
+ * reaches saved_limit, so a host writer can take the snapshot that the
+ * reference writes after that step. saved_limit < 0 removes the hold. The
+ * dt sequence is unaffected. */
+int alsvinn_hip_ensemble_set_hold(alsvinn_hip_ensemble* e, int adjuster, long long saved_limit);
+/* Save counters of a sample's two adjusters (initial value saved0 + 1 per
+ * save time reached). Synchronises the ensemble's stream. */
+int alsvinn_hip_ensemble_saves(alsvinn_hip_ensemble* e, int sample, long long saved[2]);
+
 /* Replays the captured graph of steps_per_graph timesteps nsteps/steps_per_graph
  * times without host checks (samples past their end time skip on the device).
  * Asynchronous. */

@@ -82,7 +82,9 @@ def _dp(a):
 
 
 def make_setup(equation, flux, recon, integrator, boundary, n, origin, top, gamma=1.4,
-               cfl=0.9, end_time=1.0, saves=(1,), max_steps=-1):
+               cfl=0.9, end_time=1.0, saves=(1,), max_steps=-1, intervals=None):
+    """intervals: explicit adjuster intervals (default end_time / saves[k]),
+    e.g. to stop a run at an intermediate save of a longer schedule."""
     s = OracleSetup()
     s.equation, s.flux, s.recon, s.integrator, s.boundary = equation, flux, recon, integrator, boundary
     s.nx, s.ny, s.nz = n
@@ -92,7 +94,7 @@ def make_setup(equation, flux, recon, integrator, boundary, n, origin, top, gamm
     s.gamma, s.cfl, s.end_time = gamma, cfl, end_time
     s.num_adjusters = len(saves)
     for k, ns in enumerate(saves):
-        s.adj_interval[k] = end_time / ns
+        s.adj_interval[k] = end_time / ns if intervals is None else intervals[k]
         s.adj_saved0[k] = 1
     s.max_steps = max_steps
     return s

@@ -1,0 +1,62 @@
+"""The configuration front end end to end on the GPU: a reference-format UQ
+configuration (3 Kelvin-Helmholtz samples, writer with 2 saves, meanvar and
+structure_cube at the end time) run by alsvinn_amd.cli; every snapshot in the
+writer's NetCDF file equals the oracle's state at that save time bit for bit,
+the meanvar files equal the oracle's sequential statistics bit for bit and
+the structure function agrees to 1e-12 (parallel summation)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.io
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs")
+
+
+def test_cli_uq_run_matches_oracle(hip, tmp_path):
+    print("CASE cli kh_uq 32^2 x 3", flush=True)
+    from alsvinn_amd import cli, config
+    cfg = config.read_config(os.path.join(HERE, "kh_uq.xml"))
+    summary = cli.run(cfg, str(tmp_path))
+    P = config.sample_parameters(cfg)
+    n, G = 32, 2
+    finals = []
+    f = scipy.io.netcdf_file(str(tmp_path / "kh_uq"), "r", mmap=False)
+    try:
+        assert np.allclose(f.variables["time"][:], [0.5, 1.0])
+        for i in range(3):
+            u0 = config.conserved_volume(cfg, G, P[i])
+            for k, t in enumerate((0.03, 0.06)):
+                s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                                 (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=t,
+                                 saves=(2, 1), intervals=(0.03, 0.06))
+                rc, ref, tt, steps, _ = O.run(s, u0)
+                assert rc == 0
+                for name, var in (("rho", 0), ("E", 3), ("mx", 1), ("my", 2)):
+                    slab = np.asarray(f.variables[name][i, k], dtype=np.float64)  # file is big-endian
+                    inner = ref[var, 0, G:-G, G:-G].ravel().reshape(n, n)
+                    assert np.array_equal(slab.view(np.uint64), inner.view(np.uint64)), (i, k, name)
+            finals.append(ref)
+    finally:
+        f.close()
+    mean_ref, var_ref = O.meanvar(np.stack(finals))
+    for key, ref in (("mean", mean_ref), ("variance", var_ref)):
+        g = scipy.io.netcdf_file(str(tmp_path / ("kh_uq_" + key)), "r", mmap=False)
+        try:
+            slab = np.asarray(g.variables["rho"][0, 0], dtype=np.float64)
+            inner = ref[0, 0, G:-G, G:-G].ravel().reshape(n, n)
+            assert np.array_equal(slab.view(np.uint64), inner.view(np.uint64)), key
+        finally:
+            g.close()
+    sref = sum(O.structure_cube(r, (n, n, 1), G, 5, 2.0, True) for r in finals) / 3
+    g = scipy.io.netcdf_file(str(tmp_path / "kh_uq_structure_cube_2.000000"), "r", mmap=False)
+    try:
+        got = np.asarray(g.variables["rho"][0, 0], dtype=np.float64).ravel()
+        np.testing.assert_allclose(got, sref[0], rtol=1e-12, atol=0)
+    finally:
+        g.close()
+    assert summary["samples"] == 3
