@@ -44,8 +44,10 @@ FVM_NODES = {"name", "platform", "boundary", "flux", "endTime", "equation", "equ
              "functionals"}
 EQUATIONS = {"burgers": _lib.EQ_BURGERS, "euler1": _lib.EQ_EULER1, "euler2": _lib.EQ_EULER2,
              "euler3": _lib.EQ_EULER3}
-FLUXES = {"hll": _lib.FLUX_HLL, "hll3": _lib.FLUX_HLL3, "godunov": _lib.FLUX_GODUNOV}
-RECONSTRUCTIONS = {"none": _lib.RECON_NONE, "weno2": _lib.RECON_WENO2, "eno2": _lib.RECON_ENO2}
+FLUXES = {"hll": _lib.FLUX_HLL, "hll3": _lib.FLUX_HLL3, "godunov": _lib.FLUX_GODUNOV,
+          "central": _lib.FLUX_CENTRAL, "rusanov": _lib.FLUX_RUSANOV, "roe": _lib.FLUX_ROE}
+RECONSTRUCTIONS = {"none": _lib.RECON_NONE, "weno2": _lib.RECON_WENO2, "eno2": _lib.RECON_ENO2,
+                   "mc": _lib.RECON_MC, "mm": _lib.RECON_MM}
 INTEGRATORS = {"forwardeuler": _lib.INT_FE, "rungekutta2": _lib.INT_RK2, "rungekutta3": _lib.INT_RK3}
 BOUNDARIES = {"periodic": _lib.BC_PERIODIC, "neumann": _lib.BC_NEUMANN}
 PRIMITIVES = {"burgers": ["u"], "euler1": ["rho", "ux", "p"], "euler2": ["rho", "ux", "uy", "p"],
@@ -179,6 +181,9 @@ def read_config(path):
     flux = _text(fvm, "flux")
     if flux not in FLUXES:
         raise ConfigError("flux %s is not available on the hip platform" % flux)
+    # the equation/flux pairs of numerical_flux_list.hpp:41-113 that exist here
+    if (equation == "burgers") != (flux in ("godunov", "rusanov", "roe")) and flux != "central":
+        raise ConfigError("flux %s is not available for %s" % (flux, equation))
     recon = _text(fvm, "reconstruction")
     if recon not in RECONSTRUCTIONS:
         raise ConfigError("reconstruction %s is not available on the hip platform" % recon)

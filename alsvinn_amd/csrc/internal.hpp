@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/alsvinn_hip.h"
@@ -52,11 +53,25 @@ void launch_stage(const StageArgs<T>& a, int blocks, hipStream_t s) {
     }
 }
 
+// The schemes of the measured configurations get all three register
+// budgets for the autotuner; the others one (2 waves/SIMD), which bounds the
+// number of instantiations.
+template <class Recon, class Flux>
+constexpr bool kAllBudgets =
+    (std::is_same_v<Recon, ReconNone> || std::is_same_v<Recon, ReconWENO2> ||
+     std::is_same_v<Recon, ReconENO2>) &&
+    (std::is_same_v<Flux, FluxHLL> || std::is_same_v<Flux, FluxHLL3> ||
+     std::is_same_v<Flux, FluxGodunov>);
+
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
-    k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 1>;
-    k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
-    k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
+    if constexpr (kAllBudgets<Recon, Flux>) {
+        k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 1>;
+        k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
+        k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
+    } else {
+        k.stage[0] = k.stage[1] = k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
+    }
 }
 
 template <class T, class Eq>
@@ -74,6 +89,8 @@ int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
         case ALSVINN_HIP_RECON_NONE: set_stage<T, Eq, ReconNone, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_WENO2: set_stage<T, Eq, ReconWENO2, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_ENO2: set_stage<T, Eq, ReconENO2, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_MC: set_stage<T, Eq, ReconMC, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_MM: set_stage<T, Eq, ReconMM, Flux, DIM>(k); return 0;
     }
     return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction");
 }

@@ -7,6 +7,7 @@ template <class T>
 int select_euler2(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
     if (dim != 2) return fail(ALSVINN_HIP_E_INVALID, "euler2 needs a 2D grid");
     if (sc.flux == ALSVINN_HIP_FLUX_HLL) return select_recon<T, EulerEq<T, 2>, FluxHLL, 2>(sc, k);
+    if (sc.flux == ALSVINN_HIP_FLUX_CENTRAL) return select_recon<T, EulerEq<T, 2>, FluxCentral, 2>(sc, k);
     return select_recon<T, EulerEq<T, 2>, FluxHLL3, 2>(sc, k);
 }
 

@@ -7,6 +7,7 @@ template <class T>
 int select_euler3(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
     if (dim != 3) return fail(ALSVINN_HIP_E_INVALID, "euler3 needs a 3D grid");
     if (sc.flux == ALSVINN_HIP_FLUX_HLL) return select_recon<T, EulerEq<T, 3>, FluxHLL, 3>(sc, k);
+    if (sc.flux == ALSVINN_HIP_FLUX_CENTRAL) return select_recon<T, EulerEq<T, 3>, FluxCentral, 3>(sc, k);
     return select_recon<T, EulerEq<T, 3>, FluxHLL3, 3>(sc, k);
 }
 

@@ -80,6 +80,11 @@ struct EulerEq {
         F[1 + d] += a.p;
         F[nvar - 1] = (a.q[nvar - 1] + a.p) * a.u[d];
     }
+    // Euler::computeWaveSpeed (Euler.hpp:251-258): |u_d| + sqrt(gamma * p / rho)
+    template <int d>
+    __device__ __forceinline__ T waveSpeed(const All& a) const {
+        return dabs(a.u[d]) + dsqrt(dv(gamma * a.p, a.q[0]));
+    }
     // CPUCellComputer::obeysConstraints + Euler::obeysConstraints:
     // every conserved and extra value finite, rho > 0, p > 0.
     __device__ __forceinline__ bool obeys(const T* q) const {
@@ -110,6 +115,11 @@ struct BurgersEq {
         F[0] = a.q[0] * a.q[0] / T(2);
     }
     __device__ __forceinline__ bool obeys(const T* q) const { return __builtin_isfinite(q[0]); }
+    // Burgers::computeWaveSpeed (Burgers.hpp:203-209): |u|
+    template <int d>
+    __device__ __forceinline__ T waveSpeed(const All& a) const {
+        return dabs(a.q[0]);
+    }
 };
 
 // ------------------------------------------------------------------ fluxes
@@ -238,6 +248,51 @@ struct FluxGodunov {
     }
 };
 
+// Central.hpp:38-56 (Euler and Burgers): F = 0.5*(f(L) + f(R)); wave speed
+// max of the two cells' computeWaveSpeed.
+struct FluxCentral {
+    template <int d, class T, class Eq>
+    __device__ __forceinline__ static T compute(const Eq& eq, const typename Eq::All& L,
+                                                const typename Eq::All& R, T* F) {
+        constexpr int nv = Eq::nvar;
+        T fL[nv], fR[nv];
+        eq.template pointFlux<d>(L, fL);
+        eq.template pointFlux<d>(R, fR);
+#pragma unroll
+        for (int i = 0; i < nv; ++i) F[i] = T(0.5) * (fL[i] + fR[i]);
+        return ref_fmax(eq.template waveSpeed<d>(L), eq.template waveSpeed<d>(R));
+    }
+};
+
+// Rusanov.hpp:38-60 for Burgers (eigenvalue u):
+// F = 0.5*(f(L) + f(R)) - 0.5*max(|uL|, |uR|)*(uR - uL).
+struct FluxRusanov {
+    template <int d, class T>
+    __device__ __forceinline__ static T compute(const BurgersEq<T>& eq,
+                                                const typename BurgersEq<T>::All& L,
+                                                const typename BurgersEq<T>::All& R, T* F) {
+        T fL[1], fR[1];
+        eq.template pointFlux<d>(L, fL);
+        eq.template pointFlux<d>(R, fR);
+        const T eigenMax = ref_fmax(dabs(L.q[0]), dabs(R.q[0]));
+        F[0] = T(0.5) * (fL[0] + fR[0]) - T(0.5) * eigenMax * (R.q[0] - L.q[0]);
+        return ref_fmax(eq.template waveSpeed<d>(L), eq.template waveSpeed<d>(R));
+    }
+};
+
+// burgers/Roe.hpp:38-62: f(max(uL, 0)) when uL >= -uR, else f(min(uR, 0)).
+struct FluxRoe {
+    template <int d, class T>
+    __device__ __forceinline__ static T compute(const BurgersEq<T>& eq,
+                                                const typename BurgersEq<T>::All& L,
+                                                const typename BurgersEq<T>::All& R, T* F) {
+        typename BurgersEq<T>::All a;
+        a.q[0] = (L.q[0] >= -R.q[0]) ? ref_fmax(L.q[0], T(0)) : ref_fmin(R.q[0], T(0));
+        eq.template pointFlux<d>(a, F);
+        return ref_fmax(eq.template waveSpeed<d>(L), eq.template waveSpeed<d>(R));
+    }
+};
+
 // ------------------------------------------------------------------ reconstruction
 // Each reconstruction maps the three cells (i-1, i, i+1) along one direction
 // to the left (i-1/2) and right (i+1/2) face values of cell i.
@@ -321,6 +376,53 @@ struct ReconENO2 {
             r += cr1 * b;
             L[v] = l;
             R[v] = r;
+        }
+    }
+};
+
+// sign.hpp and minmod.hpp of the reference.
+template <class T>
+__device__ __forceinline__ T ref_sign(T a) {
+    return a > T(0) ? T(1) : (a == T(0) ? T(0) : T(-1));
+}
+template <class T>
+__device__ __forceinline__ T minmod2(T a, T b) {
+    return ref_sign(a) == ref_sign(b) ? ref_sign(a) * ref_fmin(dabs(a), dabs(b)) : T(0);
+}
+template <class T>
+__device__ __forceinline__ T minmod3(T a, T b, T c) {
+    return (ref_sign(a) == ref_sign(b) && ref_sign(b) == ref_sign(c))
+               ? ref_sign(a) * ref_fmin(dabs(a), ref_fmin(dabs(b), dabs(c)))
+               : T(0);
+}
+
+// MC.hpp:29-55 (monotonized central) and MM.hpp:29-54 (minmod): per
+// variable, slope sigma from the limiter; left = m - sigma/2, right = m + sigma/2.
+struct ReconMC {
+    static constexpr int ghosts = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const T sigma = minmod3(T(2) * (qp[v] - q0[v]), (qp[v] - qm[v]) / T(2),
+                                    T(2) * (q0[v] - qm[v]));
+            L[v] = q0[v] - sigma / T(2);
+            R[v] = q0[v] + sigma / T(2);
+        }
+    }
+};
+
+struct ReconMM {
+    static constexpr int ghosts = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const T sigma = minmod2(qp[v] - q0[v], q0[v] - qm[v]);
+            L[v] = q0[v] - sigma / T(2);
+            R[v] = q0[v] + sigma / T(2);
         }
     }
 };

@@ -27,8 +27,26 @@ enum {
     ALSVINN_HIP_EQ_EULER2 = 2,
     ALSVINN_HIP_EQ_EULER3 = 3
 };
-enum { ALSVINN_HIP_FLUX_HLL = 0, ALSVINN_HIP_FLUX_HLL3 = 1, ALSVINN_HIP_FLUX_GODUNOV = 2 };
-enum { ALSVINN_HIP_RECON_NONE = 0, ALSVINN_HIP_RECON_WENO2 = 1, ALSVINN_HIP_RECON_ENO2 = 2 };
+/* Fluxes: hll (numflux/euler/HLL.hpp), hll3 (HLL3.hpp), godunov
+ * (numflux/burgers/Godunov.hpp), central (numflux/Central.hpp; Euler and
+ * Burgers), rusanov (numflux/Rusanov.hpp; Burgers), roe (numflux/burgers/Roe.hpp). */
+enum {
+    ALSVINN_HIP_FLUX_HLL = 0,
+    ALSVINN_HIP_FLUX_HLL3 = 1,
+    ALSVINN_HIP_FLUX_GODUNOV = 2,
+    ALSVINN_HIP_FLUX_CENTRAL = 3,
+    ALSVINN_HIP_FLUX_RUSANOV = 4,
+    ALSVINN_HIP_FLUX_ROE = 5
+};
+/* Reconstructions: none (NoReconstruction.cpp), weno2 (WENO2.hpp), eno2
+ * (ENOCPU.cpp, order 2), mc (MC.hpp), mm (MM.hpp). */
+enum {
+    ALSVINN_HIP_RECON_NONE = 0,
+    ALSVINN_HIP_RECON_WENO2 = 1,
+    ALSVINN_HIP_RECON_ENO2 = 2,
+    ALSVINN_HIP_RECON_MC = 3,
+    ALSVINN_HIP_RECON_MM = 4
+};
 enum {
     ALSVINN_HIP_INTEGRATOR_FORWARD_EULER = 0,
     ALSVINN_HIP_INTEGRATOR_RK2 = 1,

@@ -124,6 +124,11 @@ struct Euler {
         F[nvar - 1] = (a.q[nvar - 1] + a.p) * a.u[d];
     }
     static real weight(const real* q) { return q[0]; }
+    // Euler.hpp:251-258 computeWaveSpeed: |u_d| + sqrt(gamma * p / rho)
+    template <int d>
+    real waveSpeed(const All& a) const {
+        return std::fabs(a.u[d]) + std::sqrt(gamma * a.p / a.q[0]);
+    }
     // CPUCellComputer.cpp:79-125 + Euler.hpp:269-277
     bool obeys(const real* q) const {
         real u[NSD], p;
@@ -157,6 +162,11 @@ struct Burgers {
         F[0] = a.q[0] * a.q[0] / 2;
     }
     static real weight(const real* q) { return q[0]; }
+    // Burgers.hpp:203-209 computeWaveSpeed = |u|; 298-301 computeEigenValues = u
+    template <int d>
+    real waveSpeed(const All& a) const {
+        return std::fabs(a.q[0]);
+    }
     bool obeys(const real* q) const {
         if (std::isnan(q[0]) || std::isinf(q[0])) return false;
         return q[0] < INFINITY && (q[0] == q[0]);
@@ -272,11 +282,106 @@ struct Godunov {
     }
 };
 
+struct Central {
+    // numflux/Central.hpp:38-56: F = 0.5*(f(L) + f(R)), speed max of the two
+    template <int d, class Eq>
+    static real compute(const Eq& eq, const typename Eq::All& L, const typename Eq::All& R,
+                        real* F) {
+        constexpr int nv = Eq::nvar;
+        real fl[nv], fr[nv];
+        eq.template pointFlux<d>(L, fl);
+        eq.template pointFlux<d>(R, fr);
+        for (int i = 0; i < nv; ++i) F[i] = 0.5 * (fl[i] + fr[i]);
+        return std::fmax(eq.template waveSpeed<d>(L), eq.template waveSpeed<d>(R));
+    }
+};
+
+struct Rusanov {
+    // numflux/Rusanov.hpp:38-60 for Burgers (eigenvalue u):
+    // F = 0.5*(f(L) + f(R)) - 0.5*max(|uL|, |uR|)*(uR - uL)
+    template <int d>
+    static real compute(const Burgers& eq, const Burgers::All& L, const Burgers::All& R,
+                        real* F) {
+        real fl[1], fr[1];
+        eq.pointFlux<d>(L, fl);
+        eq.pointFlux<d>(R, fr);
+        const real eigenLeft = std::fabs(L.q[0]);
+        const real eigenRight = std::fabs(R.q[0]);
+        const real eigenMax = std::fmax(eigenLeft, eigenRight);
+        F[0] = 0.5 * (fl[0] + fr[0]) - 0.5 * eigenMax * (R.q[0] - L.q[0]);
+        return std::fmax(eq.waveSpeed<d>(L), eq.waveSpeed<d>(R));
+    }
+};
+
+struct Roe {
+    // numflux/burgers/Roe.hpp:38-62: upwind by the sign of uL + uR
+    template <int d>
+    static real compute(const Burgers& eq, const Burgers::All& L, const Burgers::All& R,
+                        real* F) {
+        Burgers::All a;
+        if (L.q[0] >= -R.q[0])
+            a.q[0] = std::fmax(L.q[0], real(0.0));
+        else
+            a.q[0] = std::fmin(R.q[0], real(0.0));
+        eq.pointFlux<d>(a, F);
+        return std::fmax(eq.waveSpeed<d>(L), eq.waveSpeed<d>(R));
+    }
+};
+
 // ---------------------------------------------------------------- reconstruction
 
-enum { RECON_NONE = 0, RECON_WENO2 = 1, RECON_ENO2 = 2 };
+enum { RECON_NONE = 0, RECON_WENO2 = 1, RECON_ENO2 = 2, RECON_MC = 3, RECON_MM = 4 };
 
 static int reconGhosts(int recon) { return recon == RECON_NONE ? 1 : 2; }
+
+// reconstruction/sign.hpp and minmod.hpp
+static real refSign(real a) {
+    if (a > 0) return 1;
+    if (a == 0) return 0;
+    return -1;
+}
+static real minmod(real a, real b) {
+    if (refSign(a) == refSign(b)) return refSign(a) * std::fmin(std::fabs(a), std::fabs(b));
+    return 0;
+}
+static real minmod(real a, real b, real c) {
+    if (refSign(a) == refSign(b) && refSign(b) == refSign(c))
+        return refSign(a) * std::fmin(std::fabs(a), std::fmin(std::fabs(b), std::fabs(c)));
+    return 0;
+}
+
+// ReconstructionCPU.cpp:38-97 with MC.hpp:29-55 / MM.hpp:29-54: per variable,
+// slope sigma from the limiter, left = middle - sigma/2, right = middle + sigma/2.
+static void reconLimiter(const Vol& in, int d, Vol& Lo, Vol& Ro, ivec3 start, ivec3 end, bool mc) {
+    const int nx = in.tx, ny = in.ty, nz = in.tz;
+    const int dim = in.dims();
+    const int g = 2;
+    const int ngx = g, ngy = (dim > 1) * g, ngz = (dim > 2) * g;
+    const int dx = d == 0, dy = d == 1, dz = d == 2;
+    const int sx = (ngx - dx) + start.x, sy = (ngy - dy) + start.y, sz = (ngz - dz) + start.z;
+    const int ex = nx - (ngx - dx) + end.x, ey = ny - (ngy - dy) + end.y,
+              ez = nz - (ngz - dz) + end.z;
+    for (int z = sz; z < ez; z++) {
+#pragma omp parallel for
+        for (int y = sy; y < ey; y++) {
+            for (int x = sx; x < ex; x++) {
+                const size_t iO = in.idx(x, y, z);
+                const size_t iR = in.idx(x + dx, y + dy, z + dz);
+                const size_t iL = in.idx(x - dx, y - dy, z - dz);
+                for (int var = 0; var < in.nvar; ++var) {
+                    const real left = in.v(var)[iL];
+                    const real middle = in.v(var)[iO];
+                    const real right = in.v(var)[iR];
+                    const real sigma = mc ? minmod(2 * (right - middle), (right - left) / 2,
+                                                   2 * (middle - left))
+                                          : minmod((right - middle), (middle - left));
+                    Lo.v(var)[iO] = middle - sigma / 2;
+                    Ro.v(var)[iO] = middle + sigma / 2;
+                }
+            }
+        }
+    }
+}
 
 // NoReconstruction.cpp:42-73: every cell, left = right = input.
 static void reconNone(const Vol& in, Vol& L, Vol& R) {
@@ -453,7 +558,7 @@ static void netFlux(const Eq& eq, const Vol& left, const Vol& right, Vol& out, V
 
 struct Setup {
     int equation;    // 0 burgers, 1..3 euler<nsd>
-    int flux;        // 0 hll, 1 hll3, 2 godunov
+    int flux;        // 0 hll, 1 hll3, 2 godunov, 3 central, 4 rusanov, 5 roe
     int recon;       // RECON_*
     int integrator;  // 0 forward euler, 1 ssp-rk2, 2 ssp-rk3
     int boundary;    // 0 neumann, 1 periodic
@@ -492,6 +597,8 @@ struct System : SystemBase {
             reconNone(U, left, right);
         else if (recon == RECON_WENO2)
             reconWeno2(U, d, left, right, s, e);
+        else if (recon == RECON_MC || recon == RECON_MM)
+            reconLimiter(U, d, left, right, s, e, recon == RECON_MC);
         else
             reconEno2(U, d, left, right, dd, s, e);
     }
@@ -599,17 +706,21 @@ static int nvarOf(int equation) { return equation == 0 ? 1 : equation + 2; }
 
 static SystemBase* makeSystem(const Setup& s, const Vol& shape) {
     if (s.equation == 0) {
-        if (s.flux != 2) return nullptr;
         Burgers b;
         b.gamma = s.gamma;
-        return new System<Burgers, Godunov>(b, s.recon, shape);
+        if (s.flux == 2) return new System<Burgers, Godunov>(b, s.recon, shape);
+        if (s.flux == 3) return new System<Burgers, Central>(b, s.recon, shape);
+        if (s.flux == 4) return new System<Burgers, Rusanov>(b, s.recon, shape);
+        if (s.flux == 5) return new System<Burgers, Roe>(b, s.recon, shape);
+        return nullptr;
     }
-    if (s.flux == 2) return nullptr;
+    if (s.flux != 0 && s.flux != 1 && s.flux != 3) return nullptr;
 #define ORC_EULER(N)                                                  \
     if (s.equation == N) {                                            \
         Euler<N> e;                                                   \
         e.gamma = s.gamma;                                            \
         if (s.flux == 0) return new System<Euler<N>, HLL>(e, s.recon, shape);  \
+        if (s.flux == 3) return new System<Euler<N>, Central>(e, s.recon, shape);  \
         return new System<Euler<N>, HLL3>(e, s.recon, shape);         \
     }
     ORC_EULER(1)
@@ -726,7 +837,7 @@ struct Sim {
 
 static bool validSetup(const Setup& s) {
     if (s.equation < 0 || s.equation > 3) return false;
-    if (s.recon < 0 || s.recon > 2) return false;
+    if (s.recon < 0 || s.recon > 4) return false;
     if (s.integrator < 0 || s.integrator > 2) return false;
     if (s.num_adjusters < 0 || s.num_adjusters > 2) return false;
     if (s.nx < 1 || s.ny < 1 || s.nz < 1) return false;
@@ -819,6 +930,9 @@ double oracle_face_flux(int equation, int flux, int direction, double gamma, con
     if (equation == 0) {
         orc::Burgers b;
         b.gamma = gamma;
+        if (flux == 3) ORC_FACE(b, orc::Central, 0)
+        if (flux == 4) ORC_FACE(b, orc::Rusanov, 0)
+        if (flux == 5) ORC_FACE(b, orc::Roe, 0)
         if (direction == 0) ORC_FACE(b, orc::Godunov, 0)
         if (direction == 1) ORC_FACE(b, orc::Godunov, 1)
         ORC_FACE(b, orc::Godunov, 2)
@@ -827,6 +941,11 @@ double oracle_face_flux(int equation, int flux, int direction, double gamma, con
     if (equation == N) {                                                       \
         orc::Euler<N> e;                                                       \
         e.gamma = gamma;                                                       \
+        if (flux == 3) {                                                       \
+            if (direction == 0) ORC_FACE(e, orc::Central, 0)                    \
+            if (N > 1 && direction == 1) ORC_FACE(e, orc::Central, (N > 1 ? 1 : 0)) \
+            if (N > 2 && direction == 2) ORC_FACE(e, orc::Central, (N > 2 ? 2 : 0)) \
+        }                                                                      \
         if (flux == 0) {                                                       \
             if (direction == 0) ORC_FACE(e, orc::HLL, 0)                        \
             if (N > 1 && direction == 1) ORC_FACE(e, orc::HLL, (N > 1 ? 1 : 0)) \
@@ -879,6 +998,8 @@ int oracle_reconstruct_1d(int recon, int nvar, int n, const double* q, double* l
     std::vector<double> dd;
     if (recon == orc::RECON_NONE) orc::reconNone(U, L, R);
     else if (recon == orc::RECON_WENO2) orc::reconWeno2(U, 0, L, R, orc::ivec3(), orc::ivec3());
+    else if (recon == orc::RECON_MC || recon == orc::RECON_MM)
+        orc::reconLimiter(U, 0, L, R, orc::ivec3(), orc::ivec3(), recon == orc::RECON_MC);
     else orc::reconEno2(U, 0, L, R, dd, orc::ivec3(), orc::ivec3());
     std::memcpy(left, L.buf.data(), L.buf.size() * sizeof(double));
     std::memcpy(right, R.buf.data(), R.buf.size() * sizeof(double));

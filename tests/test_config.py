@@ -46,14 +46,15 @@ def test_auto_rules_follow_the_reference(tmp_path):
 
 @pytest.mark.parametrize("change, message", [
     ("<boundary>periodic</boundary>", "Unsupported configuration option bogus"),
-    ("<flux>hll3</flux>", "flux rusanov is not available"),
+    ("<flux>hll3</flux>", "flux rusanov is not available for euler2"),
+    ("<flux>hll3</flux>", "flux tecno1 is not available on the hip platform"),
     ("<type>netcdf</type>\n    <basename>kh_uq</basename>\n    <numberOfSaves>2",
      "writer type hdf5 is not available"),
 ])
 def test_rejected_options(change, message, tmp_path):
     src = open(os.path.join(HERE, "kh_uq.xml")).read()
     repl = {"<boundary>periodic</boundary>": "<boundary>periodic</boundary><bogus>1</bogus>",
-            "<flux>hll3</flux>": "<flux>rusanov</flux>",
+            "<flux>hll3</flux>": "<flux>%s</flux>" % message.split()[1],
             "<type>netcdf</type>\n    <basename>kh_uq</basename>\n    <numberOfSaves>2":
                 "<type>hdf5</type>\n    <basename>kh_uq</basename>\n    <numberOfSaves>2"}[change]
     assert change in src

@@ -335,3 +335,70 @@ def test_caller_owned_state_advanced_in_place(hip):
         assert rc == 0
         assert np.array_equal(ref.view(np.uint64), got[i].view(np.uint64))
         assert np.array_equal(ref.view(np.uint64), via_api[i].view(np.uint64))
+
+
+@pytest.mark.parametrize("flux,recon,integ", [
+    (O.FLUX_CENTRAL, O.RECON_MC, O.INT_RK2),
+    (O.FLUX_RUSANOV, O.RECON_MM, O.INT_RK3),
+    (O.FLUX_ROE, O.RECON_NONE, O.INT_FE),
+    (O.FLUX_GODUNOV, O.RECON_MC, O.INT_RK2),
+    (O.FLUX_ROE, O.RECON_MM, O.INT_RK3),
+])
+def test_burgers_fluxes_and_limiters_bit_exact(hip, flux, recon, integ):
+    """The central, rusanov and roe fluxes (Central.hpp, Rusanov.hpp,
+    burgers/Roe.hpp) and the mc / mm reconstructions (MC.hpp, MM.hpp) on a
+    periodic 1D Burgers solve with shocks, against the oracle."""
+    _print_case("burgers 1024 flux=%d recon=%d integ=%d" % (flux, recon, integ))
+    from alsvinn_amd import initial_data as ic
+    g = 1 if recon == O.RECON_NONE else 2
+    u0 = ic.burgers_fbm(1024, 0.5, seed=11, ghosts=g)
+    cfl = 0.9 if recon == O.RECON_NONE else 0.475
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_BURGERS, flux, recon, integ, O.BC_PERIODIC, (1024, 1, 1), (0, 0, 0),
+        (1, 0, 0), cfl, 0.05, u0)
+    assert info["steps"] == steps_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+@pytest.mark.parametrize("flux,recon,integ", [
+    (O.FLUX_HLL, O.RECON_MC, O.INT_RK2),
+    (O.FLUX_HLL3, O.RECON_MM, O.INT_RK3),
+    (O.FLUX_CENTRAL, O.RECON_MC, O.INT_RK3),
+])
+def test_euler2_limiters_and_central_bit_exact(hip, flux, recon, integ):
+    """mc / mm reconstructions and the central flux on the 2D Kelvin-Helmholtz
+    state (periodic, 12 steps; the central flux has no dissipation, so only a
+    few steps), against the oracle."""
+    _print_case("kh 48^2 flux=%d recon=%d integ=%d" % (flux, recon, integ))
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(1, 40)[0]
+    u0 = ic.kelvin_helmholtz_2d(48, X)
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER2, flux, recon, integ, O.BC_PERIODIC, (48, 48, 1), (0, 0, 0), (1, 1, 0),
+        0.475, 2.0, u0, max_steps=12)
+    assert info["steps"] == steps_ref == 12
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
+
+
+def test_euler3_mc_grouped_kernel_bit_exact(hip):
+    """mc reconstruction through the 3D workgroup kernel (Neumann, 10 rows so
+    that the last group of four rows is partial), against the oracle."""
+    _print_case("euler3 mc hll3 14x10x9 neumann")
+    rng = np.random.default_rng(21)
+    n = (14, 10, 9)
+    s = O.make_setup(O.EQ_EULER3, O.FLUX_HLL3, O.RECON_MC, O.INT_RK2, O.BC_NEUMANN, n,
+                     (0, 0, 0), (1, 1, 1), cfl=0.475, end_time=1.0, max_steps=6)
+    shape = O.volume_shape(s)
+    u0 = np.zeros(shape)
+    u0[0] = 1.0 + 0.5 * rng.random(shape[1:])
+    for k in (1, 2, 3):
+        u0[k] = 0.3 * rng.standard_normal(shape[1:])
+    u0[4] = 3.0 + rng.random(shape[1:])
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(
+        hip, O.EQ_EULER3, O.FLUX_HLL3, O.RECON_MC, O.INT_RK2, O.BC_NEUMANN, n, (0, 0, 0),
+        (1, 1, 1), 0.475, 1.0, u0, max_steps=6)
+    assert info["steps"] == steps_ref == 6
+    _assert_bit_exact(ref, got)

@@ -165,10 +165,13 @@ def test_reconstruction_of_constant(recon):
 
 
 @pytest.mark.parametrize("recon,rate", [(O.RECON_WENO2, 1.9), (O.RECON_ENO2, 1.9),
-                                        (O.RECON_NONE, 0.9)])
+                                        (O.RECON_NONE, 0.9), (O.RECON_MC, 1.99),
+                                        (O.RECON_MM, 1.9)])
 def test_reconstruction_convergence(recon, rate):
-    """ReconstructionConvergenceTest.cpp:288-300: face values of cell averages
-    of a smooth function converge at rate >= 1.9 (weno2, eno2), 0.9 (none)."""
+    """ReconstructionConvergenceTest.cpp:288-316: face values of cell averages
+    of a smooth function converge at rate >= 1.9 (weno2, eno2), 1.99 (mc, the
+    reference's double-precision bound), 0.9 (none); mm (not in the reference
+    test) at >= 1.9."""
     errs = []
     Ns = [64, 128, 256, 512]
     for N in Ns:
@@ -188,6 +191,71 @@ def test_reconstruction_convergence(recon, rate):
     assert rates[-1] >= rate, rates
 
 
+@pytest.mark.parametrize("recon", [O.RECON_MC, O.RECON_MM])
+def test_limiters_closed_forms(recon):
+    """MC.hpp / MM.hpp with minmod.hpp: linear data give the exact face values
+    m -/+ h/2; at a local extremum the slope is zero (left = right = middle);
+    hand-computed values for a steep-then-shallow stencil (0, 1, 1.5):
+    MC slope minmod(1, 0.75, 2) = 0.75, MM slope minmod(0.5, 1) = 0.5."""
+    q = np.array([[0.0, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0]]) * 0.25 + 3.0  # linear, h = 0.25
+    left, right = O.reconstruct_1d(recon, q)
+    for i in range(1, 6):
+        assert left[0, i] == q[0, i] - 0.125 and right[0, i] == q[0, i] + 0.125
+    q = np.array([[0.0, 0.0, 2.0, 5.0, 2.0, 0.0, 0.0]])  # extremum at index 3
+    left, right = O.reconstruct_1d(recon, q)
+    assert left[0, 3] == 5.0 and right[0, 3] == 5.0
+    q = np.array([[-1.0, -1.0, 0.0, 1.0, 1.5, 1.5, 1.5]])
+    left, right = O.reconstruct_1d(recon, q)
+    slope = 0.75 if recon == O.RECON_MC else 0.5
+    assert left[0, 3] == 1.0 - slope / 2 and right[0, 3] == 1.0 + slope / 2
+
+
+@pytest.mark.parametrize("flux", [O.FLUX_CENTRAL, O.FLUX_RUSANOV, O.FLUX_ROE, O.FLUX_GODUNOV])
+def test_burgers_fluxes_consistent(flux):
+    """F(u, u) = f(u) = u^2/2 for every Burgers flux, and the wave speed is |u|."""
+    for u in (-2.5, -0.3, 0.0, 0.7, 3.0):
+        F, ws = O.face_flux(O.EQ_BURGERS, flux, 0, 1.4, [u], [u])
+        assert F[0] == u * u / 2 and ws == abs(u)
+
+
+def test_burgers_fluxes_known_answers():
+    """Hand-evaluated: Rusanov(1, -1) = 0.5*(0.5 + 0.5) - 0.5*1*(-2) = 1.5;
+    Roe upwinds on the sign of uL + uR: Roe(1, -1) = f(1) = 0.5,
+    Roe(-1, 1) = f(max(-1, 0)) = 0, Roe(-2, -1) = f(min(-1, 0)) = 0.5;
+    Central(1, 3) = 0.5*(0.5 + 4.5) = 2.5 with wave speed 3."""
+    B = O.EQ_BURGERS
+    assert O.face_flux(B, O.FLUX_RUSANOV, 0, 1.4, [1.0], [-1.0])[0][0] == 1.5
+    assert O.face_flux(B, O.FLUX_ROE, 0, 1.4, [1.0], [-1.0])[0][0] == 0.5
+    assert O.face_flux(B, O.FLUX_ROE, 0, 1.4, [-1.0], [1.0])[0][0] == 0.0
+    assert O.face_flux(B, O.FLUX_ROE, 0, 1.4, [-2.0], [-1.0])[0][0] == 0.5
+    F, ws = O.face_flux(B, O.FLUX_CENTRAL, 0, 1.4, [1.0], [3.0])
+    assert F[0] == 2.5 and ws == 3.0
+
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+def test_euler_central_flux(nsd):
+    """Central.hpp: F = 0.5*(f(L) + f(R)) with f the Euler point flux (pinned
+    by the HLL consistency tests), wave speed max(|u_d| + sqrt(gamma p/rho));
+    F(U, U) = f(U)."""
+    rng = np.random.default_rng(nsd)
+    g = 1.4
+    for d in range(nsd):
+        for _ in range(5):
+            rho, p = rng.uniform(0.5, 2, 2)
+            u = rng.uniform(-1, 1, (2, nsd))
+            qL = euler_state(1.3 * rho, list(u[0]), p)
+            qR = euler_state(rho, list(u[1]), 0.7 * p)
+            F, ws = O.face_flux(nsd, O.FLUX_CENTRAL, d, g, qL, qR)
+            fL = O.point_flux(nsd, d, g, qL)
+            fR = O.point_flux(nsd, d, g, qR)
+            np.testing.assert_array_equal(F, 0.5 * (fL + fR))
+            cL = abs(u[0][d]) + np.sqrt(g * p / (1.3 * rho))
+            cR = abs(u[1][d]) + np.sqrt(g * (0.7 * p) / rho)
+            assert ws == pytest.approx(max(cL, cR), rel=1e-14)
+            F2, _ = O.face_flux(nsd, O.FLUX_CENTRAL, d, g, qL, qL)
+            np.testing.assert_array_equal(F2, fL)
+
+
 # ---------------------------------------------------------------- operator
 
 @pytest.mark.parametrize("eq,flux,recon,n", [
@@ -195,6 +263,11 @@ def test_reconstruction_convergence(recon, rate):
     (O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, (16, 12, 1)),
     (O.EQ_EULER3, O.FLUX_HLL3, O.RECON_ENO2, (8, 7, 6)),
     (O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_ENO2, (30, 1, 1)),
+    (O.EQ_BURGERS, O.FLUX_CENTRAL, O.RECON_MC, (30, 1, 1)),
+    (O.EQ_BURGERS, O.FLUX_RUSANOV, O.RECON_MM, (30, 1, 1)),
+    (O.EQ_BURGERS, O.FLUX_ROE, O.RECON_NONE, (30, 1, 1)),
+    (O.EQ_EULER2, O.FLUX_CENTRAL, O.RECON_MC, (12, 10, 1)),
+    (O.EQ_EULER1, O.FLUX_HLL, O.RECON_MM, (24, 1, 1)),
 ])
 def test_net_flux_zero_for_constant_state(eq, flux, recon, n):
     """numericalfluxtest.cpp:65-126: the divergence of a constant state is zero."""
