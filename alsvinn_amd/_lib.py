@@ -53,6 +53,8 @@ EXPORTS = [
     "alsvinn_hip_ensemble_rhs",
     "alsvinn_hip_ensemble_last_elapsed_ms",
     "alsvinn_hip_stats_meanvar",
+    "alsvinn_hip_stats_onepoint_moment",
+    "alsvinn_hip_stats_bounded_variation",
     "alsvinn_hip_stats_structure_cube",
     "alsvinn_hip_stats_structure_basic",
     "alsvinn_hip_ensemble_totals",
@@ -145,6 +147,8 @@ def load():
     L.alsvinn_hip_ensemble_rhs.argtypes = [vp, ctypes.c_int, dp, dp]
     L.alsvinn_hip_ensemble_last_elapsed_ms.argtypes = [vp, dp]
     L.alsvinn_hip_stats_meanvar.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, dp, dp]
+    L.alsvinn_hip_stats_onepoint_moment.argtypes = [vp, ip, ctypes.c_int, ctypes.c_double, dp]
+    L.alsvinn_hip_stats_bounded_variation.argtypes = [vp, ip, ctypes.c_int, ctypes.c_double, dp]
     L.alsvinn_hip_stats_structure_cube.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int,
                                                    ctypes.c_double, dp]
     L.alsvinn_hip_stats_structure_basic.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int,

@@ -10,8 +10,9 @@ once its save counter reaches the next save (the state the reference's
 FixedIntervalWriter / FixedIntervalStatistics see after that step), the host
 writes the snapshots or accumulates the statistics, and the run continues.
 Output files are named as the reference names them: the writer's basename, and
-<basename>_mean, <basename>_variance, <basename>_structure_cube_<p> for the
-statistics (Setup.cpp:277-295, StructureCube.cpp:27).
+<basename>_mean, <basename>_variance, <basename>_structure_cube_<p>,
+<basename>_m<p> and <basename>_bv_<p> for the statistics (Setup.cpp:277-295,
+StructureCube.cpp:27, OnePointMoment.cpp:21-23, BoundedVariation.cpp:23-26).
 """
 import argparse
 import json
@@ -46,6 +47,16 @@ class _StatsOutput:
             for key in ("mean", "variance"):
                 self.files[key] = netcdf.EnsembleWriter(base + "_" + key, cfg.dims, mids, nsamples,
                                                         times, names)
+        elif stat.name == "onepointmoment":
+            key = "m%d" % int(stat.params["p"])
+            self.files[key] = netcdf.EnsembleWriter(
+                base + "_" + key, cfg.dims, netcdf.midpoints(cfg.lower, cfg.upper, cfg.dims),
+                nsamples, times, names)
+        elif stat.name == "bv":
+            # one value per variable: the reference's 1x1x1 snapshot volume
+            key = "bv_%d" % int(stat.params["p"])
+            self.files[key] = netcdf.EnsembleWriter(base + "_" + key, (1, 1, 1),
+                                                    [np.zeros(1)] * 3, nsamples, times, names)
         else:
             p = stat.params.get("p", 2.0)
             H = int(stat.params.get("numberOfH", 1))
@@ -62,6 +73,16 @@ class _StatsOutput:
             self.files["mean"].write(0, save_index, mean, g)
             self.files["variance"].write(0, save_index, var, g)
             results.setdefault("meanvar", []).append((mean, var))
+        elif st.name == "onepointmoment":
+            key = "m%d" % int(st.params["p"])
+            m = e.onepoint_moment(int(st.params["p"]))
+            self.files[key].write(0, save_index, m, _grid_ghosts(m.shape, e))
+            results.setdefault(key, []).append(m)
+        elif st.name == "bv":
+            key = "bv_%d" % int(st.params["p"])
+            bv = e.bounded_variation(int(st.params["p"]))
+            self.files[key].write(0, save_index, bv.reshape(-1, 1, 1, 1), (0, 0, 0))
+            results.setdefault(key, []).append(bv)
         else:
             p = st.params.get("p", 2.0)
             H = int(st.params.get("numberOfH", 1))

@@ -52,7 +52,7 @@ INTEGRATORS = {"forwardeuler": _lib.INT_FE, "rungekutta2": _lib.INT_RK2, "rungek
 BOUNDARIES = {"periodic": _lib.BC_PERIODIC, "neumann": _lib.BC_NEUMANN}
 PRIMITIVES = {"burgers": ["u"], "euler1": ["rho", "ux", "p"], "euler2": ["rho", "ux", "uy", "p"],
               "euler3": ["rho", "ux", "uy", "uz", "p"]}
-STATISTICS = {"meanvar", "structure_cube", "structure_basic"}
+STATISTICS = {"meanvar", "structure_cube", "structure_basic", "onepointmoment", "bv"}
 
 
 def _text(node, path, default=None):
@@ -262,6 +262,14 @@ def read_config(path):
             for key in ("p", "numberOfH", "direction"):
                 if st.find(key) is not None:
                     params[key] = float(_text(st, key))
+            if name in ("onepointmoment", "bv"):
+                # OnePointMoment.cpp:21-23, BoundedVariation.cpp:23-26: integer p
+                p = params.get("p")
+                if p is None or p != int(p):
+                    raise ConfigError("statistics %s needs an integer <p>" % name)
+                if name == "bv" and cfg.dims[2] > 1:
+                    raise ConfigError("bv is not available for 3D grids (the reference's "
+                                      "getTotalVariation rejects them)")
             wi = st.find("writeInitialTimestep")
             initial = wi is not None and wi.text.strip().lower() in ("1", "true")
             cfg.stats.append(Stat(name, n if n is not None else w.number_of_saves, w, params, initial))

@@ -11,6 +11,14 @@
 //                    |u - u_h|^p / (nx*ny*nz), h = 1..H-1, summed over samples,
 //                    divided by M.
 //   structure_basic  StructureBasic.cpp:67-120: one direction, h = 0..H-1.
+//   onepointmoment   OnePointMoment.cpp:30-45 + HostMemory::addPower
+//                    (HostMemory.cpp:283-290): per cell sum over samples (in
+//                    order) of std::pow(u, p), divided by M.
+//   bv               BoundedVariation.cpp:38-53 + HostMemory::getTotalVariation
+//                    (HostMemory.cpp:325-362): per variable, sum over samples
+//                    of sum_cells pow(sqrt((u - u_below)^2 + (u - u_left)^2), p)
+//                    over the reference's index ranges, divided by M (1D/2D);
+//                    each sample's sum is sequential, in the reference's order.
 // Accumulation is FP64 whatever the storage precision.
 #include <hip/hip_runtime.h>
 
@@ -85,6 +93,29 @@ __device__ __forceinline__ double pow_p(double x, double p) {
     if (p == 4.0) return x * (x * (x * x));
     if (p == 5.0) return x * (x * (x * (x * x)));
     return pow(x, p);
+}
+
+// std::pow(x, p) with a run-time exponent, as OnePointMoment and
+// BoundedVariation call it: p = 1 returns x (libm's pow(x, 1) is x, the
+// exact result being representable), p = 2 the correctly rounded x*x (libm's
+// pow(x, 2.0) is within 1 ulp of it), other exponents the device pow.
+__device__ __forceinline__ double ref_pow(double x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    return pow(x, p);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void onepoint_kernel(const T* __restrict__ U,
+                                                      const long long* __restrict__ bases, int m,
+                                                      long long n, double p,
+                                                      double* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int k = 0; k < m; ++k) acc += ref_pow(double(U[bases[k] + i]), p);
+        out[i] = acc / double(m);
+    }
 }
 
 struct StructGeom {
@@ -189,6 +220,58 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restric
         if (threadIdx.x == 0)
             partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
     }
+}
+
+// Total variation of one variable of one sample over the reference's ranges:
+// x in [gx+1, tx-gx), y in [gy+(ty>1), ty-gy), z in [0, tz-gz); the cell
+// below is y-1 (y itself in 1D), the cell to the left x-1. One wave per
+// (variable, sample) walks the cells in the reference's loop order (x
+// fastest), 64 at a time: every lane forms one term, then the wave adds the
+// 64 terms in lane order into one FP64 accumulator, so the sum is the
+// reference's sequential sum, bit for bit. The next batch's loads are issued
+// before the current batch is summed. out[k * nvar + var].
+template <class T>
+__global__ __launch_bounds__(64) void bv_kernel(const T* __restrict__ U,
+                                               const long long* __restrict__ bases, StructGeom g,
+                                               int tz, double p, int nvar, double* out) {
+    const int var = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+    const int x0 = g.gx + 1, x1 = g.tx - g.gx;
+    const int y0 = g.gy + (g.ty > 1 ? 1 : 0), y1 = g.ty - g.gy;
+    const int z1 = tz - g.gz;
+    const long long wx = x1 > x0 ? x1 - x0 : 0, wy = y1 > y0 ? y1 - y0 : 0;
+    const long long ncell = wx * wy * (z1 > 0 ? z1 : 0);
+    const T* u = U + bases[k] + (long long)var * g.vstride;
+    auto load = [&](long long c, double& uc, double& ub, double& ul) {
+        uc = ub = ul = 0;
+        if (c < ncell) {
+            const int x = x0 + int(c % wx);
+            const int y = y0 + int((c / wx) % wy);
+            const int z = int(c / (wx * wy));
+            const int yb = g.ty > 1 ? y - 1 : 0;
+            const long long plane = (long long)z * g.tx * g.ty;
+            uc = double(u[plane + (long long)y * g.tx + x]);
+            ub = double(u[plane + (long long)yb * g.tx + x]);
+            ul = double(u[plane + (long long)y * g.tx + (x - 1)]);
+        }
+    };
+    double acc = 0;
+    double uc, ub, ul;
+    load(lane, uc, ub, ul);
+    for (long long base = 0; base < ncell; base += 64) {
+        const double dyv = uc - ub, dxv = uc - ul;
+        // out-of-range lanes give +0, which leaves the (non-negative) sum as is
+        const double t = ref_pow(sqrt(dyv * dyv + dxv * dxv), p);
+        load(base + 64 + lane, uc, ub, ul);
+        const long long bits = __double_as_longlong(t);
+        const int lo = int(bits), hi = int(bits >> 32);
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+            const unsigned l = unsigned(__builtin_amdgcn_readlane(lo, j));
+            const unsigned h = unsigned(__builtin_amdgcn_readlane(hi, j));
+            acc += __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+        }
+    }
+    if (lane == 0) out[(long long)k * nvar + var] = acc;
 }
 
 // FP64 sum of every variable over the interior cells of each sample:
@@ -315,6 +398,75 @@ int alsvinn_hip_stats_meanvar(alsvinn_hip_ensemble* e, const int* samples, int m
     hipFree(d_var);
     for (hipError_t x : {le, se, c1, c2})
         if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
+    return 0;
+}
+
+int alsvinn_hip_stats_onepoint_moment(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                      double p, double* host_out) {
+    std::vector<long long> bases;
+    if (int rc = group_bases(e, samples, m, bases)) return rc;
+    if (!host_out) return fail(ALSVINN_HIP_E_INVALID, "output is NULL");
+    const SampleDesc& D = e->desc[samples[0]];
+    const long long n = D.vstride * e->nvar;
+    long long* d_bases = nullptr;
+    double* d_out = nullptr;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
+    HIP_TRY(hipMalloc(&d_out, sizeof(double) * n));
+    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+    if (e->esize == 8)
+        hipLaunchKernelGGL(onepoint_kernel<double>, dim3(blocks), dim3(256), 0, e->stream,
+                           (const double*)e->U, d_bases, m, n, p, d_out);
+    else
+        hipLaunchKernelGGL(onepoint_kernel<float>, dim3(blocks), dim3(256), 0, e->stream,
+                           (const float*)e->U, d_bases, m, n, p, d_out);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    hipError_t c1 = hipMemcpy(host_out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost);
+    hipFree(d_bases);
+    hipFree(d_out);
+    for (hipError_t x : {le, se, c1})
+        if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
+    return 0;
+}
+
+int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                        double p, double* host_out) {
+    if (!e || !host_out) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    if (e->dim > 2) return fail(ALSVINN_HIP_E_INVALID, "bv: not supported for 3d (as in the reference)");
+    std::vector<long long> bases;
+    if (int rc = group_bases(e, samples, m, bases)) return rc;
+    const SampleDesc& D = e->desc[samples[0]];
+    const size_t np = (size_t)m * e->nvar;
+    long long* d_bases = nullptr;
+    double* d_tv = nullptr;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
+    HIP_TRY(hipMalloc(&d_tv, sizeof(double) * np));
+    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    const dim3 grid(e->nvar, m);
+    if (e->esize == 8)
+        hipLaunchKernelGGL(bv_kernel<double>, grid, dim3(64), 0, e->stream, (const double*)e->U,
+                           d_bases, geom_of(e, D), D.tz, p, e->nvar, d_tv);
+    else
+        hipLaunchKernelGGL(bv_kernel<float>, grid, dim3(64), 0, e->stream, (const float*)e->U,
+                           d_bases, geom_of(e, D), D.tz, p, e->nvar, d_tv);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    std::vector<double> tv(np);
+    hipError_t ce = hipMemcpy(tv.data(), d_tv, sizeof(double) * np, hipMemcpyDeviceToHost);
+    hipFree(d_bases);
+    hipFree(d_tv);
+    for (hipError_t x : {le, se, ce})
+        if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
+    // the statistic's snapshot adds the samples' variations in order, then
+    // the combine divides by M
+    for (int var = 0; var < e->nvar; ++var) {
+        double acc = 0;
+        for (int k = 0; k < m; ++k) acc += tv[(size_t)k * e->nvar + var];
+        host_out[var] = acc / double(m);
+    }
     return 0;
 }
 

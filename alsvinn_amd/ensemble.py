@@ -213,6 +213,21 @@ class Ensemble:
                                                     _dp(var)))
         return mean, var
 
+    def onepoint_moment(self, p, samples=None):
+        """OnePointMoment: per cell mean over the samples of u**p."""
+        arr, ptr, m = self._group(samples)
+        out = np.empty(self.volume_shape(int(arr[0])))
+        check(_lib.load().alsvinn_hip_stats_onepoint_moment(self.h, ptr, m, float(p), _dp(out)))
+        return out
+
+    def bounded_variation(self, p=1, samples=None):
+        """BoundedVariation: per variable mean over the samples of the total
+        variation (p-th powers of the gradient magnitudes), 1D and 2D."""
+        arr, ptr, m = self._group(samples)
+        out = np.zeros(self.nvar)
+        check(_lib.load().alsvinn_hip_stats_bounded_variation(self.h, ptr, m, float(p), _dp(out)))
+        return out
+
     def structure_cube(self, number_of_h, p=2.0, samples=None):
         arr, ptr, m = self._group(samples)
         out = np.zeros((self.nvar, number_of_h))

@@ -366,9 +366,15 @@ def main():
         t1 = time.perf_counter()
         sf = e.structure_cube(33, 2.0)
         t2 = time.perf_counter()
+        e.onepoint_moment(3)
+        t3 = time.perf_counter()
+        e.bounded_variation(1)
+        t4 = time.perf_counter()
         extra["statistics"] = {"meanvar_ms": 1e3 * (t1 - t0), "structure_cube_h32_ms": 1e3 * (t2 - t1),
+                               "onepointmoment_p3_ms": 1e3 * (t3 - t2), "bv_p1_ms": 1e3 * (t4 - t3),
                                "samples": M, "note": "device kernels + host copy of the result"}
-        log("statistics: meanvar %.1f ms, structure_cube(H=33) %.1f ms" % (1e3 * (t1 - t0), 1e3 * (t2 - t1)))
+        log("statistics: meanvar %.1f ms, structure_cube(H=33) %.1f ms, m3 %.1f ms, bv_1 %.1f ms"
+            % (1e3 * (t1 - t0), 1e3 * (t2 - t1), 1e3 * (t3 - t2), 1e3 * (t4 - t3)))
     e.close()
 
     plain = None

@@ -245,6 +245,21 @@ int alsvinn_hip_ensemble_last_elapsed_ms(alsvinn_hip_ensemble* e, double* ms);
 int alsvinn_hip_stats_meanvar(alsvinn_hip_ensemble* e, const int* samples, int m, int welford,
                               double* host_mean, double* host_var);
 
+/* OnePointMoment (alsuq/src/stats/OnePointMoment.cpp:30-45 with
+ * HostMemory::addPower, HostMemory.cpp:283-290, and the combine's division by
+ * M, StatisticsHelper.cpp:35-79): per cell (ghosts included), the sum over the
+ * listed samples, in order, of std::pow(u, p), divided by M. Output in the
+ * volume layout of the samples. Bit-identical to the reference for p = 1, 2. */
+int alsvinn_hip_stats_onepoint_moment(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                      double p, double* host_out);
+/* BoundedVariation (BoundedVariation.cpp:38-53 with
+ * HostMemory::getTotalVariation, HostMemory.cpp:325-362): per variable, the
+ * sum over the listed samples of sum_cells pow(sqrt((u - u_below)^2 +
+ * (u - u_left)^2), p) over the reference's index ranges, divided by M; 1D and
+ * 2D (the reference rejects 3D). out: nvar reals. */
+int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samples, int m,
+                                        double p, double* host_out);
+
 /* structure_cube (alsuq/src/stats/StructureCube.cpp:37-76,
  * alsfvm/include/alsfvm/functional/structure_common.hpp:37-150): out[var*H + h]
  * for h = 1..H-1 (out[var*H] = 0), combined over the group (divided by m). */

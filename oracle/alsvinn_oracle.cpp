@@ -30,6 +30,8 @@
 //   grid                   alsfvm/src/grid/Grid.cpp:24-41,148-171
 //   sampler                alsuq/src/generator/STLMersenne.cpp:25-85, Uniform.cpp:27-34
 //   meanvar                alsuq/src/stats/MeanVariance.cpp:30-67, StatisticsHelper.cpp:35-79
+//   onepointmoment         alsuq/src/stats/OnePointMoment.cpp:30-45, HostMemory.cpp:283-290
+//   bv                     alsuq/src/stats/BoundedVariation.cpp:38-53, HostMemory.cpp:325-362
 //   structure functions    alsuq/src/stats/StructureBasic.cpp:67-120,
 //                          alsfvm/include/alsfvm/functional/structure_common.hpp:37-180,
 //                          alsfvm/include/alsfvm/boundary/ValueAtBoundary.hpp:20-98
@@ -1056,6 +1058,10 @@ unsigned long long oracle_mt19937_64_nth(long n) {
 // meanvar over M full volumes (variable-major, incl. ghosts) stored
 // contiguously: samples[s * len + i]. MeanVariance.cpp:30-67 and
 // StatisticsHelper.cpp:65-74 (sum in sample order, /= M, var = E[u^2]-E[u]^2).
+// The reference's addPower(u, 2) and subtractPower(mean, 2.) call libm's pow
+// with a run-time exponent; here pow(., 2.0) has a constant exponent, which
+// the compiler evaluates as the correctly rounded square (libm's pow(x, 2.0)
+// is within 1 ulp of it; the tests bound the difference).
 void oracle_meanvar(long nsamples, long len, const double* samples, double* mean, double* var) {
     for (long i = 0; i < len; ++i) {
         mean[i] = 0;
@@ -1096,6 +1102,59 @@ static inline double valueAt(const orc::Vol& V, const double* q, int x, int y, i
         }
     }
     return q[V.idx(P[0] + V.ngx, P[1] + V.ngy, P[2] + V.ngz)];
+}
+
+// std::pow(x, p) with a run-time exponent (HostMemory::addPower,
+// HostMemory.cpp:283-290; HostMemory::getTotalVariation, HostMemory.cpp:
+// 325-362). p = 1 gives x: libm's pow(x, 1) returns x, the exact result being
+// representable. p = 2 gives the correctly rounded x*x: glibc's pow(x, 2.0)
+// is within 1 ulp of it (glibc 2.35 differs in the last bit for about 0.08 %
+// of inputs, and x86-64 glibc picks an FMA or a non-FMA pow at run time, so
+// the reference's own last bit there depends on the machine). Other exponents
+// call libm's pow, as the reference does.
+static inline double statPow(double x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    return std::pow(x, p);
+}
+
+// OnePointMoment over M full volumes stored contiguously (samples[s * len +
+// i]): out[i] = (sum over s in order of pow(u_s[i], p)) / M.
+// OnePointMoment.cpp:30-45 (addPower per sample) and StatisticsHelper.cpp:
+// 65-74 (division by the number of samples).
+void oracle_onepoint_moment(long nsamples, long len, const double* samples, double p,
+                            double* out) {
+    for (long i = 0; i < len; ++i) out[i] = 0;
+    for (long s = 0; s < nsamples; ++s) {
+        const double* u = samples + s * len;
+        for (long i = 0; i < len; ++i) out[i] += statPow(u[i], p);
+    }
+    for (long i = 0; i < len; ++i) out[i] /= double(nsamples);
+}
+
+// Total variation of one variable of one volume of total extents (tx, ty,
+// tz) with ghost layers (gx, gy, gz): HostMemory::getTotalVariation
+// (HostMemory.cpp:325-362) with start = ghosts, end = size - ghosts as
+// BoundedVariation.cpp:38-53 passes them. Returns -1 for 3D (the reference
+// throws).
+double oracle_total_variation(int tx, int ty, int tz, int gx, int gy, int gz, const double* u,
+                              double p) {
+    if (tz > 1) return -1;
+    const int startX = gx + 1;
+    const int startY = gy + (ty > 1 ? 1 : 0);
+    const int endX = tx - gx, endY = ty - gy, endZ = tz - gz;
+    double bv = 0;
+    for (int z = 0; z < endZ; z++)
+        for (int y = startY; y < endY; y++)
+            for (int x = startX; x < endX; x++) {
+                const long idx = (long)z * tx * ty + (long)y * tx + x;
+                const long left = (long)z * tx * ty + (long)y * tx + (x - 1);
+                const int yBottom = ty > 1 ? y - 1 : 0;
+                const long below = (long)z * tx * ty + (long)yBottom * tx + x;
+                const double dy = u[idx] - u[below], dx = u[idx] - u[left];
+                bv += statPow(std::sqrt(dy * dy + dx * dx), p);
+            }
+    return bv;
 }
 
 // Adds one sample's structure_cube contribution to out[var * H + h]

@@ -69,6 +69,9 @@ def lib():
         L.oracle_mt19937_64_nth.argtypes = [ctypes.c_long]
         L.oracle_mt19937_64_nth.restype = ctypes.c_ulonglong
         L.oracle_meanvar.argtypes = [ctypes.c_long, ctypes.c_long, dp, dp, dp]
+        L.oracle_onepoint_moment.argtypes = [ctypes.c_long, ctypes.c_long, dp, ctypes.c_double, dp]
+        L.oracle_total_variation.argtypes = [ctypes.c_int] * 6 + [dp, ctypes.c_double]
+        L.oracle_total_variation.restype = ctypes.c_double
         L.oracle_structure_cube.argtypes = [ctypes.c_int] * 5 + [dp, ctypes.c_int, ctypes.c_double,
                                                                 ctypes.c_int, dp]
         L.oracle_structure_basic.argtypes = [ctypes.c_int] * 5 + [dp, ctypes.c_int, ctypes.c_double,
@@ -200,6 +203,36 @@ def meanvar(samples):
     var = np.zeros(flat.shape[1])
     lib().oracle_meanvar(M, flat.shape[1], _dp(flat), _dp(mean), _dp(var))
     return mean.reshape(samples.shape[1:]), var.reshape(samples.shape[1:])
+
+
+def onepoint_moment(samples, p):
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    M = samples.shape[0]
+    flat = samples.reshape(M, -1)
+    out = np.zeros(flat.shape[1])
+    lib().oracle_onepoint_moment(M, flat.shape[1], _dp(flat), float(p), _dp(out))
+    return out.reshape(samples.shape[1:])
+
+
+def total_variation(u, ghosts, p):
+    """u: one variable (tz, ty, tx) incl. ghost layers ghosts = (gx, gy, gz)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    tz, ty, tx = u.shape
+    return lib().oracle_total_variation(tx, ty, tz, ghosts[0], ghosts[1], ghosts[2], _dp(u),
+                                        float(p))
+
+
+def bounded_variation(samples, ghosts, p):
+    """BoundedVariation over samples (M, nvar, tz, ty, tx): per variable, the
+    sum over samples in order of the total variation, divided by M."""
+    M, nvar = samples.shape[:2]
+    out = np.zeros(nvar)
+    for var in range(nvar):
+        acc = 0.0
+        for k in range(M):
+            acc += total_variation(samples[k, var], ghosts, p)
+        out[var] = acc / M
+    return out
 
 
 def structure_cube(u, n, ng, H, p, periodic, out=None):

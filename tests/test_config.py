@@ -21,7 +21,7 @@ def test_read_kh_uq_config():
         ("euler2", "hll3", "weno2", "rungekutta3", "periodic")
     assert c.cfl == 0.475 and c.end_time == 0.06 and c.gamma == 1.4
     assert c.samples == 3 and c.has_uq and [p.name for p in c.uq_parameters] == ["a"]
-    assert [s.name for s in c.stats] == ["meanvar", "structure_cube"]
+    assert [s.name for s in c.stats] == ["meanvar", "structure_cube", "onepointmoment", "bv"]
     assert c.stats[1].params == {"numberOfH": 5.0, "p": 2.0}
     assert c.writer.number_of_saves == 2 and c.writer.basename == "kh_uq"
 
@@ -61,6 +61,19 @@ def test_rejected_options(change, message, tmp_path):
     path = _copy_config(tmp_path, src.replace(change, repl, 1))
     with pytest.raises(config.ConfigError, match=message):
         config.read_config(path)
+
+
+def test_onepoint_and_bv_need_integer_p(tmp_path):
+    """OnePointMoment and BoundedVariation read p with getInteger."""
+    src = open(os.path.join(HERE, "kh_uq.xml")).read()
+    c = config.read_config(os.path.join(HERE, "kh_uq.xml"))
+    assert [(s.name, s.params.get("p")) for s in c.stats][2:] == [("onepointmoment", 2.0),
+                                                                  ("bv", 1.0)]
+    bad = src.replace("<name>bv</name>\n      <numberOfSaves>1</numberOfSaves>\n      <p>1</p>",
+                      "<name>bv</name>\n      <numberOfSaves>1</numberOfSaves>\n      <p>1.5</p>")
+    assert bad != src
+    with pytest.raises(config.ConfigError, match="integer <p>"):
+        config.read_config(_copy_config(tmp_path, bad))
 
 
 def test_init_global_matches_library_kelvin_helmholtz():

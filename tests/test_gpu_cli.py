@@ -1,9 +1,10 @@
 """The configuration front end end to end on the GPU: a reference-format UQ
 configuration (3 Kelvin-Helmholtz samples, writer with 2 saves, meanvar and
-structure_cube at the end time) run by alsvinn_amd.cli; every snapshot in the
-writer's NetCDF file equals the oracle's state at that save time bit for bit,
-the meanvar files equal the oracle's sequential statistics bit for bit and
-the structure function agrees to 1e-12 (parallel summation)."""
+structure_cube, onepointmoment p = 2 and bv p = 1 at the end time) run by
+alsvinn_amd.cli; every snapshot in the writer's NetCDF file equals the
+oracle's state at that save time bit for bit, the meanvar, m2 and bv_1 files
+equal the oracle's sequential statistics bit for bit and the structure
+function agrees to 1e-12 (parallel summation)."""
 import os
 
 import numpy as np
@@ -57,6 +58,23 @@ def test_cli_uq_run_matches_oracle(hip, tmp_path):
     try:
         got = np.asarray(g.variables["rho"][0, 0], dtype=np.float64).ravel()
         np.testing.assert_allclose(got, sref[0], rtol=1e-12, atol=0)
+    finally:
+        g.close()
+    m2 = O.onepoint_moment(np.stack(finals), 2)
+    bv1 = O.bounded_variation(np.stack(finals), (G, G, 0), 1)
+    g = scipy.io.netcdf_file(str(tmp_path / "kh_uq_m2"), "r", mmap=False)
+    try:
+        for name, var in (("rho", 0), ("E", 3), ("mx", 1), ("my", 2)):
+            slab = np.asarray(g.variables[name][0, 0], dtype=np.float64)
+            inner = m2[var, 0, G:-G, G:-G].ravel().reshape(n, n)
+            assert np.array_equal(slab.view(np.uint64), inner.view(np.uint64)), name
+    finally:
+        g.close()
+    g = scipy.io.netcdf_file(str(tmp_path / "kh_uq_bv_1"), "r", mmap=False)
+    try:
+        for name, var in (("rho", 0), ("E", 3), ("mx", 1), ("my", 2)):
+            got = np.asarray(g.variables[name][0, 0], dtype=np.float64).ravel()
+            assert got.shape == (1,) and got[0] == bv1[var], name
     finally:
         g.close()
     assert summary["samples"] == 3

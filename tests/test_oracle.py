@@ -3,6 +3,8 @@ closed forms, conservation/symmetry invariants, convergence rates and an
 exact Riemann solution. Each test names the reference test or the
 mathematics that fixes the expected value.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -486,6 +488,107 @@ def test_meanvar_closed_form():
     mean, var = O.meanvar(samples)
     np.testing.assert_allclose(mean, base + (M - 1) / 2, rtol=1e-14)
     np.testing.assert_allclose(var, np.full_like(base, (M * M - 1) / 12), rtol=1e-10)
+
+
+def test_meanvar_within_an_ulp_of_runtime_pow():
+    """The reference accumulates the second moment with libm's pow(u, 2) at
+    run time (HostMemory::addPower) and subtracts pow(mean, 2.); the oracle
+    squares. math.pow is libm's pow: the means agree exactly and the
+    variances within 2 ulp of E[u^2] plus 1 ulp of mean^2."""
+    rng = np.random.default_rng(11)
+    M, n = 5, 400
+    samples = rng.standard_normal((M, n)) * rng.uniform(0.1, 10, n)
+    mean, var = O.meanvar(samples)
+    s1 = [0.0] * n
+    s2 = [0.0] * n
+    for k in range(M):
+        for i in range(n):
+            s1[i] += samples[k, i]
+            s2[i] += math.pow(samples[k, i], 2.0)
+    m_ref = np.array([a / M for a in s1])
+    e2 = np.array([b / M for b in s2])
+    v_ref = np.array([e2[i] - math.pow(m_ref[i], 2.0) for i in range(n)])
+    assert np.array_equal(mean, m_ref)
+    assert np.all(np.abs(var - v_ref) <= 2 * np.spacing(e2) + np.spacing(m_ref ** 2))
+
+
+def test_onepoint_moment_closed_forms():
+    """OnePointMoment.cpp:30-45: samples c + k, k = 0..M-1 -> m1 = c + (M-1)/2
+    (equal to meanvar's mean bit for bit), m2 = c^2 + c(M-1) + (M-1)(2M-1)/6,
+    m3 = c^3 + 3c^2(M-1)/2 + c(M-1)(2M-1)/2 + M(M-1)^2/4."""
+    M = 7
+    c = np.linspace(-2, 3, 30).reshape(5, 6)
+    samples = np.stack([c + k for k in range(M)])
+    m1 = O.onepoint_moment(samples, 1)
+    assert np.array_equal(m1, O.meanvar(samples)[0])
+    np.testing.assert_allclose(O.onepoint_moment(samples, 2),
+                               c ** 2 + c * (M - 1) + (M - 1) * (2 * M - 1) / 6, rtol=1e-13)
+    np.testing.assert_allclose(O.onepoint_moment(samples, 3),
+                               c ** 3 + 3 * c ** 2 * (M - 1) / 2 + c * (M - 1) * (2 * M - 1) / 2
+                               + M * (M - 1) ** 2 / 4, rtol=1e-12, atol=1e-12)
+
+
+def test_onepoint_moment_within_an_ulp_of_runtime_pow():
+    """p = 2 squares, p = 3 calls libm's pow as the reference does: each term
+    is within 1 ulp of math.pow (the same libm) for p = 2, equal for p = 3."""
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((1, 2000)) * 3
+    for p, ulps in ((2, 1), (3, 0)):
+        got = O.onepoint_moment(u, p)
+        ref = np.array([math.pow(x, float(p)) for x in u[0]])
+        assert np.all(np.abs(got - ref) <= ulps * np.spacing(np.abs(ref)))
+
+
+def _with_ghosts(inner, g, dim, fill):
+    """(tz, ty, tx) volume with `fill` in the ghost layers."""
+    if dim == 1:
+        out = np.full((1, 1, inner.shape[-1] + 2 * g), fill)
+        out[0, 0, g:-g] = inner
+    else:
+        out = np.full((1, inner.shape[0] + 2 * g, inner.shape[1] + 2 * g), fill)
+        out[0, g:-g, g:-g] = inner
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_total_variation_linear_closed_form(p):
+    """HostMemory::getTotalVariation on u = a*i + b*j over the interior: every
+    term is sqrt(a^2 + b^2)^p and there are (nx-1)(ny-1) of them (the first
+    interior row and column only serve as neighbours); in 1D (nx-1)|a|^p.
+    Ghost layers hold 1e9 and must not enter."""
+    nx, ny, g, a, b = 12, 9, 2, 0.75, -1.5
+    j, i = np.mgrid[0:ny, 0:nx]
+    u2 = _with_ghosts(a * i + b * j, g, 2, 1e9)
+    assert O.total_variation(u2, (g, g, 0), p) == pytest.approx(
+        (nx - 1) * (ny - 1) * math.sqrt(a * a + b * b) ** p, rel=1e-13)
+    u1 = _with_ghosts(a * np.arange(nx, dtype=float), g, 1, 1e9)
+    assert O.total_variation(u1, (g, 0, 0), p) == pytest.approx((nx - 1) * abs(a) ** p, rel=1e-13)
+
+
+def test_total_variation_steps_and_constants():
+    """A unit step across x = s contributes one unit term per row from the
+    second row on (ny-1 terms); across y = s one per column from the second
+    column on (nx-1); a constant has zero variation; 3D is rejected."""
+    nx, ny, g, s = 10, 7, 1, 4
+    j, i = np.mgrid[0:ny, 0:nx]
+    for p in (1, 2, 3):
+        assert O.total_variation(_with_ghosts((i >= s) * 1.0, g, 2, -5.0), (g, g, 0), p) == ny - 1
+        assert O.total_variation(_with_ghosts((j >= s) * 1.0, g, 2, -5.0), (g, g, 0), p) == nx - 1
+    assert O.total_variation(_with_ghosts(np.full((ny, nx), 2.5), g, 2, 7.0), (g, g, 0), 1) == 0
+    assert O.total_variation(np.zeros((4, 4, 4)), (0, 0, 0), 1) == -1
+
+
+def test_bounded_variation_mean_over_samples():
+    """BoundedVariation.cpp:38-53: the per-variable variations of the samples
+    summed in order and divided by M: samples k*u give TV(u) * (M-1)/2 for
+    p = 1."""
+    nx, ny, g, M = 8, 6, 2, 5
+    rng = np.random.default_rng(3)
+    u = _with_ghosts(rng.random((ny, nx)), g, 2, 0.0)
+    samples = np.stack([np.stack([k * u, -k * u]) for k in range(M)])
+    bv = O.bounded_variation(samples, (g, g, 0), 1)
+    tv = O.total_variation(u, (g, g, 0), 1)
+    np.testing.assert_allclose(bv, [tv * (M - 1) / 2] * 2, rtol=1e-13)
 
 
 def _structure_linear_expect(N, h, p, periodic):
