@@ -87,7 +87,7 @@ int validate(const alsvinn_hip_scheme* s) {
     if (!s) return fail(ALSVINN_HIP_E_INVALID, "scheme is NULL");
     if (s->equation < 0 || s->equation > 3)
         return fail(ALSVINN_HIP_E_INVALID, "unknown equation " + std::to_string(s->equation));
-    if (s->reconstruction < 0 || s->reconstruction > 4)
+    if (s->reconstruction < 0 || s->reconstruction > 5)
         return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction " + std::to_string(s->reconstruction));
     if (s->integrator < 0 || s->integrator > 2)
         return fail(ALSVINN_HIP_E_INVALID, "unknown integrator " + std::to_string(s->integrator));
@@ -99,6 +99,8 @@ int validate(const alsvinn_hip_scheme* s) {
         s->flux != ALSVINN_HIP_FLUX_CENTRAL && s->flux != ALSVINN_HIP_FLUX_RUSANOV &&
         s->flux != ALSVINN_HIP_FLUX_ROE)
         return fail(ALSVINN_HIP_E_INVALID, "burgers supports the godunov, central, rusanov and roe fluxes");
+    if (s->equation == ALSVINN_HIP_EQ_BURGERS && s->reconstruction == ALSVINN_HIP_RECON_WENOF2)
+        return fail(ALSVINN_HIP_E_INVALID, "wenof2 needs the Euler equations (as in the reference)");
     if (s->equation != ALSVINN_HIP_EQ_BURGERS && s->flux != ALSVINN_HIP_FLUX_HLL &&
         s->flux != ALSVINN_HIP_FLUX_HLL3 && s->flux != ALSVINN_HIP_FLUX_CENTRAL)
         return fail(ALSVINN_HIP_E_INVALID, "euler supports the hll, hll3 and central fluxes");

@@ -91,6 +91,12 @@ int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
         case ALSVINN_HIP_RECON_ENO2: set_stage<T, Eq, ReconENO2, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_MC: set_stage<T, Eq, ReconMC, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_MM: set_stage<T, Eq, ReconMM, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_WENOF2:
+            if constexpr (Eq::is_euler) {
+                set_stage<T, Eq, ReconWENOF2, Flux, DIM>(k);
+                return 0;
+            }
+            return fail(ALSVINN_HIP_E_INVALID, "wenof2 needs the Euler equations");
     }
     return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction");
 }

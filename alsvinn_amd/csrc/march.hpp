@@ -330,9 +330,9 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), vs, yc);
             load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), vs, yp);
             T Lt[NV], Rt[NV];
-            Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt);  // row r-1: keep R
+            Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt, eq.gamma);  // row r-1: keep R
             T L0[NV], R0[NV];
-            Recon::template apply<T, NV>(ym, yc, yp, L0, R0);  // row r
+            Recon::template apply<T, NV>(ym, yc, yp, L0, R0, eq.gamma);  // row r
             const T s = face_flux<T, Eq, Flux, 1>(eq, Rt, L0, Gm);
             if (interior) wsy = std_max(wsy, s);
 #pragma unroll
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                     xp[v] = lane_from_upper(yc[v]);
                 }
                 T Lx[NV], Rx[NV];
-                Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx);
+                Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx, eq.gamma);
                 T Ln[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             if constexpr (DIM >= 2) {
                 pn.finish(LAZY, c, qn);
                 T Ln[NV], Rn[NV];
-                Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn);  // row r+1
+                Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn, eq.gamma);  // row r+1
                 T Gp[NV];
                 const T s = face_flux<T, Eq, Flux, 1>(eq, Ry, Ln, Gp);
                 if (interior) wsy = std_max(wsy, s);
@@ -501,8 +501,8 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), vs, zc);
             load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), vs, zp);
             T Lt[NV], Rt[NV], L0[NV], R0[NV];
-            Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt);
-            Recon::template apply<T, NV>(zm, zc, zp, L0, R0);
+            Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt, eq.gamma);
+            Recon::template apply<T, NV>(zm, zc, zp, L0, R0, eq.gamma);
             const T s = face_flux<T, Eq, Flux, 2>(eq, Rt, L0, Hm);
             if (interior) wsz = std_max(wsz, s);
 #pragma unroll
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
                     xp[v] = lane_from_upper(zc[v]);
                 }
                 T Lx[NV], Rx[NV];
-                Recon::template apply<T, NV>(xm, zc, xp, Lx, Rx);
+                Recon::template apply<T, NV>(xm, zc, xp, Lx, Rx, eq.gamma);
                 T Ln[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
@@ -570,20 +570,20 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
                     ym1[v] = s_rows[w + 1][v][lane];
                     yp1[v] = s_rows[w + 3][v][lane];
                 }
-                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0);
+                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0, eq.gamma);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) s_L[w][v][lane] = L0[v];
                 if (lo_halo) {  // right value of row y0-1
                     T q0[NV], Lb[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) q0[v] = s_rows[0][v][lane];
-                    Recon::template apply<T, NV>(q0, ym1, zc, Lb, Rb);
+                    Recon::template apply<T, NV>(q0, ym1, zc, Lb, Rb, eq.gamma);
                 }
                 if (hi_halo) {  // left value of row y0+NW
                     T q1[NV], La[NV], Ra[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) q1[v] = s_rows[NW + 3][v][lane];
-                    Recon::template apply<T, NV>(zc, yp1, q1, La, Ra);
+                    Recon::template apply<T, NV>(zc, yp1, q1, La, Ra, eq.gamma);
 #pragma unroll
                     for (int v = 0; v < NV; ++v) s_L[NW][v][lane] = La[v];
                 }
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             {
                 pq.finish(LAZY, c, qn);
                 T Ln[NV], Rn[NV];
-                Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn);
+                Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn, eq.gamma);
                 T Hp[NV];
                 const T s = face_flux<T, Eq, Flux, 2>(eq, Rz, Ln, Hp);
                 if (interior) wsz = std_max(wsz, s);

@@ -301,7 +301,7 @@ struct ReconNone {
     static constexpr int ghosts = 1;
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R) {
+                                                 T* R, T /*gamma*/) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             L[v] = q0[v];
@@ -314,7 +314,7 @@ struct ReconWENO2 {
     static constexpr int ghosts = 2;
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R) {
+                                                 T* R, T /*gamma*/) {
         // eps = 1e-8 (WENOCoefficients.hpp:21); the FP64 path evaluates the
         // reference's expressions exactly.
         const T eps = T(1e-8);
@@ -347,11 +347,96 @@ struct ReconWENO2 {
     }
 };
 
+// WENOF2 (WENOF2.hpp:31-115), Euler only: WENO2's weights from the density
+// applied to the slopes of the primitive variables (rho, u, p); the density
+// and pressure slopes are clamped to +-1.9 times the middle value, and each
+// slope is scaled by 0.5 * sqrt(R / max(R, LL)) with R = p / (gamma - 1) and
+// LL the kinetic-energy change the slope would cause, which keeps the
+// reconstructed pressure positive. Expressions in the reference's order.
+struct ReconWENOF2 {
+    static constexpr int ghosts = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void prim(const T* q, T* P, T gamma) {
+        // Euler::computePrimitiveVariables (Euler.hpp:290-297): rho, m/rho,
+        // (gamma-1)(E - 0.5 m.m / rho)
+        constexpr int NSD = NV - 2;
+        T dot = q[1] * q[1];
+        if (NSD > 1) dot = dot + q[2] * q[2];
+        if (NSD > 2) dot = dot + q[3] * q[3];
+        P[0] = q[0];
+#pragma unroll
+        for (int k = 0; k < NSD; ++k) P[1 + k] = dv(q[1 + k], q[0]);
+        P[NV - 1] = (gamma - T(1)) * (q[NV - 1] - dv(T(0.5) * dot, q[0]));
+    }
+    template <class T, int NV>
+    __device__ __forceinline__ static T dotu(const T* a, const T* b) {
+        constexpr int NSD = NV - 2;
+        T d = a[1] * b[1];
+        if (NSD > 1) d = d + a[2] * b[2];
+        if (NSD > 2) d = d + a[3] * b[3];
+        return d;
+    }
+    // the primitive slope dP (in place) clamped, its scale a, and the
+    // conserved state of P0 + sgn * a * dP (Euler::computeConserved,
+    // Euler.hpp:236-245)
+    template <class T, int NV>
+    __device__ __forceinline__ static void side(const T* P0, T* dP, T Rr, T sgn, T gamma, T* out) {
+        const T BOUND = T(1.9);
+        dP[0] = ref_fmax(-BOUND * P0[0], ref_fmin(BOUND * P0[0], dP[0]));
+        dP[NV - 1] = ref_fmax(-BOUND * P0[NV - 1], ref_fmin(BOUND * P0[NV - 1], dP[NV - 1]));
+        const T duu = dotu<T, NV>(dP, dP);
+        const T LL = (T(0.125) * P0[0] * duu - T(0.5) * ref_fmin(T(0), dP[0] * dotu<T, NV>(P0, dP))) +
+                     dv(T(0.5) * dP[0] * dP[0] * duu, P0[0]);
+        const T a = T(0.5) * dsqrt(dv(Rr, ref_fmax(Rr, LL)));
+        T P[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) P[v] = sgn > T(0) ? P0[v] + a * dP[v] : P0[v] - a * dP[v];
+        out[0] = P[0];
+#pragma unroll
+        for (int k = 1; k < NV - 1; ++k) out[k] = P[k] * P[0];
+        out[NV - 1] = dv(P[NV - 1], gamma - T(1)) + T(0.5) * P[0] * dotu<T, NV>(P, P);
+    }
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R, T gamma) {
+        const T eps = T(1e-8);  // WENOCoefficients.hpp:21
+        const T i0 = q0[0];
+        const T d0 = qp[0] - i0;
+        const T d1 = i0 - qm[0];
+        const T e0 = eps + d0 * d0;
+        const T e1 = eps + d1 * d1;
+        const T D0 = T(3) * e0 * e0;
+        const T D1 = T(3) * e1 * e1;
+        // 2/D == 2*(1/D) exactly while 1/D is a normal number (as in WENO2)
+        const T a0L = dv(T(1), D0);
+        const T a1R = dv(T(1), D1);
+        const T a1L = T(2) * a1R;
+        const T a0R = T(2) * a0L;
+        const T w0L = dv(a0L, a0L + a1L);
+        const T w1L = dv(a1L, a0L + a1L);
+        const T w0R = dv(a0R, a0R + a1R);
+        const T w1R = dv(a1R, a0R + a1R);
+        T Pm[NV], P0[NV], Pp[NV];
+        prim<T, NV>(qm, Pm, gamma);
+        prim<T, NV>(q0, P0, gamma);
+        prim<T, NV>(qp, Pp, gamma);
+        T dL[NV], dR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            dL[v] = w1L * (P0[v] - Pm[v]) + w0L * (Pp[v] - P0[v]);
+            dR[v] = w1R * (P0[v] - Pm[v]) + w0R * (Pp[v] - P0[v]);
+        }
+        const T Rr = dv(P0[NV - 1], gamma - T(1));
+        side<T, NV>(P0, dL, Rr, T(-1), gamma, L);
+        side<T, NV>(P0, dR, Rr, T(1), gamma, R);
+    }
+};
+
 struct ReconENO2 {
     static constexpr int ghosts = 2;
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R) {
+                                                 T* R, T /*gamma*/) {
         // Divided differences of variable 0: dd_{i-1} = q_i - q_{i-1},
         // dd_i = q_{i+1} - q_i; shift left when |dd_{i-1}| < |dd_i|.
         const T ddl = q0[0] - qm[0];
@@ -402,7 +487,7 @@ struct ReconMC {
     static constexpr int ghosts = 2;
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R) {
+                                                 T* R, T /*gamma*/) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const T sigma = minmod3(T(2) * (qp[v] - q0[v]), (qp[v] - qm[v]) / T(2),
@@ -417,7 +502,7 @@ struct ReconMM {
     static constexpr int ghosts = 2;
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R) {
+                                                 T* R, T /*gamma*/) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const T sigma = minmod2(qp[v] - q0[v], q0[v] - qm[v]);

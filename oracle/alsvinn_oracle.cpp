@@ -17,6 +17,8 @@
 //   WENO2                  alsfvm/include/alsfvm/reconstruction/WENO2.hpp:31-95 (eps 1e-8,
 //                          WENOCoefficients.hpp:21); ranges ReconstructionCPU.cpp:38-97
 //   ENO2                   alsfvm/src/reconstruction/ENOCPU.cpp:41-267, ENOCoefficients.cpp:27-32
+//   WENOF2                 alsfvm/include/alsfvm/reconstruction/WENOF2.hpp:31-115,
+//                          euler/PrimitiveVariables.hpp, Euler.hpp:236-245,290-297
 //   no reconstruction      alsfvm/src/reconstruction/NoReconstruction.cpp:42-73
 //   region split           alsfvm/src/simulator/ConservedSystem.cpp:31-103
 //   CFL + adjusters        alsfvm/src/integrator/Integrator.cpp:20-66,
@@ -332,7 +334,7 @@ struct Roe {
 
 // ---------------------------------------------------------------- reconstruction
 
-enum { RECON_NONE = 0, RECON_WENO2 = 1, RECON_ENO2 = 2, RECON_MC = 3, RECON_MM = 4 };
+enum { RECON_NONE = 0, RECON_WENO2 = 1, RECON_ENO2 = 2, RECON_MC = 3, RECON_MM = 4, RECON_WENOF2 = 5 };
 
 static int reconGhosts(int recon) { return recon == RECON_NONE ? 1 : 2; }
 
@@ -436,6 +438,112 @@ static void reconWeno2(const Vol& in, int d, Vol& Lo, Vol& Ro, ivec3 start, ivec
             }
         }
     }
+}
+
+// Primitive state of the Euler equations (euler/PrimitiveVariables.hpp):
+// rho, velocity (nsd components), p, with component-wise arithmetic.
+struct Prim {
+    real rho, u[3], p;
+    int nsd;
+};
+static Prim primOf(const Vol& in, size_t i, int nsd, real gamma) {
+    // Euler::computePrimitiveVariables (Euler.hpp:290-297)
+    Prim P;
+    P.nsd = nsd;
+    const real rho = in.v(0)[i];
+    real m[3] = {0, 0, 0};
+    for (int k = 0; k < nsd; ++k) m[k] = in.v(1 + k)[i];
+    const real E = in.v(nsd + 1)[i];
+    real mm = m[0] * m[0];
+    for (int k = 1; k < nsd; ++k) mm = mm + m[k] * m[k];
+    P.rho = rho;
+    for (int k = 0; k < 3; ++k) P.u[k] = k < nsd ? m[k] / rho : 0;
+    P.p = (gamma - 1) * (E - 0.5 * mm / rho);
+    return P;
+}
+static real udot(const Prim& a, const Prim& b) {
+    real d = a.u[0] * b.u[0];
+    for (int k = 1; k < a.nsd; ++k) d = d + a.u[k] * b.u[k];
+    return d;
+}
+static Prim lin(real wa, const Prim& a, real wb, const Prim& b) {  // wa*a + wb*b
+    Prim r = a;
+    r.rho = wa * a.rho + wb * b.rho;
+    for (int k = 0; k < 3; ++k) r.u[k] = wa * a.u[k] + wb * b.u[k];
+    r.p = wa * a.p + wb * b.p;
+    return r;
+}
+static Prim diff(const Prim& a, const Prim& b) {  // a - b
+    Prim r = a;
+    r.rho = a.rho - b.rho;
+    for (int k = 0; k < 3; ++k) r.u[k] = a.u[k] - b.u[k];
+    r.p = a.p - b.p;
+    return r;
+}
+
+// WENOF2.hpp:31-115 (Euler only) inside ReconstructionCPU.cpp:38-97: WENO2
+// weights from the density applied to the primitive slopes; rho and p slopes
+// clamped to +-1.9 times the middle value; each side scaled by
+// 0.5*sqrt(R/max(R, LL)), R = p/(gamma-1); converted back with
+// Euler::computeConserved (Euler.hpp:236-245).
+static void reconWenoF2(const Vol& in, int d, Vol& Lo, Vol& Ro, ivec3 start, ivec3 end,
+                        real gamma) {
+    const int nx = in.tx, ny = in.ty, nz = in.tz;
+    const int dim = in.dims();
+    const int nsd = in.nvar - 2;
+    const int g = 2;
+    const int ngx = g, ngy = (dim > 1) * g, ngz = (dim > 2) * g;
+    const int dx = d == 0, dy = d == 1, dz = d == 2;
+    const int sx = (ngx - dx) + start.x, sy = (ngy - dy) + start.y, sz = (ngz - dz) + start.z;
+    const int ex = nx - (ngx - dx) + end.x, ey = ny - (ngy - dy) + end.y,
+              ez = nz - (ngz - dz) + end.z;
+    const real eps = 1e-8;
+    const real BOUND = 1.9;
+    const real* w = in.v(0);
+    auto store = [&](Vol& out, size_t i, const Prim& P) {
+        out.v(0)[i] = P.rho;
+        for (int k = 0; k < nsd; ++k) out.v(1 + k)[i] = P.rho * P.u[k];
+        out.v(nsd + 1)[i] = P.p / (gamma - 1) + 0.5 * P.rho * udot(P, P);
+    };
+    for (int z = sz; z < ez; z++)
+        for (int y = sy; y < ey; y++)
+            for (int x = sx; x < ex; x++) {
+                const size_t iO = in.idx(x, y, z);
+                const size_t iR = in.idx(x + dx, y + dy, z + dz);
+                const size_t iL = in.idx(x - dx, y - dy, z - dz);
+                const real i0 = w[iO];
+                const real b0 = (w[iR] - i0) * (w[iR] - i0);
+                const real b1 = (i0 - w[iL]) * (i0 - w[iL]);
+                const real a0L = 1 / (3 * (eps + b0) * (eps + b0));
+                const real a1L = 2 / (3 * (eps + b1) * (eps + b1));
+                const real w0L = a0L / (a0L + a1L);
+                const real w1L = a1L / (a0L + a1L);
+                const real a0R = 2 / (3 * (eps + b0) * (eps + b0));
+                const real a1R = 1 / (3 * (eps + b1) * (eps + b1));
+                const real w0R = a0R / (a0R + a1R);
+                const real w1R = a1R / (a0R + a1R);
+                const Prim pL = primOf(in, iL, nsd, gamma);
+                const Prim pM = primOf(in, iO, nsd, gamma);
+                const Prim pR = primOf(in, iR, nsd, gamma);
+                const real Rr = pM.p / (gamma - 1);
+                for (int side = 0; side < 2; ++side) {
+                    const real wl = side == 0 ? w1L : w1R, wr = side == 0 ? w0L : w0R;
+                    Prim dP = lin(wl, diff(pM, pL), wr, diff(pR, pM));
+                    dP.rho = std::fmax(-BOUND * pM.rho, std::fmin(BOUND * pM.rho, dP.rho));
+                    dP.p = std::fmax(-BOUND * pM.p, std::fmin(BOUND * pM.p, dP.p));
+                    const real LL = 0.125 * pM.rho * udot(dP, dP) -
+                                    0.5 * std::fmin(real(0.0), dP.rho * udot(pM, dP)) +
+                                    0.5 * dP.rho * dP.rho * udot(dP, dP) / pM.rho;
+                    const real a = 0.5 * std::sqrt(Rr / std::fmax(Rr, LL));
+                    // left: middle - a*dP, right: middle + a*dP
+                    Prim P = pM;
+                    P.rho = side == 0 ? pM.rho - a * dP.rho : pM.rho + a * dP.rho;
+                    for (int k = 0; k < 3; ++k)
+                        P.u[k] = side == 0 ? pM.u[k] - a * dP.u[k] : pM.u[k] + a * dP.u[k];
+                    P.p = side == 0 ? pM.p - a * dP.p : pM.p + a * dP.p;
+                    store(side == 0 ? Lo : Ro, iO, P);
+                }
+            }
 }
 
 // ENOCPU.cpp:41-185 (order 2) with the divided differences of 205-264.
@@ -601,6 +709,8 @@ struct System : SystemBase {
             reconWeno2(U, d, left, right, s, e);
         else if (recon == RECON_MC || recon == RECON_MM)
             reconLimiter(U, d, left, right, s, e, recon == RECON_MC);
+        else if (recon == RECON_WENOF2)
+            reconWenoF2(U, d, left, right, s, e, eq.gamma);
         else
             reconEno2(U, d, left, right, dd, s, e);
     }
@@ -839,7 +949,8 @@ struct Sim {
 
 static bool validSetup(const Setup& s) {
     if (s.equation < 0 || s.equation > 3) return false;
-    if (s.recon < 0 || s.recon > 4) return false;
+    if (s.recon < 0 || s.recon > 5) return false;
+    if (s.recon == RECON_WENOF2 && s.equation == 0) return false;  // Euler only
     if (s.integrator < 0 || s.integrator > 2) return false;
     if (s.num_adjusters < 0 || s.num_adjusters > 2) return false;
     if (s.nx < 1 || s.ny < 1 || s.nz < 1) return false;
@@ -989,7 +1100,7 @@ void oracle_point_flux(int nsd, int direction, double gamma, const double* q, do
 // Reconstruction of a 1D line (nvar variables, n cells incl. ghosts):
 // left/right face values at every cell where the reference computes them.
 int oracle_reconstruct_1d(int recon, int nvar, int n, const double* q, double* left,
-                          double* right) {
+                          double* right, double gamma) {
     const int ng = orc::reconGhosts(recon);
     if (n <= 2 * ng) return 1;
     orc::Vol U(nvar, n - 2 * ng, 1, 1, ng), L(nvar, n - 2 * ng, 1, 1, ng),
@@ -1002,7 +1113,10 @@ int oracle_reconstruct_1d(int recon, int nvar, int n, const double* q, double* l
     else if (recon == orc::RECON_WENO2) orc::reconWeno2(U, 0, L, R, orc::ivec3(), orc::ivec3());
     else if (recon == orc::RECON_MC || recon == orc::RECON_MM)
         orc::reconLimiter(U, 0, L, R, orc::ivec3(), orc::ivec3(), recon == orc::RECON_MC);
-    else orc::reconEno2(U, 0, L, R, dd, orc::ivec3(), orc::ivec3());
+    else if (recon == orc::RECON_WENOF2) {
+        if (nvar < 3) return 1;
+        orc::reconWenoF2(U, 0, L, R, orc::ivec3(), orc::ivec3(), gamma);
+    } else orc::reconEno2(U, 0, L, R, dd, orc::ivec3(), orc::ivec3());
     std::memcpy(left, L.buf.data(), L.buf.size() * sizeof(double));
     std::memcpy(right, R.buf.data(), R.buf.size() * sizeof(double));
     return 0;

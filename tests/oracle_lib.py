@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 EQ_BURGERS, EQ_EULER1, EQ_EULER2, EQ_EULER3 = 0, 1, 2, 3
 FLUX_HLL, FLUX_HLL3, FLUX_GODUNOV, FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE = 0, 1, 2, 3, 4, 5
-RECON_NONE, RECON_WENO2, RECON_ENO2, RECON_MC, RECON_MM = 0, 1, 2, 3, 4
+RECON_NONE, RECON_WENO2, RECON_ENO2, RECON_MC, RECON_MM, RECON_WENOF2 = 0, 1, 2, 3, 4, 5
 INT_FE, INT_RK2, INT_RK3 = 0, 1, 2
 BC_NEUMANN, BC_PERIODIC = 0, 1
 
@@ -61,7 +61,7 @@ def lib():
         L.oracle_face_flux.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, dp, dp, dp]
         L.oracle_face_flux.restype = ctypes.c_double
         L.oracle_point_flux.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp]
-        L.oracle_reconstruct_1d.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp]
+        L.oracle_reconstruct_1d.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp, ctypes.c_double]
         L.oracle_compute_timestep.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_int, dp, lp]
         L.oracle_compute_timestep.restype = ctypes.c_double
@@ -167,12 +167,12 @@ def point_flux(nsd, direction, gamma, q):
     return F
 
 
-def reconstruct_1d(recon, q):
+def reconstruct_1d(recon, q, gamma=1.4):
     q = np.ascontiguousarray(q, dtype=np.float64)
     nv, n = q.shape
     left = np.full_like(q, np.nan)
     right = np.full_like(q, np.nan)
-    assert lib().oracle_reconstruct_1d(recon, nv, n, _dp(q), _dp(left), _dp(right)) == 0
+    assert lib().oracle_reconstruct_1d(recon, nv, n, _dp(q), _dp(left), _dp(right), gamma) == 0
     return left, right
 
 

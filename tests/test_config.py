@@ -63,6 +63,19 @@ def test_rejected_options(change, message, tmp_path):
         config.read_config(path)
 
 
+def test_wenof2_is_euler_only(tmp_path):
+    """ReconstructionFactory.cpp:102-117: wenof2 exists for euler1/2/3 only."""
+    src = open(os.path.join(HERE, "kh_uq.xml")).read().replace(
+        "<reconstruction>weno2</reconstruction>", "<reconstruction>wenof2</reconstruction>")
+    assert config.read_config(_copy_config(tmp_path, src)).reconstruction == "wenof2"
+    src = open(os.path.join(HERE, "sod_snippet.xml")).read()
+    src = src.replace("<equation>euler1</equation>", "<equation>burgers</equation>").replace(
+        "<flux>hll</flux>", "<flux>godunov</flux>").replace(
+        "<reconstruction>none</reconstruction>", "<reconstruction>wenof2</reconstruction>")
+    with pytest.raises(config.ConfigError, match="WENO2F for Burgers"):
+        config.read_config(_copy_config(tmp_path, src))
+
+
 def test_onepoint_and_bv_need_integer_p(tmp_path):
     """OnePointMoment and BoundedVariation read p with getInteger."""
     src = open(os.path.join(HERE, "kh_uq.xml")).read()

@@ -45,6 +45,10 @@ def test_invalid_arguments_fail_without_gpu_fault(hip):
     with pytest.raises(hip.HipError, match="same dimension"):
         hip.Ensemble(scheme, [hip.make_sample((8, 8, 1), (0, 0, 0), (1, 1, 0), 1.0),
                               hip.make_sample((8, 1, 1), (0, 0, 0), (1, 1, 0), 1.0)])
+    burgers_wenof2 = hip.make_scheme(O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_WENOF2, O.INT_RK2,
+                                     O.BC_PERIODIC)
+    with pytest.raises(hip.HipError, match="wenof2 needs the Euler equations"):
+        hip.Ensemble(burgers_wenof2, [hip.make_sample((8, 1, 1), (0, 0, 0), (1, 0, 0), 1.0)])
     with hip.Ensemble(scheme, [hip.make_sample((8, 8, 1), (0, 0, 0), (1, 1, 0), 1.0)]) as e:
         with pytest.raises(hip.HipError, match="bad sample"):
             e.info(3)

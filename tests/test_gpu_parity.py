@@ -402,3 +402,48 @@ def test_euler3_mc_grouped_kernel_bit_exact(hip):
         (1, 1, 1), 0.475, 1.0, u0, max_steps=6)
     assert info["steps"] == steps_ref == 6
     _assert_bit_exact(ref, got)
+
+
+@pytest.mark.parametrize("eq", [O.EQ_EULER1, O.EQ_EULER2, O.EQ_EULER3])
+def test_wenof2_bit_exact(hip, eq):
+    """wenof2 (primitive-variable WENO2 with the density/pressure clamp and the
+    velocity scaling) on Sod (1D, HLL3, SSP-RK3), the Kelvin-Helmholtz state
+    (2D, periodic, HLL, SSP-RK3) and random 3D data (Neumann, HLL3, SSP-RK2),
+    against the oracle."""
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    if eq == O.EQ_EULER1:
+        _print_case("wenof2 euler1 sod 200")
+        args = (O.FLUX_HLL3, O.RECON_WENOF2, O.INT_RK3, O.BC_NEUMANN, (200, 1, 1), (-1, 0, 0),
+                (1, 0, 0), 0.475, 0.1, ic.sod(200, ghosts=2))
+        steps = None
+    elif eq == O.EQ_EULER2:
+        _print_case("wenof2 euler2 kh 48^2")
+        X = sampling.stl_mersenne_uniform(1, 40)[0]
+        args = (O.FLUX_HLL, O.RECON_WENOF2, O.INT_RK3, O.BC_PERIODIC, (48, 48, 1), (0, 0, 0),
+                (1, 1, 0), 0.475, 2.0, ic.kelvin_helmholtz_2d(48, X))
+        steps = 20
+    else:
+        _print_case("wenof2 euler3 14x10x9 neumann")
+        rng = np.random.default_rng(23)
+        n = (14, 10, 9)
+        s = O.make_setup(O.EQ_EULER3, O.FLUX_HLL3, O.RECON_WENOF2, O.INT_RK2, O.BC_NEUMANN, n,
+                         (0, 0, 0), (1, 1, 1), cfl=0.475, end_time=1.0, max_steps=6)
+        shape = O.volume_shape(s)
+        u0 = np.zeros(shape)
+        u0[0] = 1.0 + 0.5 * rng.random(shape[1:])
+        for k in (1, 2, 3):
+            u0[k] = 0.3 * rng.standard_normal(shape[1:])
+        u0[4] = 3.0 + rng.random(shape[1:])
+        args = (O.FLUX_HLL3, O.RECON_WENOF2, O.INT_RK2, O.BC_NEUMANN, n, (0, 0, 0), (1, 1, 1),
+                0.475, 1.0, u0)
+        steps = 6
+    kw = {} if steps is None else {"max_steps": steps}
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(hip, eq, *args, **kw)
+    assert info["steps"] == steps_ref
+    if steps is not None:
+        assert steps_ref == steps
+    else:
+        assert info["t"] == t_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)

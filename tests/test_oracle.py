@@ -193,6 +193,103 @@ def test_reconstruction_convergence(recon, rate):
     assert rates[-1] >= rate, rates
 
 
+def _euler3_cells(rho, u, p, gamma=GAMMA):
+    """Conserved euler3 cells (5, n) from primitive rows (Euler::computeConserved)."""
+    rho = np.asarray(rho, dtype=float)
+    n = rho.size
+    u = np.asarray(u, dtype=float)
+    u = np.broadcast_to(u if u.ndim == 2 else u.reshape(-1, 1), (3, n))
+    p = np.broadcast_to(np.asarray(p, dtype=float), (n,))
+    q = np.zeros((5, n))
+    q[0] = rho
+    q[1:4] = rho * u
+    q[4] = p / (gamma - 1) + 0.5 * rho * (u * u).sum(axis=0)
+    return q
+
+
+def test_wenof2_convergence_reference_case():
+    """ReconstructionConvergenceTest.cpp:160-240 with its wenof2 bound 1.99
+    (double precision): euler3 cells with rho the cell averages of
+    sin(2 pi x) + 2, p = 2.5, u = 0; L1 error of the density face values."""
+    errs = []
+    for k in range(6, 11):
+        N = 1 << k
+        h = 1.0 / N
+        x0 = (np.arange(N + 4) - 2) * h
+        avg = (-np.cos(2 * np.pi * (x0 + h)) + np.cos(2 * np.pi * x0)) / (2 * np.pi * h) + 2
+        left, right = O.reconstruct_1d(O.RECON_WENOF2, _euler3_cells(avg, 0.0, 2.5))
+        f = lambda x: np.sin(2 * np.pi * x) + 2
+        sl = slice(2, 2 + N)
+        errs.append(np.mean(np.abs(left[0, sl] - f(x0[sl]))) +
+                    np.mean(np.abs(right[0, sl] - f(x0[sl] + h))))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] >= 1.99, rates
+
+
+def test_wenof2_at_rest_is_weno2_on_density():
+    """With u = 0 and constant p the velocity factor is 1/2 and the density
+    faces are WENO2's (the same weights; M -/+ dP/2 equals WENO2's
+    combination algebraically: a few ulp apart); the pressure faces stay p,
+    so E = p/(gamma-1) exactly and the momenta are zero."""
+    rng = np.random.default_rng(4)
+    rho = 1.0 + rng.random(40)
+    q = _euler3_cells(rho, 0.0, 0.75)
+    l5, r5 = O.reconstruct_1d(O.RECON_WENOF2, q)
+    l2, r2 = O.reconstruct_1d(O.RECON_WENO2, q)
+    sl = slice(2, 38)
+    np.testing.assert_allclose(l5[0, sl], l2[0, sl], rtol=4e-15)
+    np.testing.assert_allclose(r5[0, sl], r2[0, sl], rtol=4e-15)
+    assert np.all(l5[1:4, sl] == 0) and np.all(r5[1:4, sl] == 0)
+    assert np.all(l5[4, sl] == 0.75 / (GAMMA - 1)) and np.all(r5[4, sl] == 0.75 / (GAMMA - 1))
+
+
+def test_wenof2_clamps_density_and_pressure_slopes():
+    """A one-cell dip (10, 1, 10) in rho (or p) gives slopes -3 / +3 (weights
+    2/3, 1/3 from equal smoothness), clamped to -/+1.9 times the middle value:
+    both faces are 1 + 0.5 * 1.9."""
+    q = _euler3_cells([10.0, 1.0, 10.0], 0.0, 1.0)
+    left, right = O.reconstruct_1d(O.RECON_WENOF2, np.hstack([q[:, :1]] * 2 + [q] + [q[:, 2:]] * 2))
+    assert left[0, 3] == 1 + 0.5 * 1.9 and right[0, 3] == 1 + 0.5 * 1.9
+    q = _euler3_cells(1.0 + np.zeros(3), 0.0, [10.0, 1.0, 10.0])
+    left, right = O.reconstruct_1d(O.RECON_WENOF2, np.hstack([q[:, :1]] * 2 + [q] + [q[:, 2:]] * 2))
+    pf = 1 + 0.5 * 1.9
+    assert left[4, 3] == pf / (GAMMA - 1) and right[4, 3] == pf / (GAMMA - 1)
+
+
+def test_wenof2_velocity_scaling_caps_kinetic_energy():
+    """rho = 1, u = (-1, 0, 1) in x, p = 1e-3: the velocity slope 1 would add
+    kinetic energy LL = 1/8 > R = p/(gamma-1); the slope is scaled by
+    0.5*sqrt(R/LL), so each face has kinetic energy exactly R (closed form):
+    E_face = 2R, u_face = -/+ sqrt(2R). With p = 1 (LL < R) the scale is 1/2:
+    u_face = -/+ 0.5."""
+    for p, uf in ((1e-3, math.sqrt(2 * 1e-3 / (GAMMA - 1))), (1.0, 0.5)):
+        q = _euler3_cells(np.ones(3), np.array([[-1.0, 0.0, 1.0], [0, 0, 0], [0, 0, 0]]), p)
+        left, right = O.reconstruct_1d(O.RECON_WENOF2,
+                                       np.hstack([q[:, :1]] * 2 + [q] + [q[:, 2:]] * 2))
+        assert left[1, 3] == pytest.approx(-uf, rel=1e-14)
+        assert right[1, 3] == pytest.approx(uf, rel=1e-14)
+        R = p / (GAMMA - 1)
+        assert right[4, 3] == pytest.approx(R + 0.5 * uf * uf, rel=1e-14)
+        if p < 0.1:
+            assert right[4, 3] == pytest.approx(2 * R, rel=1e-14)
+
+
+def test_wenof2_euler1_equals_euler3_without_transverse_velocity():
+    """The primitive slopes are per component: euler1 and euler3 cells with
+    zero transverse velocity reconstruct the same rho, mx and E faces."""
+    rng = np.random.default_rng(9)
+    rho = 1 + rng.random(30)
+    ux = rng.standard_normal(30)
+    p = 1 + rng.random(30)
+    q3 = _euler3_cells(rho, np.vstack([ux, 0 * ux, 0 * ux]), p)
+    q1 = q3[[0, 1, 4]]
+    l3, r3 = O.reconstruct_1d(O.RECON_WENOF2, q3)
+    l1, r1 = O.reconstruct_1d(O.RECON_WENOF2, q1)
+    sl = slice(2, 28)
+    for a, b in ((l1, l3), (r1, r3)):
+        assert np.array_equal(a[:, sl], b[[0, 1, 4]][:, sl])
+
+
 @pytest.mark.parametrize("recon", [O.RECON_MC, O.RECON_MM])
 def test_limiters_closed_forms(recon):
     """MC.hpp / MM.hpp with minmod.hpp: linear data give the exact face values
