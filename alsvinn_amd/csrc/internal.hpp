@@ -63,9 +63,36 @@ constexpr bool kAllBudgets =
     (std::is_same_v<Flux, FluxHLL> || std::is_same_v<Flux, FluxHLL3> ||
      std::is_same_v<Flux, FluxGodunov>);
 
+// Reconstructions wider than three cells run on the per-cell wide kernel
+// (one register budget).
+template <class T, class Eq, class Recon, class Flux, int DIM>
+void launch_stage_wide(const StageArgs<T>& a, int blocks, hipStream_t s) {
+    if (blocks <= 0) return;
+    switch (a.mode) {
+        case MODE_RHS:
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RHS>), dim3(blocks),
+                               dim3(256), 0, s, a);
+            break;
+        case MODE_RK2_1:
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK2_1>),
+                               dim3(blocks), dim3(256), 0, s, a);
+            break;
+        case MODE_RK3_1:
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK3_1>),
+                               dim3(blocks), dim3(256), 0, s, a);
+            break;
+        case MODE_RK3_2:
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK3_2>),
+                               dim3(blocks), dim3(256), 0, s, a);
+            break;
+    }
+}
+
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
-    if constexpr (kAllBudgets<Recon, Flux>) {
+    if constexpr (recon_radius<Recon>::value > 1) {
+        k.stage[0] = k.stage[1] = k.stage[2] = &launch_stage_wide<T, Eq, Recon, Flux, DIM>;
+    } else if constexpr (kAllBudgets<Recon, Flux>) {
         k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 1>;
         k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
         k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
@@ -91,6 +118,8 @@ int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
         case ALSVINN_HIP_RECON_ENO2: set_stage<T, Eq, ReconENO2, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_MC: set_stage<T, Eq, ReconMC, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_MM: set_stage<T, Eq, ReconMM, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_ENO3: set_stage<T, Eq, ReconENO<3>, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_ENO4: set_stage<T, Eq, ReconENO<4>, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_WENOF2:
             if constexpr (Eq::is_euler) {
                 set_stage<T, Eq, ReconWENOF2, Flux, DIM>(k);

@@ -10,8 +10,8 @@
 //
 // Work decomposition. A wavefront owns a strip of columns (x) of one sample
 // and marches upward in y (and z for 3D), keeping a sliding register window
-// of the rows it needs. x-neighbours come from the adjacent lanes (ds_bpermute
-// shuffles); the G = ghost-count lanes at each end of a strip are halo lanes
+// of the rows it needs. x-neighbours come from the adjacent lanes (DPP lane
+// moves); the G = ghost-count lanes at each end of a strip are halo lanes
 // that only feed their neighbours. y-neighbours come from the window, so each
 // reconstruction and each face flux is evaluated once per cell. Nothing is
 // staged through LDS and no barrier is needed: waves are independent.
@@ -654,6 +654,114 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             }
         }
         __syncthreads();  // the next task reuses the LDS rows
+    }
+}
+
+// Stage kernel for reconstructions wider than three cells (ENO3, ENO4): the
+// same tasks, arithmetic, outputs and wave-speed reduction as the marching
+// kernels, but every interior lane evaluates its own cell from memory: per
+// direction it loads the 2*radius+3 cells of its line, reconstructs the cell
+// and both neighbours and evaluates the fluxes of both faces (each face flux
+// is thus evaluated by the two cells beside it). The divergence is summed in
+// the marching kernels' order: ((0 + x part) + y part) + z part.
+template <class T, class Eq, class Recon, class Flux, int DIM, int MODE>
+__global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
+    constexpr bool LAZY = MODE == MODE_RK3_1;
+    constexpr int NV = Eq::nvar;
+    constexpr int G = Recon::ghosts;
+    constexpr int RAD = recon_radius<Recon>::value;
+    constexpr int NS = 2 * RAD + 3;  // cells p-1-RAD .. p+1+RAD of one line
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    Eq eq;
+    eq.gamma = T(a.gamma);
+
+    for (int w = wave; w < a.ntasks; w += nwaves) {
+        const WaveTask& tk = a.tasks[w];
+        int seg = -1;
+        for (int k = 0; k < tk.nseg; ++k)
+            if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
+        const bool in_strip = seg >= 0;
+        const int sk = in_strip ? seg : 0;
+        const int sample = tk.sample[sk];
+        const int li = lane - tk.lane0[sk];
+        const int ncols = tk.ncols[sk];
+        const SampleDesc D = a.desc[sample];
+        SampleState* Sp = a.state + sample;
+        const bool active = in_strip && runnable(*Sp);
+        if (!__any(active)) continue;
+        const bool need_dt = LAZY || MODE != MODE_RHS;
+        T c = T(0);
+        if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
+        const int col = tk.x0[sk] - G + li;
+        const bool interior = active && li >= G && li < G + ncols;
+        const long long vs = D.vstride;
+        const T* in = a.in + D.base;
+        const T* in2 = a.in2 + D.base;
+        T* out = a.out + D.base;
+        const long long sxd[3] = {1, (long long)D.tx, (long long)D.tx * D.ty};
+        T ws[3] = {T(0), T(0), T(0)};
+
+        // divergence part of direction d at flat index i0: (-F(p+1/2)) + F(p-1/2)
+        auto direction = [&](auto dc, long long i0, T* part) {
+            constexpr int d = decltype(dc)::value;
+            T q[NS][NV];
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                load_cell<T, NV>(in, in2, LAZY, c, i0 + (k - 1 - RAD) * sxd[d], vs, q[k]);
+            T Lm[NV], Rm[NV], L0[NV], R0[NV], Lp[NV], Rp[NV];
+            recon_stencil<Recon, T, NV>(q + 0, Lm, Rm, eq.gamma);
+            recon_stencil<Recon, T, NV>(q + 1, L0, R0, eq.gamma);
+            recon_stencil<Recon, T, NV>(q + 2, Lp, Rp, eq.gamma);
+            T Fm[NV], Fp[NV];
+            const T sm = face_flux<T, Eq, Flux, d>(eq, Rm, L0, Fm);
+            const T sp = face_flux<T, Eq, Flux, d>(eq, R0, Lp, Fp);
+            ws[d] = std_max(std_max(ws[d], sm), sp);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part[v] = (T(-1.0) * Fp[v]) + Fm[v];
+        };
+
+        const int zb = DIM >= 3 ? tk.z0 : 0, ze = DIM >= 3 ? tk.z1 : 1;
+        for (int z = zb; z < ze; ++z)
+            for (int r = tk.y0; r < tk.y1; ++r) {
+                if (!interior) continue;
+                const long long i0 =
+                    ((long long)(z + D.gz) * D.ty + (r + D.gy)) * D.tx + (col + D.gx);
+                T out_v[NV], part[NV];
+                direction(std::integral_constant<int, 0>(), i0, part);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out_v[v] = T(0) + part[v];
+                if constexpr (DIM >= 2) {
+                    direction(std::integral_constant<int, 1>(), i0, part);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + part[v];
+                }
+                if constexpr (DIM >= 3) {
+                    direction(std::integral_constant<int, 2>(), i0, part);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + part[v];
+                }
+                T uin[NV], u0[NV];
+                load_cell<T, NV>(in, in2, LAZY, c, i0, vs, uin);
+                if (MODE != MODE_RHS) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                }
+                T res[NV];
+                combine<T, NV>(MODE, c, out_v, uin, u0, res);
+                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
+                store_cell<T, NV, DIM>(out, vs, D, col, r, z, res);
+            }
+
+        if (MODE == MODE_RHS && !LAZY) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const T m = segment_max(ws[d], seg);
+                if (in_strip && li == 0 && runnable(*Sp))
+                    atomicMax(a.ws + 3 * sample + d, (unsigned long long)__double_as_longlong(double(m)));
+            }
+        }
     }
 }
 

@@ -1,5 +1,6 @@
 // Device-side physics for the hip platform: Euler<nsd> / Burgers equations,
-// HLL / HLL3 / Godunov face fluxes and none / WENO2 / ENO2 reconstruction.
+// their face fluxes (HLL, HLL3, Godunov, central, Rusanov, Roe) and
+// reconstructions (none, WENO2, WENOF2, ENO2-4, MC, MM).
 //
 // Every expression keeps the association order of the reference's CPU path so
 // that FP64 results are bit-identical (the library is built with
@@ -9,9 +10,11 @@
 //   HLL3::computeFlux / computeHLLSpeeds     alsfvm/include/alsfvm/numflux/euler/HLL3.hpp:52-148
 //   burgers::Godunov::computeFlux            alsfvm/include/alsfvm/numflux/burgers/Godunov.hpp:40-60
 //   WENO2::reconstruct                       alsfvm/include/alsfvm/reconstruction/WENO2.hpp:37-71
-//   ENOCPU<2> shift + coefficients           alsfvm/src/reconstruction/ENOCPU.cpp:143-176,
-//                                            alsfvm/src/reconstruction/ENOCoefficients.cpp:27-32
+//   ENOCPU<2..4> shift + coefficients        alsfvm/src/reconstruction/ENOCPU.cpp:143-176,
+//                                            alsfvm/src/reconstruction/ENOCoefficients.cpp:27-50
 #pragma once
+
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 
@@ -464,6 +467,90 @@ struct ReconENO2 {
         }
     }
 };
+
+// ENO of order K = 3, 4 (ENOCPU.cpp:41-185, differences 205-264,
+// coefficients ENOCoefficients.cpp:34-50) on a stencil of 2K-1 cells
+// q[0 .. 2K-2] around cell c = K-1. Differences of variable 0 up to level
+// K-2; at every level the shift grows by one when the difference at position
+// c-1 is smaller in magnitude than the one at c (the reference compares the
+// same two positions at every level). Left/right face values: coefficient
+// rows shift and shift+1 over cells c-shift .. c-shift+K-1, summed from 0 in
+// stencil order.
+template <int K>
+struct EnoRows;
+template <>
+struct EnoRows<3> {
+    static constexpr double c[4][3] = {{11.0 / 6.0, -7.0 / 6.0, 1.0 / 3.0},
+                                       {1.0 / 3.0, 5.0 / 6.0, -1.0 / 6.0},
+                                       {-1.0 / 6.0, 5.0 / 6.0, 1.0 / 3.0},
+                                       {1.0 / 3.0, -7.0 / 6.0, 11.0 / 6.0}};
+};
+template <>
+struct EnoRows<4> {
+    static constexpr double c[5][4] = {{25.0 / 12.0, -23.0 / 12.0, 13.0 / 12.0, -1.0 / 4.0},
+                                       {1.0 / 4.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0},
+                                       {-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},
+                                       {1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 1.0 / 4.0},
+                                       {-1.0 / 4.0, 13.0 / 12.0, -23.0 / 12.0, 25.0 / 12.0}};
+};
+
+template <int K>
+struct ReconENO {
+    static constexpr int ghosts = K;
+    static constexpr int radius = K - 1;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply_wide(const T (*q)[NV], T* L, T* R, T /*gamma*/) {
+        constexpr int c = K - 1;
+        T dd[2 * K - 2];
+#pragma unroll
+        for (int j = 0; j < 2 * K - 2; ++j) dd[j] = q[j + 1][0] - q[j][0];
+        int shift = 0;
+#pragma unroll
+        for (int l = 0; l < K - 1; ++l) {
+            if (dabs(dd[c - 1]) < dabs(dd[c])) ++shift;
+#pragma unroll
+            for (int j = 0; j < 2 * K - 3 - l; ++j) dd[j] = dd[j + 1] - dd[j];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T l = T(0);
+            T r = T(0);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                T val = q[c + j][v];
+                T cl = T(EnoRows<K>::c[0][j]);
+                T cr = T(EnoRows<K>::c[1][j]);
+#pragma unroll
+                for (int s = 1; s < K; ++s)
+                    if (shift == s) {
+                        val = q[c - s + j][v];
+                        cl = T(EnoRows<K>::c[s][j]);
+                        cr = T(EnoRows<K>::c[s + 1][j]);
+                    }
+                l += cl * val;
+                r += cr * val;
+            }
+            L[v] = l;
+            R[v] = r;
+        }
+    }
+};
+
+// Stencil radius of a reconstruction: cells each side of the reconstructed
+// cell (1 unless the reconstruction declares more).
+template <class R, class = void>
+struct recon_radius : std::integral_constant<int, 1> {};
+template <class R>
+struct recon_radius<R, std::void_t<decltype(R::radius)>> : std::integral_constant<int, R::radius> {};
+
+// Reconstruction of cell q[radius] of a stencil of 2*radius+1 cells.
+template <class Recon, class T, int NV>
+__device__ __forceinline__ void recon_stencil(const T (*q)[NV], T* L, T* R, T gamma) {
+    if constexpr (recon_radius<Recon>::value == 1)
+        Recon::template apply<T, NV>(q[0], q[1], q[2], L, R, gamma);
+    else
+        Recon::template apply_wide<T, NV>(q, L, R, gamma);
+}
 
 // sign.hpp and minmod.hpp of the reference.
 template <class T>

@@ -38,16 +38,19 @@ enum {
     ALSVINN_HIP_FLUX_RUSANOV = 4,
     ALSVINN_HIP_FLUX_ROE = 5
 };
-/* Reconstructions: none (NoReconstruction.cpp), weno2 (WENO2.hpp), eno2
- * (ENOCPU.cpp, order 2), mc (MC.hpp), mm (MM.hpp), wenof2 (WENOF2.hpp,
- * Euler only; ReconstructionFactory.cpp:102-115). */
+/* Reconstructions: none (NoReconstruction.cpp), weno2 (WENO2.hpp), eno2,
+ * eno3, eno4 (ENOCPU.cpp, orders 2-4; ghost cells = order), mc (MC.hpp), mm
+ * (MM.hpp), wenof2 (WENOF2.hpp, Euler only; ReconstructionFactory.cpp:
+ * 59-117). */
 enum {
     ALSVINN_HIP_RECON_NONE = 0,
     ALSVINN_HIP_RECON_WENO2 = 1,
     ALSVINN_HIP_RECON_ENO2 = 2,
     ALSVINN_HIP_RECON_MC = 3,
     ALSVINN_HIP_RECON_MM = 4,
-    ALSVINN_HIP_RECON_WENOF2 = 5
+    ALSVINN_HIP_RECON_WENOF2 = 5,
+    ALSVINN_HIP_RECON_ENO3 = 6,
+    ALSVINN_HIP_RECON_ENO4 = 7
 };
 enum {
     ALSVINN_HIP_INTEGRATOR_FORWARD_EULER = 0,

@@ -16,7 +16,7 @@
 //   net flux / makeZero    alsfvm/src/numflux/NumericalFluxCPU.cpp:32-146,170-231
 //   WENO2                  alsfvm/include/alsfvm/reconstruction/WENO2.hpp:31-95 (eps 1e-8,
 //                          WENOCoefficients.hpp:21); ranges ReconstructionCPU.cpp:38-97
-//   ENO2                   alsfvm/src/reconstruction/ENOCPU.cpp:41-267, ENOCoefficients.cpp:27-32
+//   ENO2, ENO3, ENO4       alsfvm/src/reconstruction/ENOCPU.cpp:41-267, ENOCoefficients.cpp:27-50
 //   WENOF2                 alsfvm/include/alsfvm/reconstruction/WENOF2.hpp:31-115,
 //                          euler/PrimitiveVariables.hpp, Euler.hpp:236-245,290-297
 //   no reconstruction      alsfvm/src/reconstruction/NoReconstruction.cpp:42-73
@@ -334,9 +334,23 @@ struct Roe {
 
 // ---------------------------------------------------------------- reconstruction
 
-enum { RECON_NONE = 0, RECON_WENO2 = 1, RECON_ENO2 = 2, RECON_MC = 3, RECON_MM = 4, RECON_WENOF2 = 5 };
+enum {
+    RECON_NONE = 0,
+    RECON_WENO2 = 1,
+    RECON_ENO2 = 2,
+    RECON_MC = 3,
+    RECON_MM = 4,
+    RECON_WENOF2 = 5,
+    RECON_ENO3 = 6,
+    RECON_ENO4 = 7
+};
 
-static int reconGhosts(int recon) { return recon == RECON_NONE ? 1 : 2; }
+static int reconGhosts(int recon) {
+    if (recon == RECON_NONE) return 1;
+    if (recon == RECON_ENO3) return 3;
+    if (recon == RECON_ENO4) return 4;
+    return 2;
+}
 
 // reconstruction/sign.hpp and minmod.hpp
 static real refSign(real a) {
@@ -546,33 +560,55 @@ static void reconWenoF2(const Vol& in, int d, Vol& Lo, Vol& Ro, ivec3 start, ive
             }
 }
 
-// ENOCPU.cpp:41-185 (order 2) with the divided differences of 205-264.
-static void reconEno2(const Vol& in, int d, Vol& Lo, Vol& Ro, std::vector<real>& dd, ivec3 start,
-                      ivec3 end) {
-    const int order = 2;
+// ENOCoefficients.cpp:27-50: rows r = 0..order of the reconstruction
+// coefficients over cells i-r .. i-r+order-1 (row r left face, r+1 right).
+static const real kEno2[3][2] = {{real(3.0 / 2.0), real(-1.0 / 2.0)},
+                                 {real(1.0 / 2.0), real(1.0 / 2.0)},
+                                 {real(-1.0 / 2.0), real(3.0 / 2.0)}};
+static const real kEno3[4][3] = {{real(11.0 / 6.0), real(-7.0 / 6.0), real(1.0 / 3.0)},
+                                 {real(1.0 / 3.0), real(5.0 / 6.0), real(-1.0 / 6.0)},
+                                 {real(-1.0 / 6.0), real(5.0 / 6.0), real(1.0 / 3.0)},
+                                 {real(1.0 / 3.0), real(-7.0 / 6.0), real(11.0 / 6.0)}};
+static const real kEno4[5][4] = {
+    {real(25.0 / 12.0), real(-23.0 / 12.0), real(13.0 / 12.0), real(-1.0 / 4.0)},
+    {real(1.0 / 4.0), real(13.0 / 12.0), real(-5.0 / 12.0), real(1.0 / 12.0)},
+    {real(-1.0 / 12.0), real(7.0 / 12.0), real(7.0 / 12.0), real(-1.0 / 12.0)},
+    {real(1.0 / 12.0), real(-5.0 / 12.0), real(13.0 / 12.0), real(1.0 / 4.0)},
+    {real(-1.0 / 4.0), real(13.0 / 12.0), real(-23.0 / 12.0), real(25.0 / 12.0)}};
+static real enoCoefficient(int order, int row, int j) {
+    if (order == 2) return kEno2[row][j];
+    if (order == 3) return kEno3[row][j];
+    return kEno4[row][j];
+}
+
+// ENOCPU.cpp:41-185 (orders 2-4) with the divided differences of 205-264:
+// (order-1) difference arrays of variable 0, each level from the one below
+// over the reference's index ranges (kept between calls, as the reference
+// keeps its arrays); at every level the shift grows when the difference at
+// i-1 is smaller in magnitude than the one at i.
+static void reconEno(int order, const Vol& in, int d, Vol& Lo, Vol& Ro, std::vector<real>& dd,
+                     ivec3 start, ivec3 end) {
     const int nx = in.tx, ny = in.ty, nz = in.tz;
     const int dxv = d == 0, dyv = d == 1, dzv = d == 2;
-    dd.resize(in.cells());
-    {
-        const int level = 1;
+    const size_t cells = in.cells();
+    if (dd.size() != (order - 1) * cells) dd.assign((order - 1) * cells, 0.0);
+    for (int level = 1; level < order; ++level) {
+        const real* pin = level == 1 ? in.v(0) : dd.data() + (level - 2) * cells;
+        real* pout = dd.data() + (level - 1) * cells;
         const int sx = (dxv == 0) * order + dxv * level + start.x;
         const int sy = (ny > 1) * (dyv == 0) * order + dyv * level + start.y;
         const int sz = (nz > 1) * (dzv == 0) * order + dzv * level + start.z;
         const int ex = nx - ((dxv == 0) * (order - 1) + dxv * (level - 1)) + end.x;
         const int ey = ny - ((ny > 1) * (dyv == 0) * (order - 1) + dyv * (level - 1)) + end.y;
         const int ez = nz - ((nz > 1) * (dzv == 0) * (order - 1) + dzv * (level - 1)) + end.z;
-        const real* pin = in.v(0);
         for (int z = sz; z < ez; z++)
             for (int y = sy; y < ey; y++)
                 for (int x = sx; x < ex; x++) {
                     const size_t iR = in.idx(x, y, z);
                     const size_t iL = in.idx(x - dxv, y - dyv, z - dzv);
-                    dd[iL] = pin[iR] - pin[iL];
+                    pout[iL] = pin[iR] - pin[iL];
                 }
     }
-    static const real C[3][2] = {{real(3.0 / 2.0), real(-1.0 / 2.0)},
-                                 {real(1.0 / 2.0), real(1.0 / 2.0)},
-                                 {real(-1.0 / 2.0), real(3.0 / 2.0)}};
     const int sx = (order - dxv) + start.x;
     const int sy = (ny > 1) * (order - dyv) + start.y;
     const int sz = (nz > 1) * (order - dzv) + start.z;
@@ -590,16 +626,17 @@ static void reconEno2(const Vol& in, int d, Vol& Lo, Vol& Ro, std::vector<real>&
                     const size_t iR = in.idx(x, y, z);
                     const size_t iL = in.idx(x - dxv, y - dyv, z - dzv);
                     int shift = 0;
-                    if (std::fabs(dd[iL]) < std::fabs(dd[iR])) shift++;
-                    const real* cR = C[shift + 1];
-                    const real* cL = C[shift];
+                    for (int level = 0; level < order - 1; ++level) {
+                        const real* ddl = dd.data() + level * cells;
+                        if (std::fabs(ddl[iL]) < std::fabs(ddl[iR])) shift++;
+                    }
                     real lv = 0.0, rv = 0.0;
                     for (int j = 0; j < order; j++) {
                         const size_t index = in.idx(x - (shift - j) * dxv, y - (shift - j) * dyv,
                                                     z - (shift - j) * dzv);
                         const real value = pin[index];
-                        lv += cL[j] * value;
-                        rv += cR[j] * value;
+                        lv += enoCoefficient(order, shift, j) * value;
+                        rv += enoCoefficient(order, shift + 1, j) * value;
                     }
                     pl[iR] = lv;
                     pr[iR] = rv;
@@ -712,7 +749,8 @@ struct System : SystemBase {
         else if (recon == RECON_WENOF2)
             reconWenoF2(U, d, left, right, s, e, eq.gamma);
         else
-            reconEno2(U, d, left, right, dd, s, e);
+            reconEno(recon == RECON_ENO2 ? 2 : (recon == RECON_ENO3 ? 3 : 4), U, d, left, right,
+                     dd, s, e);
     }
 
     // NumericalFluxCPU::computeFlux (NumericalFluxCPU.cpp:170-221)
@@ -949,7 +987,7 @@ struct Sim {
 
 static bool validSetup(const Setup& s) {
     if (s.equation < 0 || s.equation > 3) return false;
-    if (s.recon < 0 || s.recon > 5) return false;
+    if (s.recon < 0 || s.recon > 7) return false;
     if (s.recon == RECON_WENOF2 && s.equation == 0) return false;  // Euler only
     if (s.integrator < 0 || s.integrator > 2) return false;
     if (s.num_adjusters < 0 || s.num_adjusters > 2) return false;
@@ -1116,7 +1154,9 @@ int oracle_reconstruct_1d(int recon, int nvar, int n, const double* q, double* l
     else if (recon == orc::RECON_WENOF2) {
         if (nvar < 3) return 1;
         orc::reconWenoF2(U, 0, L, R, orc::ivec3(), orc::ivec3(), gamma);
-    } else orc::reconEno2(U, 0, L, R, dd, orc::ivec3(), orc::ivec3());
+    } else
+        orc::reconEno(recon == orc::RECON_ENO2 ? 2 : (recon == orc::RECON_ENO3 ? 3 : 4), U, 0, L, R,
+                      dd, orc::ivec3(), orc::ivec3());
     std::memcpy(left, L.buf.data(), L.buf.size() * sizeof(double));
     std::memcpy(right, R.buf.data(), R.buf.size() * sizeof(double));
     return 0;
