@@ -48,7 +48,7 @@ FLUXES = {"hll": _lib.FLUX_HLL, "hll3": _lib.FLUX_HLL3, "godunov": _lib.FLUX_GOD
           "central": _lib.FLUX_CENTRAL, "rusanov": _lib.FLUX_RUSANOV, "roe": _lib.FLUX_ROE}
 RECONSTRUCTIONS = {"none": _lib.RECON_NONE, "weno2": _lib.RECON_WENO2, "eno2": _lib.RECON_ENO2,
                    "eno3": _lib.RECON_ENO3, "eno4": _lib.RECON_ENO4, "mc": _lib.RECON_MC,
-                   "mm": _lib.RECON_MM, "wenof2": _lib.RECON_WENOF2}
+                   "mm": _lib.RECON_MM, "wenof2": _lib.RECON_WENOF2, "weno3": _lib.RECON_WENO3}
 INTEGRATORS = {"forwardeuler": _lib.INT_FE, "rungekutta2": _lib.INT_RK2, "rungekutta3": _lib.INT_RK3}
 BOUNDARIES = {"periodic": _lib.BC_PERIODIC, "neumann": _lib.BC_NEUMANN}
 PRIMITIVES = {"burgers": ["u"], "euler1": ["rho", "ux", "p"], "euler2": ["rho", "ux", "uy", "p"],

@@ -87,7 +87,7 @@ int validate(const alsvinn_hip_scheme* s) {
     if (!s) return fail(ALSVINN_HIP_E_INVALID, "scheme is NULL");
     if (s->equation < 0 || s->equation > 3)
         return fail(ALSVINN_HIP_E_INVALID, "unknown equation " + std::to_string(s->equation));
-    if (s->reconstruction < 0 || s->reconstruction > 7)
+    if (s->reconstruction < 0 || s->reconstruction > 8)
         return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction " + std::to_string(s->reconstruction));
     if (s->integrator < 0 || s->integrator > 2)
         return fail(ALSVINN_HIP_E_INVALID, "unknown integrator " + std::to_string(s->integrator));
@@ -112,6 +112,7 @@ int ghost_cells(const alsvinn_hip_scheme& s) {
     if (s.reconstruction == ALSVINN_HIP_RECON_NONE) return 1;
     if (s.reconstruction == ALSVINN_HIP_RECON_ENO3) return 3;
     if (s.reconstruction == ALSVINN_HIP_RECON_ENO4) return 4;
+    if (s.reconstruction == ALSVINN_HIP_RECON_WENO3) return 3;
     return 2;
 }
 

@@ -120,6 +120,7 @@ int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
         case ALSVINN_HIP_RECON_MM: set_stage<T, Eq, ReconMM, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_ENO3: set_stage<T, Eq, ReconENO<3>, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_ENO4: set_stage<T, Eq, ReconENO<4>, Flux, DIM>(k); return 0;
+        case ALSVINN_HIP_RECON_WENO3: set_stage<T, Eq, ReconWENO3, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_WENOF2:
             if constexpr (Eq::is_euler) {
                 set_stage<T, Eq, ReconWENOF2, Flux, DIM>(k);

@@ -536,6 +536,72 @@ struct ReconENO {
     }
 };
 
+// WENO of order 3 (WENOCPU.cpp:32-186, WENOCoefficients.hpp:51-63,
+// WENOCoefficients.cpp:26-38) on the five cells q[0..4] around cell 2:
+// smoothness indicators of variable 0 for the three candidate stencils,
+// alpha = c / (beta + 1e-8)^2 (linear weights 3/10, 3/5, 1/10, mirrored for
+// the left face), summed from the third stencil down; each candidate is the
+// ENO3 row combination, weighted by alpha / sum in stencil order.
+struct ReconWENO3 {
+    static constexpr int ghosts = 3;
+    static constexpr int radius = 2;
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply_wide(const T (*q)[NV], T* L, T* R, T /*gamma*/) {
+        const T V0 = q[0][0], V1 = q[1][0], V2 = q[2][0], V3 = q[3][0], V4 = q[4][0];
+        const T c1312 = T(13.0 / 12.0), c14 = T(1 / 4.0);
+        T beta[3];
+        {
+            const T a = V2 - T(2) * V3 + V4, b = T(3) * V2 - T(4) * V3 + V4;
+            beta[0] = c1312 * (a * a) + c14 * (b * b);
+        }
+        {
+            const T a = V1 - T(2) * V2 + V3, b = V1 - V3;
+            beta[1] = c1312 * (a * a) + c14 * (b * b);
+        }
+        {
+            const T a = V0 - T(2) * V1 + V2, b = V0 - T(4) * V1 + T(3) * V2;
+            beta[2] = c1312 * (a * a) + c14 * (b * b);
+        }
+        const T lin[3] = {T(3.0 / 10.0), T(3.0 / 5.0), T(1.0 / 10.0)};
+        const T eps = T(1e-8);
+        T aR[3], aL[3];
+        T sR = T(0), sL = T(0);
+#pragma unroll
+        for (int i = 2; i >= 0; --i) {
+            const T e = beta[i] + eps;
+            const T e2 = e * e;
+            aR[i] = dv(lin[i], e2);
+            sR += aR[i];
+            aL[i] = dv(lin[2 - i], e2);
+            sL += aL[i];
+        }
+        T wL[3], wR[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            wL[i] = dv(aL[i], sL);
+            wR[i] = dv(aR[i], sR);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T lw = T(0), rw = T(0);
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh) {
+                T l = T(0), r = T(0);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const T val = q[2 - sh + j][v];
+                    l += T(EnoRows<3>::c[sh][j]) * val;
+                    r += T(EnoRows<3>::c[sh + 1][j]) * val;
+                }
+                lw += l * wL[sh];
+                rw += r * wR[sh];
+            }
+            L[v] = lw;
+            R[v] = rw;
+        }
+    }
+};
+
 // Stencil radius of a reconstruction: cells each side of the reconstructed
 // cell (1 unless the reconstruction declares more).
 template <class R, class = void>

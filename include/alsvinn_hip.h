@@ -40,8 +40,8 @@ enum {
 };
 /* Reconstructions: none (NoReconstruction.cpp), weno2 (WENO2.hpp), eno2,
  * eno3, eno4 (ENOCPU.cpp, orders 2-4; ghost cells = order), mc (MC.hpp), mm
- * (MM.hpp), wenof2 (WENOF2.hpp, Euler only; ReconstructionFactory.cpp:
- * 59-117). */
+ * (MM.hpp), wenof2 (WENOF2.hpp, Euler only), weno3 (WENOCPU.cpp, order 3,
+ * 3 ghost cells); ReconstructionFactory.cpp:59-160. */
 enum {
     ALSVINN_HIP_RECON_NONE = 0,
     ALSVINN_HIP_RECON_WENO2 = 1,
@@ -50,7 +50,8 @@ enum {
     ALSVINN_HIP_RECON_MM = 4,
     ALSVINN_HIP_RECON_WENOF2 = 5,
     ALSVINN_HIP_RECON_ENO3 = 6,
-    ALSVINN_HIP_RECON_ENO4 = 7
+    ALSVINN_HIP_RECON_ENO4 = 7,
+    ALSVINN_HIP_RECON_WENO3 = 8
 };
 enum {
     ALSVINN_HIP_INTEGRATOR_FORWARD_EULER = 0,

@@ -17,6 +17,8 @@
 //   WENO2                  alsfvm/include/alsfvm/reconstruction/WENO2.hpp:31-95 (eps 1e-8,
 //                          WENOCoefficients.hpp:21); ranges ReconstructionCPU.cpp:38-97
 //   ENO2, ENO3, ENO4       alsfvm/src/reconstruction/ENOCPU.cpp:41-267, ENOCoefficients.cpp:27-50
+//   WENO3                  alsfvm/src/reconstruction/WENOCPU.cpp:32-186,
+//                          WENOCoefficients.hpp:51-63, WENOCoefficients.cpp:26-38
 //   WENOF2                 alsfvm/include/alsfvm/reconstruction/WENOF2.hpp:31-115,
 //                          euler/PrimitiveVariables.hpp, Euler.hpp:236-245,290-297
 //   no reconstruction      alsfvm/src/reconstruction/NoReconstruction.cpp:42-73
@@ -342,13 +344,15 @@ enum {
     RECON_MM = 4,
     RECON_WENOF2 = 5,
     RECON_ENO3 = 6,
-    RECON_ENO4 = 7
+    RECON_ENO4 = 7,
+    RECON_WENO3 = 8
 };
 
 static int reconGhosts(int recon) {
     if (recon == RECON_NONE) return 1;
     if (recon == RECON_ENO3) return 3;
     if (recon == RECON_ENO4) return 4;
+    if (recon == RECON_WENO3) return 3;
     return 2;
 }
 
@@ -646,6 +650,59 @@ static void reconEno(int order, const Vol& in, int d, Vol& Lo, Vol& Ro, std::vec
     }
 }
 
+// WENOCPU<3> (WENOCPU.cpp:32-186) with WENOCoefficients<3> (beta:
+// WENOCoefficients.hpp:51-63; epsilon 1e-8 and linear weights 3/10, 3/5,
+// 1/10: WENOCoefficients.cpp:26-38): every cell with two neighbours each side
+// in direction d (all rows of the other axes), indicator variable 0.
+static void reconWeno3(const Vol& in, int d, Vol& Lo, Vol& Ro) {
+    const int order = 3;
+    const int nx = in.tx, ny = in.ty, nz = in.tz;
+    const int dxv = d == 0, dyv = d == 1, dzv = d == 2;
+    const real epsilon = 1e-8;
+    const real lin[3] = {3.0 / 10.0, 3.0 / 5.0, 1.0 / 10.0};
+    const real* w = in.v(0);
+    auto sq = [](real x) { return std::pow(x, 2); };
+    for (int z = dzv * (order - 1); z < nz - dzv * (order - 1); z++)
+        for (int y = dyv * (order - 1); y < ny - dyv * (order - 1); y++)
+            for (int x = dxv * (order - 1); x < nx - dxv * (order - 1); x++) {
+                const size_t out = in.idx(x, y, z);
+                real V[5];
+                for (int i = -order + 1; i < order; i++)
+                    V[i + order - 1] = w[in.idx(x + i * dxv, y + i * dyv, z + i * dzv)];
+                real beta[3];
+                beta[0] = 13.0 / 12.0 * sq(V[2] - 2 * V[3] + V[4]) +
+                          1 / 4.0 * sq(3 * V[2] - 4 * V[3] + V[4]);
+                beta[1] = 13.0 / 12.0 * sq(V[1] - 2 * V[2] + V[3]) + 1 / 4.0 * sq(V[1] - V[3]);
+                beta[2] = 13.0 / 12.0 * sq(V[0] - 2 * V[1] + V[2]) +
+                          1 / 4.0 * sq(V[0] - 4 * V[1] + 3 * V[2]);
+                real aR[3], aL[3], sumR = 0.0, sumL = 0.0;
+                for (int i = order - 1; i >= 0; --i) {  // computeAlpha recursion: i = 2, 1, 0
+                    aR[i] = lin[i] / sq(beta[i] + epsilon);
+                    sumR += aR[i];
+                    aL[i] = lin[order - 1 - i] / sq(beta[i] + epsilon);
+                    sumL += aL[i];
+                }
+                for (int var = 0; var < in.nvar; ++var) {
+                    const real* pin = in.v(var);
+                    real lw = 0.0, rw = 0.0;
+                    for (int shift = 0; shift < order; shift++) {
+                        real lv = 0.0, rv = 0.0;
+                        for (int j = 0; j < order; j++) {
+                            const real value = pin[in.idx(x - (shift - j) * dxv,
+                                                          y - (shift - j) * dyv,
+                                                          z - (shift - j) * dzv)];
+                            lv += enoCoefficient(order, shift, j) * value;
+                            rv += enoCoefficient(order, shift + 1, j) * value;
+                        }
+                        lw += lv * (aL[shift] / sumL);
+                        rw += rv * (aR[shift] / sumR);
+                    }
+                    Lo.v(var)[out] = lw;
+                    Ro.v(var)[out] = rw;
+                }
+            }
+}
+
 // ---------------------------------------------------------------- numerical flux
 
 // NumericalFluxCPU.cpp:121-146
@@ -748,6 +805,8 @@ struct System : SystemBase {
             reconLimiter(U, d, left, right, s, e, recon == RECON_MC);
         else if (recon == RECON_WENOF2)
             reconWenoF2(U, d, left, right, s, e, eq.gamma);
+        else if (recon == RECON_WENO3)
+            reconWeno3(U, d, left, right);
         else
             reconEno(recon == RECON_ENO2 ? 2 : (recon == RECON_ENO3 ? 3 : 4), U, d, left, right,
                      dd, s, e);
@@ -987,7 +1046,7 @@ struct Sim {
 
 static bool validSetup(const Setup& s) {
     if (s.equation < 0 || s.equation > 3) return false;
-    if (s.recon < 0 || s.recon > 7) return false;
+    if (s.recon < 0 || s.recon > 8) return false;
     if (s.recon == RECON_WENOF2 && s.equation == 0) return false;  // Euler only
     if (s.integrator < 0 || s.integrator > 2) return false;
     if (s.num_adjusters < 0 || s.num_adjusters > 2) return false;
@@ -1154,6 +1213,8 @@ int oracle_reconstruct_1d(int recon, int nvar, int n, const double* q, double* l
     else if (recon == orc::RECON_WENOF2) {
         if (nvar < 3) return 1;
         orc::reconWenoF2(U, 0, L, R, orc::ivec3(), orc::ivec3(), gamma);
+    } else if (recon == orc::RECON_WENO3) {
+        orc::reconWeno3(U, 0, L, R);
     } else
         orc::reconEno(recon == orc::RECON_ENO2 ? 2 : (recon == orc::RECON_ENO3 ? 3 : 4), U, 0, L, R,
                       dd, orc::ivec3(), orc::ivec3());

@@ -449,17 +449,17 @@ def test_wenof2_bit_exact(hip, eq):
     _assert_bit_exact(ref, got)
 
 
-@pytest.mark.parametrize("recon", [O.RECON_ENO3, O.RECON_ENO4])
+@pytest.mark.parametrize("recon", [O.RECON_ENO3, O.RECON_ENO4, O.RECON_WENO3])
 @pytest.mark.parametrize("case", ["sod", "kh2d", "euler3", "burgers2d"])
-def test_eno3_eno4_wide_kernel_bit_exact(hip, recon, case):
-    """eno3 / eno4 (3 and 4 ghost layers) through the per-cell wide-stencil
+def test_wide_stencil_reconstructions_bit_exact(hip, recon, case):
+    """eno3 / eno4 / weno3 (3, 4, 3 ghost layers) through the per-cell wide-stencil
     kernel: Sod (1D, HLL3, SSP-RK3), Kelvin-Helmholtz (2D periodic, HLL3,
     SSP-RK3), random 3D data (periodic, HLL, SSP-RK2) and 2D Burgers (Neumann,
     Godunov, forward Euler), against the oracle."""
     from alsvinn_amd import initial_data as ic
     from alsvinn_amd import sampling
-    g = 3 if recon == O.RECON_ENO3 else 4
-    _print_case("eno%d %s" % (g, case))
+    g = 4 if recon == O.RECON_ENO4 else 3
+    _print_case("recon %d %s" % (recon, case))
     rng = np.random.default_rng(31 + g)
     steps = 10
     if case == "sod":

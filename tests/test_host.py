@@ -43,7 +43,9 @@ def test_ghost_cells_and_validation():
     assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == 3
     s.reconstruction = hip.RECON_ENO4
     assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == 4
-    s.reconstruction = 8
+    s.reconstruction = hip.RECON_WENO3
+    assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == 3
+    s.reconstruction = 9
     assert L.alsvinn_hip_ghost_cells(ctypes.byref(s)) == -1
     assert b"unknown reconstruction" in L.alsvinn_hip_last_error()
     bad = hip.make_scheme(hip.EQ_BURGERS, hip.FLUX_HLL, hip.RECON_NONE, hip.INT_FE, hip.BC_NEUMANN)

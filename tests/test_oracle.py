@@ -214,11 +214,44 @@ def test_eno3_eno4_convergence(order, recon, rate):
     assert rates[-1] >= rate, rates
 
 
-@pytest.mark.parametrize("order,recon", [(2, O.RECON_ENO2), (3, O.RECON_ENO3), (4, O.RECON_ENO4)])
+def test_weno3_fifth_order():
+    """WENOCPU<3> with the linear weights 3/10, 3/5, 1/10 combines three
+    third-order stencils into a fifth-order face value on smooth data (cell
+    averages of sin(2 pi x) + 2; not in the reference's convergence test)."""
+    errs = []
+    for k in range(5, 10):
+        N = 1 << k
+        h = 1.0 / N
+        x0 = (np.arange(N + 6) - 3) * h
+        avg = (-np.cos(2 * np.pi * (x0 + h)) + np.cos(2 * np.pi * x0)) / (2 * np.pi * h) + 2
+        left, right = O.reconstruct_1d(O.RECON_WENO3, np.vstack([avg, avg]))
+        f = lambda x: np.sin(2 * np.pi * x) + 2
+        sl = slice(3, 3 + N)
+        errs.append(np.mean(np.abs(left[0, sl] - f(x0[sl]))) +
+                    np.mean(np.abs(right[0, sl] - f(x0[sl] + h))))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] >= 4.9, rates
+
+
+def test_weno3_mirror_symmetry():
+    """The left face uses the mirrored linear weights and coefficient rows:
+    reconstructing the reversed line swaps the left and right face values."""
+    rng = np.random.default_rng(17)
+    q = rng.random((2, 30)) + 1
+    left, right = O.reconstruct_1d(O.RECON_WENO3, q)
+    lr, rr = O.reconstruct_1d(O.RECON_WENO3, q[:, ::-1].copy())
+    sl = slice(2, 28)
+    np.testing.assert_allclose(left[:, sl], rr[:, ::-1][:, sl], rtol=1e-14)
+    np.testing.assert_allclose(right[:, sl], lr[:, ::-1][:, sl], rtol=1e-14)
+
+
+@pytest.mark.parametrize("order,recon", [(2, O.RECON_ENO2), (3, O.RECON_ENO3), (4, O.RECON_ENO4),
+                                         (3, O.RECON_WENO3)])
 def test_eno_reproduces_polynomials(order, recon):
     """Every candidate stencil of ENO-k reconstructs the face values of a
     polynomial of degree k-1 exactly from its cell averages, so the result is
-    exact whichever stencil the differences select (rounding only)."""
+    exact whichever stencil the differences select (rounding only); WENO3's
+    convex combination of the ENO3 candidates likewise."""
     rng = np.random.default_rng(order)
     coef = rng.standard_normal(order)
     P = np.polynomial.Polynomial(coef)
