@@ -118,6 +118,19 @@ __global__ __launch_bounds__(256) void onepoint_kernel(const T* __restrict__ U,
     }
 }
 
+// Device scratch of one statistics call, freed on every return path.
+template <class T>
+struct DeviceArray {
+    T* p = nullptr;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() {
+        if (p) hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, sizeof(T) * (n ? n : 1)); }
+};
+
 struct StructGeom {
     int nx, ny, nz, gx, gy, gz, tx, ty, dims, periodic;
     long long vstride;
@@ -333,22 +346,20 @@ int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, dou
     const SampleDesc& D = e->desc[samples[0]];
     const long long ncell = (long long)D.nx * D.ny * D.nz;
     const int nblocks = int((ncell + 255) / 256);
-    long long* d_bases = nullptr;
-    double* d_part = nullptr;
+    DeviceArray<long long> d_bases;
+    DeviceArray<double> d_part;
     const size_t npart = (size_t)m * e->nvar * nblocks * H;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
-    HIP_TRY(hipMalloc(&d_part, sizeof(double) * npart));
-    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(d_part, 0, sizeof(double) * npart));
-    launch(dim3(nblocks, e->nvar, m), d_bases, geom_of(e, D), d_part);
+    HIP_TRY(d_bases.alloc(m));
+    HIP_TRY(d_part.alloc(npart));
+    HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d_part.p, 0, sizeof(double) * npart));
+    launch(dim3(nblocks, e->nvar, m), d_bases.p, geom_of(e, D), d_part.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
     if (le == hipSuccess) le = se;
     std::vector<double> part(npart);
-    hipError_t ce = hipMemcpy(part.data(), d_part, sizeof(double) * npart, hipMemcpyDeviceToHost);
-    hipFree(d_bases);
-    hipFree(d_part);
+    hipError_t ce = hipMemcpy(part.data(), d_part.p, sizeof(double) * npart, hipMemcpyDeviceToHost);
     if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
     if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
     // fixed-order reduction: samples in list order, blocks in index order
@@ -375,27 +386,24 @@ int alsvinn_hip_stats_meanvar(alsvinn_hip_ensemble* e, const int* samples, int m
     if (!host_mean || !host_var) return fail(ALSVINN_HIP_E_INVALID, "output is NULL");
     const SampleDesc& D = e->desc[samples[0]];
     const long long n = D.vstride * e->nvar;
-    long long* d_bases = nullptr;
-    double *d_mean = nullptr, *d_var = nullptr;
+    DeviceArray<long long> d_bases;
+    DeviceArray<double> d_mean, d_var;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
-    HIP_TRY(hipMalloc(&d_mean, sizeof(double) * n));
-    HIP_TRY(hipMalloc(&d_var, sizeof(double) * n));
-    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    HIP_TRY(d_bases.alloc(m));
+    HIP_TRY(d_mean.alloc(n));
+    HIP_TRY(d_var.alloc(n));
+    HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
     if (e->esize == 8)
         hipLaunchKernelGGL(meanvar_kernel<double>, dim3(blocks), dim3(256), 0, e->stream,
-                           (const double*)e->U, d_bases, m, n, welford, d_mean, d_var);
+                           (const double*)e->U, d_bases.p, m, n, welford, d_mean.p, d_var.p);
     else
         hipLaunchKernelGGL(meanvar_kernel<float>, dim3(blocks), dim3(256), 0, e->stream,
-                           (const float*)e->U, d_bases, m, n, welford, d_mean, d_var);
+                           (const float*)e->U, d_bases.p, m, n, welford, d_mean.p, d_var.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
-    hipError_t c1 = hipMemcpy(host_mean, d_mean, sizeof(double) * n, hipMemcpyDeviceToHost);
-    hipError_t c2 = hipMemcpy(host_var, d_var, sizeof(double) * n, hipMemcpyDeviceToHost);
-    hipFree(d_bases);
-    hipFree(d_mean);
-    hipFree(d_var);
+    hipError_t c1 = hipMemcpy(host_mean, d_mean.p, sizeof(double) * n, hipMemcpyDeviceToHost);
+    hipError_t c2 = hipMemcpy(host_var, d_var.p, sizeof(double) * n, hipMemcpyDeviceToHost);
     for (hipError_t x : {le, se, c1, c2})
         if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
     return 0;
@@ -408,24 +416,22 @@ int alsvinn_hip_stats_onepoint_moment(alsvinn_hip_ensemble* e, const int* sample
     if (!host_out) return fail(ALSVINN_HIP_E_INVALID, "output is NULL");
     const SampleDesc& D = e->desc[samples[0]];
     const long long n = D.vstride * e->nvar;
-    long long* d_bases = nullptr;
-    double* d_out = nullptr;
+    DeviceArray<long long> d_bases;
+    DeviceArray<double> d_out;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
-    HIP_TRY(hipMalloc(&d_out, sizeof(double) * n));
-    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    HIP_TRY(d_bases.alloc(m));
+    HIP_TRY(d_out.alloc(n));
+    HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
     if (e->esize == 8)
         hipLaunchKernelGGL(onepoint_kernel<double>, dim3(blocks), dim3(256), 0, e->stream,
-                           (const double*)e->U, d_bases, m, n, p, d_out);
+                           (const double*)e->U, d_bases.p, m, n, p, d_out.p);
     else
         hipLaunchKernelGGL(onepoint_kernel<float>, dim3(blocks), dim3(256), 0, e->stream,
-                           (const float*)e->U, d_bases, m, n, p, d_out);
+                           (const float*)e->U, d_bases.p, m, n, p, d_out.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
-    hipError_t c1 = hipMemcpy(host_out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost);
-    hipFree(d_bases);
-    hipFree(d_out);
+    hipError_t c1 = hipMemcpy(host_out, d_out.p, sizeof(double) * n, hipMemcpyDeviceToHost);
     for (hipError_t x : {le, se, c1})
         if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
     return 0;
@@ -439,25 +445,23 @@ int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samp
     if (int rc = group_bases(e, samples, m, bases)) return rc;
     const SampleDesc& D = e->desc[samples[0]];
     const size_t np = (size_t)m * e->nvar;
-    long long* d_bases = nullptr;
-    double* d_tv = nullptr;
+    DeviceArray<long long> d_bases;
+    DeviceArray<double> d_tv;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMalloc(&d_bases, sizeof(long long) * m));
-    HIP_TRY(hipMalloc(&d_tv, sizeof(double) * np));
-    HIP_TRY(hipMemcpy(d_bases, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+    HIP_TRY(d_bases.alloc(m));
+    HIP_TRY(d_tv.alloc(np));
+    HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     const dim3 grid(e->nvar, m);
     if (e->esize == 8)
         hipLaunchKernelGGL(bv_kernel<double>, grid, dim3(64), 0, e->stream, (const double*)e->U,
-                           d_bases, geom_of(e, D), D.tz, p, e->nvar, d_tv);
+                           d_bases.p, geom_of(e, D), D.tz, p, e->nvar, d_tv.p);
     else
         hipLaunchKernelGGL(bv_kernel<float>, grid, dim3(64), 0, e->stream, (const float*)e->U,
-                           d_bases, geom_of(e, D), D.tz, p, e->nvar, d_tv);
+                           d_bases.p, geom_of(e, D), D.tz, p, e->nvar, d_tv.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
     std::vector<double> tv(np);
-    hipError_t ce = hipMemcpy(tv.data(), d_tv, sizeof(double) * np, hipMemcpyDeviceToHost);
-    hipFree(d_bases);
-    hipFree(d_tv);
+    hipError_t ce = hipMemcpy(tv.data(), d_tv.p, sizeof(double) * np, hipMemcpyDeviceToHost);
     for (hipError_t x : {le, se, ce})
         if (x != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(x));
     // the statistic's snapshot adds the samples' variations in order, then
@@ -508,21 +512,20 @@ int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host) {
     if (!e || !host) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
     const int blocks = 64;
     const size_t np = (size_t)e->nsamples * e->nvar * blocks;
-    double* d_part = nullptr;
+    DeviceArray<double> d_part;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMalloc(&d_part, sizeof(double) * np));
+    HIP_TRY(d_part.alloc(np));
     if (e->esize == 8)
         hipLaunchKernelGGL(totals_kernel<double>, dim3(blocks, e->nvar, e->nsamples), dim3(256), 0,
-                           e->stream, (const double*)e->U, e->d_desc, e->nvar, d_part);
+                           e->stream, (const double*)e->U, e->d_desc, e->nvar, d_part.p);
     else
         hipLaunchKernelGGL(totals_kernel<float>, dim3(blocks, e->nvar, e->nsamples), dim3(256), 0,
-                           e->stream, (const float*)e->U, e->d_desc, e->nvar, d_part);
+                           e->stream, (const float*)e->U, e->d_desc, e->nvar, d_part.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
     if (le == hipSuccess) le = se;
     std::vector<double> part(np);
-    hipError_t ce = hipMemcpy(part.data(), d_part, sizeof(double) * np, hipMemcpyDeviceToHost);
-    hipFree(d_part);
+    hipError_t ce = hipMemcpy(part.data(), d_part.p, sizeof(double) * np, hipMemcpyDeviceToHost);
     if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
     if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
     for (int s = 0; s < e->nsamples; ++s)
