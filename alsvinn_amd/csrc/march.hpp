@@ -657,20 +657,24 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     }
 }
 
-// Stage kernel for reconstructions wider than three cells (ENO3, ENO4): the
-// same tasks, arithmetic, outputs and wave-speed reduction as the marching
-// kernels, but every interior lane evaluates its own cell from memory: per
-// direction it loads the 2*radius+3 cells of its line, reconstructs the cell
-// and both neighbours and evaluates the fluxes of both faces (each face flux
-// is thus evaluated by the two cells beside it). The divergence is summed in
-// the marching kernels' order: ((0 + x part) + y part) + z part.
+// Stage kernel for reconstructions wider than three cells (ENO3, ENO4,
+// WENO3): the same tasks, arithmetic, outputs and wave-speed reduction as the
+// marching kernels, with the stencils read from memory (L1/L2) instead of a
+// register window. x: every lane of the strip (and the halo lane each side)
+// reconstructs its cell and evaluates the flux of its right face; the left
+// face's flux comes from the lane below (DPP), as in the marching kernels.
+// The wavefront marches along y (2D) or z (3D), keeping the last face flux
+// and right face value, so each face flux in that direction is evaluated
+// once; in 3D the y faces of a row are evaluated by the row itself (twice
+// per face). The divergence is summed in the marching kernels' order:
+// ((0 + x part) + y part) + z part.
 template <class T, class Eq, class Recon, class Flux, int DIM, int MODE>
 __global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     constexpr int RAD = recon_radius<Recon>::value;
-    constexpr int NS = 2 * RAD + 3;  // cells p-1-RAD .. p+1+RAD of one line
+    constexpr int MD = DIM == 3 ? 2 : 1;  // marched direction (DIM >= 2)
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -690,12 +694,14 @@ __global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
         const SampleDesc D = a.desc[sample];
         SampleState* Sp = a.state + sample;
         const bool active = in_strip && runnable(*Sp);
-        if (!__any(active)) continue;
+        if (!__any(active)) continue;  // wave-uniform
         const bool need_dt = LAZY || MODE != MODE_RHS;
         T c = T(0);
         if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
         const int col = tk.x0[sk] - G + li;
         const bool interior = active && li >= G && li < G + ncols;
+        const bool xlane = active && li >= G - 1 && li <= G + ncols;  // reconstructs in x
+        const bool xface = active && li >= G - 1 && li < G + ncols;   // face col+1/2
         const long long vs = D.vstride;
         const T* in = a.in + D.base;
         const T* in2 = a.in2 + D.base;
@@ -703,56 +709,115 @@ __global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
         const long long sxd[3] = {1, (long long)D.tx, (long long)D.tx * D.ty};
         T ws[3] = {T(0), T(0), T(0)};
 
-        // divergence part of direction d at flat index i0: (-F(p+1/2)) + F(p-1/2)
-        auto direction = [&](auto dc, long long i0, T* part) {
+        // stencil of 2*RAD+1 cells around flat index i in direction d; the
+        // centre cell (the stage input there) is left in q[RAD]
+        auto recon_at = [&](auto dc, long long i, T (*q)[NV], T* L, T* R) {
             constexpr int d = decltype(dc)::value;
-            T q[NS][NV];
 #pragma unroll
-            for (int k = 0; k < NS; ++k)
-                load_cell<T, NV>(in, in2, LAZY, c, i0 + (k - 1 - RAD) * sxd[d], vs, q[k]);
-            T Lm[NV], Rm[NV], L0[NV], R0[NV], Lp[NV], Rp[NV];
-            recon_stencil<Recon, T, NV>(q + 0, Lm, Rm, eq.gamma);
-            recon_stencil<Recon, T, NV>(q + 1, L0, R0, eq.gamma);
-            recon_stencil<Recon, T, NV>(q + 2, Lp, Rp, eq.gamma);
-            T Fm[NV], Fp[NV];
-            const T sm = face_flux<T, Eq, Flux, d>(eq, Rm, L0, Fm);
-            const T sp = face_flux<T, Eq, Flux, d>(eq, R0, Lp, Fp);
-            ws[d] = std_max(std_max(ws[d], sm), sp);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) part[v] = (T(-1.0) * Fp[v]) + Fm[v];
+            for (int k = 0; k <= 2 * RAD; ++k)
+                load_cell<T, NV>(in, in2, LAZY, c, i + (k - RAD) * sxd[d], vs, q[k]);
+            recon_stencil<Recon, T, NV>(q, L, R, eq.gamma);
         };
 
-        const int zb = DIM >= 3 ? tk.z0 : 0, ze = DIM >= 3 ? tk.z1 : 1;
-        for (int z = zb; z < ze; ++z)
-            for (int r = tk.y0; r < tk.y1; ++r) {
-                if (!interior) continue;
-                const long long i0 =
-                    ((long long)(z + D.gz) * D.ty + (r + D.gy)) * D.tx + (col + D.gx);
-                T out_v[NV], part[NV];
-                direction(std::integral_constant<int, 0>(), i0, part);
+        const int o0 = DIM == 3 ? tk.y0 : 0, o1 = DIM == 3 ? tk.y1 : 1;
+        const int m0 = DIM == 3 ? tk.z0 : (DIM == 2 ? tk.y0 : 0);
+        const int m1 = DIM == 3 ? tk.z1 : (DIM == 2 ? tk.y1 : 1);
+        for (int o = o0; o < o1; ++o) {  // uniform over the wavefront
+            auto flat = [&](int m) -> long long {
+                const int y = DIM == 3 ? o : (DIM == 2 ? m : 0);
+                const int z = DIM == 3 ? m : 0;
+                return ((long long)(z + D.gz) * D.ty + (y + D.gy)) * D.tx + (col + D.gx);
+            };
+            T Rprev[NV], Gm[NV];  // marched direction: right value of cell m-1, flux at m-1/2
+            if constexpr (DIM >= 2) {
+                if (interior) {
+                    T q[2 * RAD + 1][NV], Lt[NV], L0[NV];
+                    recon_at(std::integral_constant<int, MD>(), flat(m0 - 1), q, Lt, Rprev);
+                    T Rt[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) out_v[v] = T(0) + part[v];
-                if constexpr (DIM >= 2) {
-                    direction(std::integral_constant<int, 1>(), i0, part);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + part[v];
+                    for (int v = 0; v < NV; ++v) Rt[v] = Rprev[v];
+                    recon_at(std::integral_constant<int, MD>(), flat(m0), q, L0, Rprev);
+                    const T s = face_flux<T, Eq, Flux, MD>(eq, Rt, L0, Gm);
+                    ws[MD] = std_max(ws[MD], s);
                 }
-                if constexpr (DIM >= 3) {
-                    direction(std::integral_constant<int, 2>(), i0, part);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + part[v];
-                }
-                T uin[NV], u0[NV];
-                load_cell<T, NV>(in, in2, LAZY, c, i0, vs, uin);
-                if (MODE != MODE_RHS) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
-                }
-                T res[NV];
-                combine<T, NV>(MODE, c, out_v, uin, u0, res);
-                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, DIM>(out, vs, D, col, r, z, res);
             }
+            for (int m = m0; m < m1; ++m) {  // uniform over the wavefront
+                const long long i0 = flat(m);
+                T out_v[NV], uin[NV];
+                {  // x: own right face, left face from the lane below
+                    T q[2 * RAD + 1][NV], Lx[NV], Rx[NV];
+                    if (xlane) {
+                        recon_at(std::integral_constant<int, 0>(), i0, q, Lx, Rx);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) uin[v] = q[RAD][v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Lx[v] = Rx[v] = uin[v] = T(1);
+                    }
+                    T Ln[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
+                    T Fp[NV];
+                    const T s = face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp);
+                    if (xface) ws[0] = std_max(ws[0], s);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const T Fm = lane_from_lower(Fp[v]);
+                        out_v[v] = T(0) + ((T(-1.0) * Fp[v]) + Fm);
+                    }
+                }
+                if constexpr (DIM == 3) {  // y: both faces of the row
+                    T part[NV];
+                    if (interior) {
+                        T q[2 * RAD + 1][NV], Lm[NV], Rm[NV], L0[NV], R0[NV], Lp[NV], Rp[NV];
+                        recon_at(std::integral_constant<int, 1>(), i0 - sxd[1], q, Lm, Rm);
+                        recon_at(std::integral_constant<int, 1>(), i0, q, L0, R0);
+                        recon_at(std::integral_constant<int, 1>(), i0 + sxd[1], q, Lp, Rp);
+                        T Fm[NV], Fp[NV];
+                        const T sm = face_flux<T, Eq, Flux, 1>(eq, Rm, L0, Fm);
+                        const T sp = face_flux<T, Eq, Flux, 1>(eq, R0, Lp, Fp);
+                        ws[1] = std_max(std_max(ws[1], sm), sp);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) part[v] = (T(-1.0) * Fp[v]) + Fm[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) part[v] = T(0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) out_v[v] = out_v[v] + part[v];
+                }
+                if constexpr (DIM >= 2) {  // marched direction
+                    T Gp[NV];
+                    if (interior) {
+                        T q[2 * RAD + 1][NV], Ln[NV], Rn[NV];
+                        recon_at(std::integral_constant<int, MD>(), flat(m + 1), q, Ln, Rn);
+                        const T s = face_flux<T, Eq, Flux, MD>(eq, Rprev, Ln, Gp);
+                        ws[MD] = std_max(ws[MD], s);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Rprev[v] = Rn[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Gp[v] = T(0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+                        Gm[v] = Gp[v];
+                    }
+                }
+                if (interior) {
+                    T u0[NV], res[NV];
+                    if (MODE != MODE_RHS) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                    }
+                    combine<T, NV>(MODE, c, out_v, uin, u0, res);
+                    if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
+                    store_cell<T, NV, DIM>(out, vs, D, col, DIM == 3 ? o : m, DIM == 3 ? m : 0,
+                                           res);
+                }
+            }
+        }
 
         if (MODE == MODE_RHS && !LAZY) {
 #pragma unroll
