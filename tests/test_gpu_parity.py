@@ -505,3 +505,51 @@ def test_wide_stencil_reconstructions_bit_exact(hip, recon, case):
         assert info["t"] == t_ref
     assert np.array_equal(dts, dts_ref)
     _assert_bit_exact(ref, got)
+
+
+@pytest.mark.parametrize("eq,recon,bc,sizes", [
+    (O.EQ_BURGERS, O.RECON_WENO2, O.BC_PERIODIC, [(2, 2, 1), (3, 5, 1), (61, 7, 1), (62, 3, 1),
+                                                  (125, 2, 1)]),
+    (O.EQ_EULER3, O.RECON_MC, O.BC_NEUMANN, [(2, 2, 2), (3, 2, 5), (61, 3, 2)]),
+    (O.EQ_EULER1, O.RECON_ENO4, O.BC_PERIODIC, [(4, 1, 1), (5, 1, 1), (57, 1, 1), (200, 1, 1)]),
+])
+def test_minimal_and_ragged_grids_mixed_end_times(hip, eq, recon, bc, sizes):
+    """One ensemble of samples on the smallest grids the ghost layers allow,
+    on grids whose strips are narrow or split across wavefronts, each with
+    its own end time: every sample equals its own oracle run bit for bit, with
+    the same number of steps and the same final time."""
+    _print_case("edge eq=%d recon=%d sizes=%s" % (eq, recon, sizes))
+    rng = np.random.default_rng(41 + eq)
+    flux = O.FLUX_GODUNOV if eq == O.EQ_BURGERS else O.FLUX_HLL3
+    integ = O.INT_RK3
+    dim = 3 if eq == O.EQ_EULER3 else (1 if eq == O.EQ_EULER1 else 2)
+    top = (1, 1 if dim >= 2 else 0, 1 if dim == 3 else 0)
+    ends = [0.1 + 0.05 * k for k in range(len(sizes))]
+    setups, u0s = [], []
+    for n, T in zip(sizes, ends):
+        s = O.make_setup(eq, flux, recon, integ, bc, n, (0, 0, 0), top, cfl=0.475, end_time=T)
+        shape = O.volume_shape(s)
+        if eq == O.EQ_BURGERS:
+            u0 = 0.5 + rng.random(shape)
+        else:
+            u0 = np.zeros(shape)
+            u0[0] = 1.0 + 0.2 * rng.random(shape[1:])
+            for k in range(1, shape[0] - 1):
+                u0[k] = 0.1 * rng.standard_normal(shape[1:])
+            u0[-1] = 3.0 + 0.3 * rng.random(shape[1:])
+        setups.append(s)
+        u0s.append(u0)
+    scheme = hip.make_scheme(eq, flux, recon, integ, bc, gamma=1.4, cfl=0.475)
+    samples = [hip.make_sample(n, (0, 0, 0), top, T) for n, T in zip(sizes, ends)]
+    with hip.Ensemble(scheme, samples) as e:
+        for i, u in enumerate(u0s):
+            e.upload(i, u)
+        e.apply_boundary()
+        e.run(4)
+        got = [e.download(i) for i in range(len(sizes))]
+        infos = [e.info(i) for i in range(len(sizes))]
+    for i, s in enumerate(setups):
+        rc, ref, t, steps, _ = O.run(s, u0s[i])
+        assert rc == 0
+        assert infos[i]["steps"] == steps and infos[i]["t"] == t == ends[i], (i, sizes[i])
+        _assert_bit_exact(ref, got[i])
