@@ -12,9 +12,10 @@
 //   alsfvm::hip::Boundary         boundary::Boundary::applyBoundaryConditions (Boundary.hpp:30-31)
 //   alsuq::hip::Runner            alsuq::run::Runner::run (Runner.cpp:39-90): all samples
 //                                 of the rank advance together
-//   alsuq::hip::MeanVariance, StructureCube, StructureBasic
-//                                 alsuq::stats (MeanVariance.cpp:30-67, StructureCube.cpp:37-76,
-//                                 StructureBasic.cpp:67-120)
+//   alsuq::hip::MeanVariance, StructureCube, StructureBasic, OnePointMoment,
+//   BoundedVariation              alsuq::stats (MeanVariance.cpp:30-67, StructureCube.cpp:37-76,
+//                                 StructureBasic.cpp:67-120, OnePointMoment.cpp:30-45,
+//                                 BoundedVariation.cpp:38-53)
 //
 // Volumes cross the boundary as host vectors in the reference's layout
 // (variable-major, z*tx*ty + y*tx + x over padded extents).
@@ -314,6 +315,71 @@ private:
     Runner& runner;
     double p;
     int numberOfH;
+};
+
+// stats::StructureBasic (p, numberOfH, direction) over all samples of a Runner.
+class StructureBasic {
+public:
+    StructureBasic(Runner& r, double p, int numberOfH, int direction)
+        : runner(r), p(p), numberOfH(numberOfH), direction(direction) {}
+    std::vector<double> computeStatistics(int nvar) {
+        std::vector<int> idx = all_samples();
+        std::vector<double> out((size_t)nvar * numberOfH);
+        check(alsvinn_hip_stats_structure_basic(runner.handle(), idx.data(), (int)idx.size(),
+                                                numberOfH, p, direction, out.data()),
+              "StructureBasic");
+        return out;
+    }
+
+private:
+    std::vector<int> all_samples() const {
+        std::vector<int> idx(runner.getNumberOfSamples());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        return idx;
+    }
+    Runner& runner;
+    double p;
+    int numberOfH, direction;
+};
+
+// stats::OnePointMoment (integer p, OnePointMoment.cpp:21-45): per cell, the
+// mean over all samples of pow(u, p), in the volume layout.
+class OnePointMoment {
+public:
+    OnePointMoment(Runner& r, int p) : runner(r), p(p) {}
+    std::vector<double> computeStatistics() {
+        std::vector<int> idx(runner.getNumberOfSamples());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::vector<double> out(runner.getSample(0).size());
+        check(alsvinn_hip_stats_onepoint_moment(runner.handle(), idx.data(), (int)idx.size(),
+                                                double(p), out.data()),
+              "OnePointMoment");
+        return out;
+    }
+
+private:
+    Runner& runner;
+    int p;
+};
+
+// stats::BoundedVariation (integer p, BoundedVariation.cpp:23-53): per
+// variable, the mean over all samples of the total variation; 1D and 2D.
+class BoundedVariation {
+public:
+    BoundedVariation(Runner& r, int p) : runner(r), p(p) {}
+    std::vector<double> computeStatistics(int nvar) {
+        std::vector<int> idx(runner.getNumberOfSamples());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::vector<double> out(nvar);
+        check(alsvinn_hip_stats_bounded_variation(runner.handle(), idx.data(), (int)idx.size(),
+                                                  double(p), out.data()),
+              "BoundedVariation");
+        return out;
+    }
+
+private:
+    Runner& runner;
+    int p;
 };
 
 }  // namespace hip

@@ -22,6 +22,12 @@ int oracle_run(const oracle_setup* setup, const double* u0, double* uout, double
 int oracle_system_rhs(const oracle_setup* setup, const double* u, double* out, double* ws3);
 int oracle_apply_boundary(const oracle_setup* setup, double* u);
 void oracle_meanvar(long nsamples, long len, const double* samples, double* mean, double* var);
+void oracle_onepoint_moment(long nsamples, long len, const double* samples, double p,
+                            double* out);
+double oracle_total_variation(int tx, int ty, int tz, int gx, int gy, int gz, const double* u,
+                              double p);
+void oracle_structure_basic(int nvar, int nx, int ny, int nz, int ng, const double* u, int H,
+                            double p, int direction, int periodic, double* out);
 void oracle_stl_mersenne_uniform(long sample_start, long nsamples, long dim, double lower,
                                  double upper, double* out);
 

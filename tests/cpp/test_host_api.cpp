@@ -1,6 +1,7 @@
 // The C++ host layer (include/alsvinn_hip.hpp) against the oracle, on the GPU:
 // the reference-style Simulator loop, NumericalFlux::computeFlux, the batched
-// Runner with the STLMersenne sampler and MeanVariance. Prints one line per
+// Runner with the STLMersenne sampler, MeanVariance, OnePointMoment,
+// BoundedVariation and StructureBasic. Prints one line per
 // case and exits non-zero on the first mismatch.
 #include <cmath>
 #include <cstdio>
@@ -142,6 +143,32 @@ int main() {
         mv.computeStatistics(mean, var);
         oracle_meanvar(3, (long)mref.size(), all.data(), mref.data(), vref.data());
         EXPECT(same_bits(mean, mref) && same_bits(var, vref), "meanvar bit-identical");
+        // OnePointMoment p = 2 and BoundedVariation p = 1: sequential sums, bit for bit
+        std::vector<double> m2ref(mref.size());
+        oracle_onepoint_moment(3, (long)m2ref.size(), all.data(), 2.0, m2ref.data());
+        EXPECT(same_bits(alsuq::hip::OnePointMoment(runner, 2).computeStatistics(), m2ref),
+               "onepointmoment bit-identical");
+        const size_t vol = (size_t)36 * 36;
+        std::vector<double> bvref(4, 0.0);
+        for (int v = 0; v < 4; ++v) {
+            double acc = 0;
+            for (int k = 0; k < 3; ++k)
+                acc += oracle_total_variation(36, 36, 1, 2, 2, 0, all.data() + (size_t)k * 4 * vol + v * vol,
+                                              1.0);
+            bvref[v] = acc / 3;
+        }
+        EXPECT(same_bits(alsuq::hip::BoundedVariation(runner, 1).computeStatistics(4), bvref),
+               "bounded variation bit-identical");
+        // StructureBasic in y, p = 2: parallel summation, 1e-12 relative
+        std::vector<double> sbref(4 * 5, 0.0);
+        for (int k = 0; k < 3; ++k)
+            oracle_structure_basic(4, 32, 32, 1, 2, all.data() + (size_t)k * 4 * vol, 5, 2.0, 1, 1,
+                                   sbref.data());
+        const auto sb = alsuq::hip::StructureBasic(runner, 2.0, 5, 1).computeStatistics(4);
+        bool sok = sb.size() == sbref.size();
+        for (size_t i = 0; sok && i < sb.size(); ++i)
+            sok = std::fabs(sb[i] - sbref[i] / 3) <= 1e-12 * std::fabs(sbref[i] / 3);
+        EXPECT(sok, "structure_basic within 1e-12");
     }
     // 3. NumericalFlux::computeFlux vs ConservedSystem
     {
