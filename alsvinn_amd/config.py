@@ -49,7 +49,8 @@ FLUXES = {"hll": _lib.FLUX_HLL, "hll3": _lib.FLUX_HLL3, "godunov": _lib.FLUX_GOD
 RECONSTRUCTIONS = {"none": _lib.RECON_NONE, "weno2": _lib.RECON_WENO2, "eno2": _lib.RECON_ENO2,
                    "eno3": _lib.RECON_ENO3, "eno4": _lib.RECON_ENO4, "mc": _lib.RECON_MC,
                    "mm": _lib.RECON_MM, "wenof2": _lib.RECON_WENOF2, "weno3": _lib.RECON_WENO3}
-INTEGRATORS = {"forwardeuler": _lib.INT_FE, "rungekutta2": _lib.INT_RK2, "rungekutta3": _lib.INT_RK3}
+INTEGRATORS = {"forwardeuler": _lib.INT_FE, "rungekutta2": _lib.INT_RK2, "rungekutta3": _lib.INT_RK3,
+               "rungekutta4": _lib.INT_RK4}
 BOUNDARIES = {"periodic": _lib.BC_PERIODIC, "neumann": _lib.BC_NEUMANN}
 PRIMITIVES = {"burgers": ["u"], "euler1": ["rho", "ux", "p"], "euler2": ["rho", "ux", "uy", "p"],
               "euler3": ["rho", "ux", "uy", "uz", "p"]}

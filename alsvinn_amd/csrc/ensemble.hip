@@ -89,7 +89,7 @@ int validate(const alsvinn_hip_scheme* s) {
         return fail(ALSVINN_HIP_E_INVALID, "unknown equation " + std::to_string(s->equation));
     if (s->reconstruction < 0 || s->reconstruction > 8)
         return fail(ALSVINN_HIP_E_INVALID, "unknown reconstruction " + std::to_string(s->reconstruction));
-    if (s->integrator < 0 || s->integrator > 2)
+    if (s->integrator < 0 || s->integrator > 3)
         return fail(ALSVINN_HIP_E_INVALID, "unknown integrator " + std::to_string(s->integrator));
     if (s->boundary < 0 || s->boundary > 1)
         return fail(ALSVINN_HIP_E_INVALID, "unknown boundary " + std::to_string(s->boundary));
@@ -289,6 +289,7 @@ struct KernelTimer {
 template <class T>
 int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm = nullptr) {
     StageArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
     a.desc = e->d_desc;
     a.state = e->d_state;
     a.ws = e->d_ws;
@@ -299,6 +300,8 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     T* U = (T*)e->U;
     T* A = (T*)e->A;
     T* B = (T*)e->B;
+    T* C = (T*)e->C;
+    T* Dv = (T*)e->D;
     const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->max_vstride + 1023) / 1024));
     const dim3 pgrid(bx, e->nsamples);
 
@@ -329,6 +332,33 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
         a.out = U;
         a.check = 1;
         stage();
+    } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK4) {
+        // RungeKutta4.cpp:50-96: U1 = ((L0*c)*0.5) + U0, U2 = ((L1*c)*0.5) + U0,
+        // U3 = L2*c + U0, U' = (c/6)L3 + U1/3 + 2U2/3 + U3/3 - U0/3; only the
+        // first evaluation feeds the CFL condition
+        auto rk4 = [&](int kind, const T* X, const T* Z, const T* W, const T* V, T* O, int check) {
+            if (tm) tm->begin();
+            k.rk4(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, kind, X, U, Z, W, V, O, check,
+                  e->scheme.gamma);
+            if (tm) tm->end(1);
+        };
+        rk4(0, A, nullptr, nullptr, nullptr, B, 0);  // U1
+        a.no_ws = 1;
+        a.in = B;
+        a.in2 = B;
+        a.out = C;
+        stage();                                      // L1
+        rk4(0, C, nullptr, nullptr, nullptr, C, 0);  // U2
+        a.in = C;
+        a.in2 = C;
+        a.out = Dv;
+        stage();                                      // L2
+        axpy(Dv, U, Dv, 0);                           // U3
+        a.in = Dv;
+        a.in2 = Dv;
+        a.out = A;
+        stage();                                      // L3
+        rk4(1, A, B, C, Dv, U, 1);                    // U^{n+1}
     } else {
         a.mode = MODE_RK3_1;
         a.in = A;
@@ -478,10 +508,15 @@ static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* sche
     const size_t bytes = (size_t)e->total * e->esize;
     if (e->own_U) HIP_TRY_C(hipMalloc(&e->U, bytes));
     HIP_TRY_C(hipMalloc(&e->A, bytes));
-    if (scheme->integrator == ALSVINN_HIP_INTEGRATOR_RK3) HIP_TRY_C(hipMalloc(&e->B, bytes));
+    const bool rk4 = scheme->integrator == ALSVINN_HIP_INTEGRATOR_RK4;
+    if (scheme->integrator == ALSVINN_HIP_INTEGRATOR_RK3 || rk4) HIP_TRY_C(hipMalloc(&e->B, bytes));
+    if (rk4) HIP_TRY_C(hipMalloc(&e->C, bytes));
+    if (rk4) HIP_TRY_C(hipMalloc(&e->D, bytes));
     if (e->own_U) HIP_TRY_C(hipMemset(e->U, 0, bytes));
     HIP_TRY_C(hipMemset(e->A, 0, bytes));
     if (e->B) HIP_TRY_C(hipMemset(e->B, 0, bytes));
+    if (e->C) HIP_TRY_C(hipMemset(e->C, 0, bytes));
+    if (e->D) HIP_TRY_C(hipMemset(e->D, 0, bytes));
     HIP_TRY_C(hipMalloc(&e->d_desc, sizeof(SampleDesc) * nsamples));
     HIP_TRY_C(hipMalloc(&e->d_state, sizeof(SampleState) * nsamples));
     HIP_TRY_C(hipMalloc(&e->d_ws, sizeof(unsigned long long) * 3 * nsamples));
@@ -593,6 +628,8 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     if (e->own_U) hipFree(e->U);
     hipFree(e->A);
     hipFree(e->B);
+    hipFree(e->C);
+    hipFree(e->D);
     hipFree(e->d_desc);
     hipFree(e->d_state);
     hipFree(e->d_ws);

@@ -51,6 +51,61 @@ __global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict_
     if (bad) atomicOr(&Sp->status, 2);
 }
 
+// Stage combinations of the classical RK4 integrator (RungeKutta4.cpp:50-96)
+// over whole volumes (ghost images included), c = dt/dx per sample:
+//   kind 0: out = ((X*c)*0.5) + Y                       (U1, U2; Y = U0)
+//   kind 1: out = (((((c/6)*X + (1/3)*Z) + (2/3)*W) + (1/3)*V) + (-1/3)*Y)
+//           with X = L(U3), Z = U1, W = U2, V = U3, Y = U0
+//           (HostMemory::addLinearCombination, HostMemory.cpp:266-279),
+//           followed by the constraint check when `check` is set.
+template <class T, class Eq>
+__global__ __launch_bounds__(256) void rk4_kernel(const SampleDesc* __restrict__ desc,
+                                                 SampleState* state,
+                                                 const unsigned long long* __restrict__ ws,
+                                                 int kind, const T* X, const T* Y, const T* Z,
+                                                 const T* W, const T* V, T* out, int check,
+                                                 double gamma) {
+    const int s = blockIdx.y;
+    const SampleDesc D = desc[s];
+    SampleState* Sp = state + s;
+    if (!runnable(*Sp)) return;
+    __shared__ double cs;
+    if (threadIdx.x == 0) cs = sample_dt(D, *Sp, ws + 3 * s) / D.dx[0];
+    __syncthreads();
+    const T c = T(cs);
+    const T a1 = T((1.0 / 6.0) * cs);
+    const long long n = D.vstride;
+    const long long b = D.base;
+    Eq eq;
+    eq.gamma = T(gamma);
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        T r[Eq::nvar];
+#pragma unroll
+        for (int v = 0; v < Eq::nvar; ++v) {
+            const long long j = b + i + v * n;
+            if (kind == 0) {
+                r[v] = ((X[j] * c) * T(0.5)) + Y[j];
+            } else {
+                r[v] = ((((a1 * X[j] + T(1.0 / 3.0) * Z[j]) + T(2.0 / 3.0) * W[j]) +
+                         T(1.0 / 3.0) * V[j]) +
+                        T(-1.0 / 3.0) * Y[j]);
+            }
+            out[j] = r[v];
+        }
+        if (check) {
+            const int px = int(i % D.tx);
+            const int py = int((i / D.tx) % D.ty);
+            const int pz = int(i / ((long long)D.tx * D.ty));
+            const bool inside = px >= D.gx && px < D.gx + D.nx && py >= D.gy && py < D.gy + D.ny &&
+                                pz >= D.gz && pz < D.gz + D.nz;
+            if (inside && !eq.obeys(r)) bad = true;
+        }
+    }
+    if (bad) atomicOr(&Sp->status, 2);
+}
+
 // Source cell of padded position p on one axis (BoundaryCPU.cpp:53-104 with
 // Neumann.hpp / Periodic.hpp; corners compose axis by axis).
 __device__ __forceinline__ int ghost_source(int p, int N, int G, int bc) {

@@ -26,6 +26,9 @@ struct KernelSet {
     void (*axpy)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                  const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
                  double gamma) = nullptr;
+    void (*rk4)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
+                const unsigned long long* ws, int kind, const T* X, const T* Y, const T* Z,
+                const T* W, const T* V, T* out, int check, double gamma) = nullptr;
 };
 
 // Each stage kind (first stage, RK2 second stage, SSP-RK3 second and third
@@ -109,9 +112,18 @@ void launch_axpy(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* 
                        check, gamma);
 }
 
+template <class T, class Eq>
+void launch_rk4(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
+                const unsigned long long* ws, int kind, const T* X, const T* Y, const T* Z,
+                const T* W, const T* V, T* out, int check, double gamma) {
+    hipLaunchKernelGGL((rk4_kernel<T, Eq>), grid, dim3(256), 0, s, desc, state, ws, kind, X, Y,
+                       Z, W, V, out, check, gamma);
+}
+
 template <class T, class Eq, class Flux, int DIM>
 int select_recon(const alsvinn_hip_scheme& sc, KernelSet<T>& k) {
     k.axpy = &launch_axpy<T, Eq>;
+    k.rk4 = &launch_rk4<T, Eq>;
     switch (sc.reconstruction) {
         case ALSVINN_HIP_RECON_NONE: set_stage<T, Eq, ReconNone, Flux, DIM>(k); return 0;
         case ALSVINN_HIP_RECON_WENO2: set_stage<T, Eq, ReconWENO2, Flux, DIM>(k); return 0;
@@ -170,7 +182,9 @@ struct alsvinn_hip_ensemble {
     void* U = nullptr;  // conserved state (the caller's buffer when own_U is false)
     bool own_U = true;
     void* A = nullptr;
-    void* B = nullptr;
+    void* B = nullptr;  // SSP-RK3 U2; RK4 U1
+    void* C = nullptr;  // RK4 U2
+    void* D = nullptr;  // RK4 U3
     alsvinn_hip::SampleDesc* d_desc = nullptr;
     alsvinn_hip::SampleState* d_state = nullptr;
     unsigned long long* d_ws = nullptr;

@@ -85,6 +85,7 @@ struct StageArgs {
     int ntasks;
     int mode;   // StageMode; MODE_RK3_1 forms its stage input as in*c + in2
     int check;  // evaluate the constraint check on the result
+    int no_ws;  // MODE_RHS without the wave-speed reduction (RK4 stages 2-4)
     const T* in;
     const T* in2;  // U0
     T* out;
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             }
         }
 
-        if (MODE == MODE_RHS && !LAZY) {
+        if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             if (in_strip && li == 0 && runnable(*Sp)) {
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             }
         }
 
-        if (MODE == MODE_RHS && !LAZY) {
+        if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             const T mz = segment_max(wsz, seg);
@@ -819,7 +820,7 @@ __global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
             }
         }
 
-        if (MODE == MODE_RHS && !LAZY) {
+        if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 const T m = segment_max(ws[d], seg);

@@ -53,10 +53,14 @@ enum {
     ALSVINN_HIP_RECON_ENO4 = 7,
     ALSVINN_HIP_RECON_WENO3 = 8
 };
+/* Integrators: forward Euler (ForwardEuler.cpp), SSP-RK2 (RungeKutta2.cpp),
+ * SSP-RK3 (RungeKutta3.cpp), classical RK4 (RungeKutta4.cpp:50-96);
+ * IntegratorFactory.cpp:31-44. */
 enum {
     ALSVINN_HIP_INTEGRATOR_FORWARD_EULER = 0,
     ALSVINN_HIP_INTEGRATOR_RK2 = 1,
-    ALSVINN_HIP_INTEGRATOR_RK3 = 2
+    ALSVINN_HIP_INTEGRATOR_RK3 = 2,
+    ALSVINN_HIP_INTEGRATOR_RK4 = 3
 };
 enum { ALSVINN_HIP_BC_NEUMANN = 0, ALSVINN_HIP_BC_PERIODIC = 1 };
 enum { ALSVINN_HIP_FP64 = 0, ALSVINN_HIP_FP32 = 1 };

@@ -26,7 +26,8 @@
 //   CFL + adjusters        alsfvm/src/integrator/Integrator.cpp:20-66,
 //                          alsfvm/src/io/FixedIntervalWriter.cpp:45-53,
 //                          alsuq/src/stats/FixedIntervalStatistics.cpp:33-41
-//   integrators            ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67, RungeKutta3.cpp:49-86
+//   integrators            ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67, RungeKutta3.cpp:49-86,
+//                          RungeKutta4.cpp:50-96
 //   boundary               alsfvm/src/boundary/BoundaryCPU.cpp:30-121, Neumann.hpp:24-35,
 //                          Periodic.hpp:24-42
 //   simulator loop         alsfvm/src/simulator/Simulator.cpp:77-85,180-214
@@ -764,7 +765,7 @@ struct Setup {
     int equation;    // 0 burgers, 1..3 euler<nsd>
     int flux;        // 0 hll, 1 hll3, 2 godunov, 3 central, 4 rusanov, 5 roe
     int recon;       // RECON_*
-    int integrator;  // 0 forward euler, 1 ssp-rk2, 2 ssp-rk3
+    int integrator;  // 0 forward euler, 1 ssp-rk2, 2 ssp-rk3, 3 rk4
     int boundary;    // 0 neumann, 1 periodic
     int nx, ny, nz;
     double origin[3], top[3];
@@ -1001,13 +1002,28 @@ struct Sim {
     }
     ~Sim() { delete sys; }
 
-    // ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67, RungeKutta3.cpp:49-86
+    // ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67, RungeKutta3.cpp:49-86,
+    // RungeKutta4.cpp:50-96
     real substep(int k, real dt) {
         real ws[3] = {0, 0, 0};
         Vol& out = U[k + 1];
         sys->rhs(U[k], ws, out);
         if (k == 0) dt = computeTimestep(ws, dx, s.cfl, t, adj);
         const real c = dt / dx[0];
+        if (s.integrator == 3) {
+            if (k < 3) scaleVol(out, c);
+            if (k < 2) scaleVol(out, 0.5);
+            if (k < 3) addVol(out, U[0]);
+            if (k == 3) {
+                // Volume::addLinearCombination -> HostMemory.cpp:266-279
+                const real a1 = (1.0 / 6.0) * c, a2 = 1.0 / 3.0, a3 = 2.0 / 3.0, a4 = 1.0 / 3.0,
+                           a5 = -1.0 / 3.0;
+                for (size_t i = 0; i < out.buf.size(); ++i)
+                    out.buf[i] = a1 * out.buf[i] + a2 * U[1].buf[i] + a3 * U[2].buf[i] +
+                                 a4 * U[3].buf[i] + a5 * U[0].buf[i];
+            }
+            return dt;
+        }
         scaleVol(out, c);
         addVol(out, U[k]);
         if (s.integrator == 1 && k == 1) {
@@ -1048,7 +1064,7 @@ static bool validSetup(const Setup& s) {
     if (s.equation < 0 || s.equation > 3) return false;
     if (s.recon < 0 || s.recon > 8) return false;
     if (s.recon == RECON_WENOF2 && s.equation == 0) return false;  // Euler only
-    if (s.integrator < 0 || s.integrator > 2) return false;
+    if (s.integrator < 0 || s.integrator > 3) return false;
     if (s.num_adjusters < 0 || s.num_adjusters > 2) return false;
     if (s.nx < 1 || s.ny < 1 || s.nz < 1) return false;
     return true;

@@ -15,7 +15,7 @@ EQ_BURGERS, EQ_EULER1, EQ_EULER2, EQ_EULER3 = 0, 1, 2, 3
 FLUX_HLL, FLUX_HLL3, FLUX_GODUNOV, FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE = 0, 1, 2, 3, 4, 5
 RECON_NONE, RECON_WENO2, RECON_ENO2, RECON_MC, RECON_MM, RECON_WENOF2 = 0, 1, 2, 3, 4, 5
 RECON_ENO3, RECON_ENO4, RECON_WENO3 = 6, 7, 8
-INT_FE, INT_RK2, INT_RK3 = 0, 1, 2
+INT_FE, INT_RK2, INT_RK3, INT_RK4 = 0, 1, 2, 3
 BC_NEUMANN, BC_PERIODIC = 0, 1
 
 

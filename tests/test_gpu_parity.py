@@ -553,3 +553,57 @@ def test_minimal_and_ragged_grids_mixed_end_times(hip, eq, recon, bc, sizes):
         assert rc == 0
         assert infos[i]["steps"] == steps and infos[i]["t"] == t == ends[i], (i, sizes[i])
         _assert_bit_exact(ref, got[i])
+
+
+@pytest.mark.parametrize("case", ["sod", "kh2d", "euler3", "burgers_eno3"])
+def test_rk4_bit_exact(hip, case):
+    """The classical RK4 integrator (RungeKutta4.cpp:50-96): three stage
+    evaluations without the wave-speed reduction, the half and full stage
+    updates and the five-term final combination, on the marching kernels (1D,
+    2D, 3D) and the wide-stencil kernel, against the oracle."""
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    _print_case("rk4 %s" % case)
+    rng = np.random.default_rng(51)
+    steps = 12
+    if case == "sod":
+        eq = O.EQ_EULER1
+        args = (O.FLUX_HLL, O.RECON_NONE, O.INT_RK4, O.BC_NEUMANN, (300, 1, 1), (-1, 0, 0),
+                (1, 0, 0), 0.9, 0.2, ic.sod(300, ghosts=1))
+        steps = None
+    elif case == "kh2d":
+        eq = O.EQ_EULER2
+        X = sampling.stl_mersenne_uniform(1, 40)[0]
+        args = (O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK4, O.BC_PERIODIC, (48, 48, 1), (0, 0, 0),
+                (1, 1, 0), 0.475, 2.0, ic.kelvin_helmholtz_2d(48, X))
+    elif case == "euler3":
+        eq = O.EQ_EULER3
+        n = (14, 10, 9)
+        s = O.make_setup(eq, O.FLUX_HLL3, O.RECON_MC, O.INT_RK4, O.BC_PERIODIC, n, (0, 0, 0),
+                         (1, 1, 1), cfl=0.475, end_time=1.0, max_steps=6)
+        shape = O.volume_shape(s)
+        u0 = np.zeros(shape)
+        u0[0] = 1.0 + 0.2 * rng.random(shape[1:])
+        for k in (1, 2, 3):
+            u0[k] = 0.1 * rng.standard_normal(shape[1:])
+        u0[4] = 3.0 + 0.3 * rng.random(shape[1:])
+        args = (O.FLUX_HLL3, O.RECON_MC, O.INT_RK4, O.BC_PERIODIC, n, (0, 0, 0), (1, 1, 1), 0.475,
+                1.0, u0)
+        steps = 6
+    else:
+        eq = O.EQ_BURGERS
+        n = (40, 30, 1)
+        s = O.make_setup(eq, O.FLUX_GODUNOV, O.RECON_ENO3, O.INT_RK4, O.BC_PERIODIC, n, (0, 0, 0),
+                         (1, 1, 0), cfl=0.475, end_time=1.0, max_steps=12)
+        u0 = rng.standard_normal(O.volume_shape(s))
+        args = (O.FLUX_GODUNOV, O.RECON_ENO3, O.INT_RK4, O.BC_PERIODIC, n, (0, 0, 0), (1, 1, 0),
+                0.475, 1.0, u0)
+    kw = {} if steps is None else {"max_steps": steps}
+    ref, got, (t_ref, steps_ref, dts_ref), (info, dts) = _run_both(hip, eq, *args, **kw)
+    assert info["steps"] == steps_ref
+    if steps is not None:
+        assert steps_ref == steps
+    else:
+        assert info["t"] == t_ref
+    assert np.array_equal(dts, dts_ref)
+    _assert_bit_exact(ref, got)
