@@ -601,7 +601,7 @@ def test_sod_converges_to_exact_riemann_solution():
                                          (O.INT_RK4, 4)])
 def test_integrator_temporal_order(integ, order):
     """IntegratorConvergenceTest.cpp:158-170 (FE/RK2/RK3/RK4 >= 0.9/1.9/2.9/3.9): on a
-    fixed grid, the difference to a run with a 16x smaller CFL number shrinks
+    fixed grid, the difference to a run with a 64x smaller CFL number shrinks
     like cfl^order."""
     from alsvinn_amd import initial_data as ic
     N = 64
