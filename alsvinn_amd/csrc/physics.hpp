@@ -40,13 +40,56 @@ __device__ __forceinline__ T std_max(T a, T b) {
     return (a < b) ? b : a;
 }
 
-// Division and square root. FP64 (the parity path): IEEE correctly rounded,
-// exactly the reference's operations. FP32-state variant: the hardware
-// reciprocal and square root (about 1 ulp each), traded for speed; the
-// variant's error against the FP64 path is measured, not assumed.
-__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+// Division and square root. FP64 (the parity path): the correctly rounded
+// IEEE results, i.e. exactly the reference's operations, computed by gfx950's
+// own IEEE sequences without their range-scaling and special-case steps.
+// The compiler's FP64 division is v_div_scale (x2), v_rcp_f64, two Newton
+// steps on the reciprocal, the quotient, its residual, v_div_fmas and
+// v_div_fixup; for operands that need no scaling (y and x/y normal, |x| >=
+// 2^-969 or x = 0, exponents less than 768 apart) the scale steps return
+// their inputs, v_div_fmas is a plain FMA and v_div_fixup returns the
+// quotient, so the sequence below gives the same bits. The residual is
+// formed as y*q - x and subtracted, which also keeps the sign of a zero
+// quotient of a negative zero (IEEE -0/y = -0). The refined reciprocal
+// depends on y only, so quotients with a common denominator share it (the
+// compiler merges the identical chains). The square root is the compiler's
+// sequence (v_rsq_f64 and Newton corrections) without the rescaling of inputs
+// below 2^-767 and with zero passed through. Every operand of a valid state
+// (positive density and pressure, finite values) is in these ranges; zero,
+// infinite or NaN denominators and non-finite square-root inputs, which only
+// arise in states the constraint check rejects, give NaN here instead of the
+// IEEE special values.
+// FP32-state variant: the hardware reciprocal and square root (about 1 ulp
+// each), traded for speed; the variant's error against the FP64 path is
+// measured, not assumed.
+__device__ __forceinline__ double rcp_refined(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double t = __builtin_fma(-y, r, 1.0);
+    r = __builtin_fma(r, t, r);
+    t = __builtin_fma(-y, r, 1.0);
+    r = __builtin_fma(r, t, r);
+    return r;
+}
+__device__ __forceinline__ double dv(double a, double b) {
+    const double r = rcp_refined(b);
+    const double q = a * r;
+    const double e = __builtin_fma(b, q, -a);
+    return __builtin_fma(-e, r, q);
+}
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ double dsqrt(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double g = x * y0;
+    double h = y0 * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? x : g;
+}
 __device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }

@@ -11,8 +11,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _requests():
+    # a burst: all eight arrive at once and the arenas hold fewer, so the
+    # later ones are admitted while earlier ones are on the GPU
     from alsvinn_amd import serving
-    rs = serving.make_requests(8, rate=200.0, seed=7, t_range=(0.005, 0.02),
+    rs = serving.make_requests(8, rate=0.0, seed=7, t_range=(0.005, 0.02),
                                burgers_n=(2000, 6000), euler_n=(40, 90))
     return rs
 
@@ -39,8 +41,7 @@ def test_serving_results_same_alone_and_in_mix(hip):
     # arena smaller than the offered load: some requests had to queue
     assert any(r.admitted > r.arrival + 1e-4 for r in mix)
     for r in mix:
-        alone = serving.make_requests(8, rate=200.0, seed=7, t_range=(0.005, 0.02),
-                                      burgers_n=(2000, 6000), euler_n=(40, 90))[r.rid]
+        alone = _requests()[r.rid]
         alone.arrival = 0.0
         s1 = serving.Server(_small_config())
         try:
