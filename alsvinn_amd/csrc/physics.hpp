@@ -45,20 +45,22 @@ __device__ __forceinline__ T std_max(T a, T b) {
 // own IEEE sequences without their range-scaling and special-case steps.
 // The compiler's FP64 division is v_div_scale (x2), v_rcp_f64, two Newton
 // steps on the reciprocal, the quotient, its residual, v_div_fmas and
-// v_div_fixup; for operands that need no scaling (y and x/y normal, |x| >=
-// 2^-969 or x = 0, exponents less than 768 apart) the scale steps return
-// their inputs, v_div_fmas is a plain FMA and v_div_fixup returns the
-// quotient, so the sequence below gives the same bits. The residual is
-// formed as y*q - x and subtracted, which also keeps the sign of a zero
-// quotient of a negative zero (IEEE -0/y = -0). The refined reciprocal
-// depends on y only, so quotients with a common denominator share it (the
-// compiler merges the identical chains). The square root is the compiler's
-// sequence (v_rsq_f64 and Newton corrections) without the rescaling of inputs
-// below 2^-767 and with zero passed through. Every operand of a valid state
-// (positive density and pressure, finite values) is in these ranges; zero,
-// infinite or NaN denominators and non-finite square-root inputs, which only
-// arise in states the constraint check rejects, give NaN here instead of the
-// IEEE special values.
+// v_div_fixup. For operands that need no scaling (y and x/y normal, |x| >=
+// 2^-969, exponents less than 768 apart) the scale steps return their
+// inputs, v_div_fmas is a plain FMA and v_div_fixup returns the quotient, so
+// the sequence below gives the same bits. A zero numerator gives the signed
+// zero x*r, the IEEE -0/+0 result: dv() selects it explicitly; dvp(), for
+// denominators that are positive in every valid state (densities, pressures,
+// sums of positive weights), gets it from the residual step itself (the
+// residual is formed as y*q - x and subtracted, which keeps the zero's sign
+// when y > 0). The refined reciprocal depends on y only, so quotients with a
+// common denominator share it (the compiler merges the identical chains).
+// The square root is the compiler's sequence (v_rsq_f64 and Newton
+// corrections) without the rescaling of inputs below 2^-767, with zero passed
+// through. Every operand of a valid state (positive density and pressure,
+// finite values) is in these ranges; zero, infinite or NaN denominators and
+// infinite square-root inputs, which arise only in states the constraint
+// check rejects, give NaN here instead of the IEEE special values.
 // FP32-state variant: the hardware reciprocal and square root (about 1 ulp
 // each), traded for speed; the variant's error against the FP64 path is
 // measured, not assumed.
@@ -70,13 +72,21 @@ __device__ __forceinline__ double rcp_refined(double y) {
     r = __builtin_fma(r, t, r);
     return r;
 }
-__device__ __forceinline__ double dv(double a, double b) {
+__device__ __forceinline__ double dvp(double a, double b) {
     const double r = rcp_refined(b);
     const double q = a * r;
     const double e = __builtin_fma(b, q, -a);
     return __builtin_fma(-e, r, q);
 }
+__device__ __forceinline__ double dv(double a, double b) {
+    const double r = rcp_refined(b);
+    const double q = a * r;
+    const double e = __builtin_fma(b, q, -a);
+    const double x = __builtin_fma(-e, r, q);
+    return a == 0.0 ? q : x;
+}
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float dvp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double dsqrt(double x) {
     const double y0 = __builtin_amdgcn_rsq(x);
     double g = x * y0;
@@ -113,9 +123,9 @@ struct EulerEq {
         T dot = q[1] * q[1];
         if (NSD > 1) dot = dot + q[2] * q[2];
         if (NSD > 2) dot = dot + q[3] * q[3];
-        const T ie = q[nvar - 1] - dv(T(0.5) * dot, q[0]);
+        const T ie = q[nvar - 1] - dvp(T(0.5) * dot, q[0]);
 #pragma unroll
-        for (int k = 0; k < NSD; ++k) a.u[k] = dv(q[1 + k], q[0]);
+        for (int k = 0; k < NSD; ++k) a.u[k] = dvp(q[1 + k], q[0]);
         a.p = (gamma - T(1)) * ie;
     }
     template <int d>
@@ -129,7 +139,7 @@ struct EulerEq {
     // Euler::computeWaveSpeed (Euler.hpp:251-258): |u_d| + sqrt(gamma * p / rho)
     template <int d>
     __device__ __forceinline__ T waveSpeed(const All& a) const {
-        return dabs(a.u[d]) + dsqrt(dv(gamma * a.p, a.q[0]));
+        return dabs(a.u[d]) + dsqrt(dvp(gamma * a.p, a.q[0]));
     }
     // CPUCellComputer::obeysConstraints + Euler::obeysConstraints:
     // every conserved and extra value finite, rho > 0, p > 0.
@@ -181,11 +191,11 @@ struct FluxHLL {
         const T wR = dsqrt(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
-        const T ud = dv(L.u[d] * wL + R.u[d] * wR, wsum);
-        const T p = dv(L.p * wL + R.p * wR, wsum);
-        const T cs = dsqrt(dv(g * p, rho));
-        const T sL = ref_fmin(L.u[d] - dsqrt(dv(g * L.p, L.q[0])), ud - cs);
-        const T sR = ref_fmax(R.u[d] + dsqrt(dv(g * R.p, R.q[0])), ud + cs);
+        const T ud = dvp(L.u[d] * wL + R.u[d] * wR, wsum);
+        const T p = dvp(L.p * wL + R.p * wR, wsum);
+        const T cs = dsqrt(dvp(g * p, rho));
+        const T sL = ref_fmin(L.u[d] - dsqrt(dvp(g * L.p, L.q[0])), ud - cs);
+        const T sR = ref_fmax(R.u[d] + dsqrt(dvp(g * R.p, R.q[0])), ud + cs);
         if (sL == T(0)) {
 #pragma unroll
             for (int i = 0; i < nv; ++i) F[i] = T(0);
@@ -218,12 +228,12 @@ struct FluxHLL3 {
         const T wR = dsqrt(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
-        const T ud = dv(L.u[d] * wL + R.u[d] * wR, wsum);
-        const T p = dv(wL * L.p + wR * R.p, wsum);
-        const T cfL = dsqrt(dv(g * L.p, L.q[0]));
-        const T cfR = dsqrt(dv(g * R.p, R.q[0]));
+        const T ud = dvp(L.u[d] * wL + R.u[d] * wR, wsum);
+        const T p = dvp(wL * L.p + wR * R.p, wsum);
+        const T cfL = dsqrt(dvp(g * L.p, L.q[0]));
+        const T cfR = dsqrt(dvp(g * R.p, R.q[0]));
         const T corr = T(0.5) * ref_fmax(T(0), L.u[d] - R.u[d]);
-        const T cs = dsqrt(dv(g * p, rho));
+        const T cs = dsqrt(dvp(g * p, rho));
         const T sL = ref_fmin(L.u[d] + corr - cfL, ud - cs);
         const T sR = ref_fmax(R.u[d] - corr + cfR, ud + cs);
         if (sL == T(0)) {
@@ -375,14 +385,14 @@ struct ReconWENO2 {
         const T D1 = T(3) * e1 * e1;
         // 2/D == 2*(1/D) exactly while 1/D is a normal number (D < 2^1021,
         // i.e. |w_{i+1}-w_i| < 2^510), so two of the four divisions are shared.
-        const T a0L = dv(T(1), D0);
-        const T a1R = dv(T(1), D1);
+        const T a0L = dvp(T(1), D0);
+        const T a1R = dvp(T(1), D1);
         const T a1L = T(2) * a1R;
         const T a0R = T(2) * a0L;
-        const T w0L = dv(a0L, a0L + a1L);
-        const T w1L = dv(a1L, a0L + a1L);
-        const T w0R = dv(a0R, a0R + a1R);
-        const T w1R = dv(a1R, a0R + a1R);
+        const T w0L = dvp(a0L, a0L + a1L);
+        const T w1L = dvp(a1L, a0L + a1L);
+        const T w0R = dvp(a0R, a0R + a1R);
+        const T w1R = dvp(a1R, a0R + a1R);
         const T cL = T(3) * w0L + w1L;
         const T cR = T(3) * w1R + w0R;
 #pragma unroll
@@ -411,8 +421,8 @@ struct ReconWENOF2 {
         if (NSD > 2) dot = dot + q[3] * q[3];
         P[0] = q[0];
 #pragma unroll
-        for (int k = 0; k < NSD; ++k) P[1 + k] = dv(q[1 + k], q[0]);
-        P[NV - 1] = (gamma - T(1)) * (q[NV - 1] - dv(T(0.5) * dot, q[0]));
+        for (int k = 0; k < NSD; ++k) P[1 + k] = dvp(q[1 + k], q[0]);
+        P[NV - 1] = (gamma - T(1)) * (q[NV - 1] - dvp(T(0.5) * dot, q[0]));
     }
     template <class T, int NV>
     __device__ __forceinline__ static T dotu(const T* a, const T* b) {
@@ -432,15 +442,15 @@ struct ReconWENOF2 {
         dP[NV - 1] = ref_fmax(-BOUND * P0[NV - 1], ref_fmin(BOUND * P0[NV - 1], dP[NV - 1]));
         const T duu = dotu<T, NV>(dP, dP);
         const T LL = (T(0.125) * P0[0] * duu - T(0.5) * ref_fmin(T(0), dP[0] * dotu<T, NV>(P0, dP))) +
-                     dv(T(0.5) * dP[0] * dP[0] * duu, P0[0]);
-        const T a = T(0.5) * dsqrt(dv(Rr, ref_fmax(Rr, LL)));
+                     dvp(T(0.5) * dP[0] * dP[0] * duu, P0[0]);
+        const T a = T(0.5) * dsqrt(dvp(Rr, ref_fmax(Rr, LL)));
         T P[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) P[v] = sgn > T(0) ? P0[v] + a * dP[v] : P0[v] - a * dP[v];
         out[0] = P[0];
 #pragma unroll
         for (int k = 1; k < NV - 1; ++k) out[k] = P[k] * P[0];
-        out[NV - 1] = dv(P[NV - 1], gamma - T(1)) + T(0.5) * P[0] * dotu<T, NV>(P, P);
+        out[NV - 1] = dvp(P[NV - 1], gamma - T(1)) + T(0.5) * P[0] * dotu<T, NV>(P, P);
     }
     template <class T, int NV>
     __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
@@ -454,14 +464,14 @@ struct ReconWENOF2 {
         const T D0 = T(3) * e0 * e0;
         const T D1 = T(3) * e1 * e1;
         // 2/D == 2*(1/D) exactly while 1/D is a normal number (as in WENO2)
-        const T a0L = dv(T(1), D0);
-        const T a1R = dv(T(1), D1);
+        const T a0L = dvp(T(1), D0);
+        const T a1R = dvp(T(1), D1);
         const T a1L = T(2) * a1R;
         const T a0R = T(2) * a0L;
-        const T w0L = dv(a0L, a0L + a1L);
-        const T w1L = dv(a1L, a0L + a1L);
-        const T w0R = dv(a0R, a0R + a1R);
-        const T w1R = dv(a1R, a0R + a1R);
+        const T w0L = dvp(a0L, a0L + a1L);
+        const T w1L = dvp(a1L, a0L + a1L);
+        const T w0R = dvp(a0R, a0R + a1R);
+        const T w1R = dvp(a1R, a0R + a1R);
         T Pm[NV], P0[NV], Pp[NV];
         prim<T, NV>(qm, Pm, gamma);
         prim<T, NV>(q0, P0, gamma);
@@ -472,7 +482,7 @@ struct ReconWENOF2 {
             dL[v] = w1L * (P0[v] - Pm[v]) + w0L * (Pp[v] - P0[v]);
             dR[v] = w1R * (P0[v] - Pm[v]) + w0R * (Pp[v] - P0[v]);
         }
-        const T Rr = dv(P0[NV - 1], gamma - T(1));
+        const T Rr = dvp(P0[NV - 1], gamma - T(1));
         side<T, NV>(P0, dL, Rr, T(-1), gamma, L);
         side<T, NV>(P0, dR, Rr, T(1), gamma, R);
     }
@@ -613,16 +623,16 @@ struct ReconWENO3 {
         for (int i = 2; i >= 0; --i) {
             const T e = beta[i] + eps;
             const T e2 = e * e;
-            aR[i] = dv(lin[i], e2);
+            aR[i] = dvp(lin[i], e2);
             sR += aR[i];
-            aL[i] = dv(lin[2 - i], e2);
+            aL[i] = dvp(lin[2 - i], e2);
             sL += aL[i];
         }
         T wL[3], wR[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            wL[i] = dv(aL[i], sL);
-            wR[i] = dv(aR[i], sR);
+            wL[i] = dvp(aL[i], sL);
+            wR[i] = dvp(aR[i], sR);
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
