@@ -313,7 +313,7 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     a.out = A;
     auto stage = [&]() {
         if (tm) tm->begin();
-        k.stage[e->variant](a, blocks, e->stream);
+        k.stage[e->variant[stage_kind(a.mode)]](a, blocks, e->stream);
         if (tm) tm->end(0);
     };
     auto axpy = [&](const T* X, const T* Y, T* O, int check) {
@@ -427,13 +427,14 @@ int upload_tasks(alsvinn_hip_ensemble* e) {
 // wave speeds on the current state; writes only the scratch buffer and the
 // wave-speed slots, which are cleared afterwards).
 template <class T>
-double time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps) {
+void time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps, double ms[3]) {
     // A dry run of the step's stage kernels on the live state: every output
     // goes to the scratch volumes A and B, the constraint check is off and the
     // wave-speed slots are cleared by the caller afterwards, so U and the
-    // samples' state are untouched. SSP-RK3 times all three stages (the
-    // second and third read two volumes and use more registers than the
-    // first); the other integrators time the first stage.
+    // samples' state are untouched. SSP-RK3 times each of its three stages
+    // (the second and third read two volumes and use more registers than the
+    // first); the other integrators time the first stage. ms[kind] receives
+    // the average time of each stage kind (0 where not timed).
     StageArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.desc = e->d_desc;
@@ -447,32 +448,42 @@ double time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps) {
     T* A = (T*)e->A;
     T* B = (T*)e->B;
     const bool rk3 = e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK3;
-    auto launch_all = [&]() {
+    hipEvent_t ev[4];
+    for (auto& x : ev) hipEventCreate(&x);
+    auto launch_all = [&](bool timed) {
+        if (timed) hipEventRecord(ev[0], e->stream);
         a.mode = MODE_RHS;
         a.in = U;
         a.in2 = U;
         a.out = A;
-        k.stage[e->variant](a, blocks, e->stream);
+        k.stage[e->variant[0]](a, blocks, e->stream);
+        if (timed) hipEventRecord(ev[1], e->stream);
         if (!rk3) return;
         a.mode = MODE_RK3_1;
         a.in = A;
         a.in2 = U;
         a.out = B;
-        k.stage[e->variant](a, blocks, e->stream);
+        k.stage[e->variant[1]](a, blocks, e->stream);
+        if (timed) hipEventRecord(ev[2], e->stream);
         a.mode = MODE_RK3_2;
         a.in = B;
         a.in2 = U;
         a.out = A;
-        k.stage[e->variant](a, blocks, e->stream);
+        k.stage[e->variant[2]](a, blocks, e->stream);
+        if (timed) hipEventRecord(ev[3], e->stream);
     };
-    launch_all();  // warm-up
-    hipEventRecord(e->ev0, e->stream);
-    for (int i = 0; i < reps; ++i) launch_all();
-    hipEventRecord(e->ev1, e->stream);
-    hipEventSynchronize(e->ev1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, e->ev0, e->ev1);
-    return ms / reps;
+    launch_all(false);  // warm-up
+    ms[0] = ms[1] = ms[2] = 0;
+    for (int i = 0; i < reps; ++i) {
+        launch_all(true);
+        hipEventSynchronize(rk3 ? ev[3] : ev[1]);
+        for (int s = 0; s < (rk3 ? 3 : 1); ++s) {
+            float t = 0;
+            hipEventElapsedTime(&t, ev[s], ev[s + 1]);
+            ms[s] += t / reps;
+        }
+    }
+    for (auto x : ev) hipEventDestroy(x);
 }
 
 }  // namespace
@@ -815,16 +826,22 @@ int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int var
     HIP_TRY(hipStreamSynchronize(e->stream));
     destroy_graph(e);
     e->band_rows = rows;
-    e->variant = variant;
+    e->variant[0] = e->variant[1] = e->variant[2] = variant;
     e->tuned = true;
     return upload_tasks(e);
+}
+
+int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* variants) {
+    if (!e || !variants) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    for (int s = 0; s < 3; ++s) variants[s] = e->variant[s];
+    return 0;
 }
 
 int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
                                       int* ntasks) {
     if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");
     if (rows) *rows = e->band_rows;
-    if (variant) *variant = e->variant;
+    if (variant) *variant = e->variant[0];
     if (ntasks) *ntasks = e->ntasks;
     return 0;
 }
@@ -844,32 +861,55 @@ int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* varia
     } else {
         cand.push_back(1);
     }
-    double best = 1e300;
-    int best_rows = cand[0], best_var = 0;
-    const int reps = 2;
-    for (int v = 0; v < kStageVariants; ++v) {
-        for (int r : cand) {
-            e->variant = v;
-            e->band_rows = r;
+    // Each (band height, budget) pair is timed twice, the budgets in opposite
+    // orders, so that the clock's drift under sustained load does not favour
+    // the configurations timed first; every stage kind keeps its own fastest
+    // budget, and the band height with the smallest total wins.
+    const int nc = (int)cand.size();
+    std::vector<double> t((size_t)nc * kStageVariants * 3, 0.0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < nc; ++c) {
+            e->band_rows = cand[c];
             if (int rc = upload_tasks(e)) return rc;
-            const double ms = e->esize == 8 ? time_stages<double>(e, e->k64, reps)
-                                            : time_stages<float>(e, e->k32, reps);
-            if (ms < best) {
-                best = ms;
-                best_rows = r;
-                best_var = v;
+            for (int i = 0; i < kStageVariants; ++i) {
+                const int v = pass == 0 ? i : kStageVariants - 1 - i;
+                e->variant[0] = e->variant[1] = e->variant[2] = v;
+                double ms[3];
+                if (e->esize == 8)
+                    time_stages<double>(e, e->k64, 1, ms);
+                else
+                    time_stages<float>(e, e->k32, 1, ms);
+                for (int s = 0; s < 3; ++s) t[((size_t)c * kStageVariants + v) * 3 + s] += ms[s];
             }
+        }
+    double best = 1e300;
+    int best_rows = cand[0], best_var[3] = {0, 0, 0};
+    for (int c = 0; c < nc; ++c) {
+        double total = 0;
+        int vs[3];
+        for (int s = 0; s < 3; ++s) {
+            vs[s] = 0;
+            for (int v = 1; v < kStageVariants; ++v)
+                if (t[((size_t)c * kStageVariants + v) * 3 + s] <
+                    t[((size_t)c * kStageVariants + vs[s]) * 3 + s])
+                    vs[s] = v;
+            total += t[((size_t)c * kStageVariants + vs[s]) * 3 + s];
+        }
+        if (total < best) {
+            best = total;
+            best_rows = cand[c];
+            for (int s = 0; s < 3; ++s) best_var[s] = vs[s];
         }
     }
     std::vector<unsigned long long> z(3 * (size_t)e->nsamples, 0ull);
     HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
-    e->variant = best_var;
+    for (int s = 0; s < 3; ++s) e->variant[s] = best_var[s];
     e->band_rows = best_rows;
     e->tuned = true;
     if (int rc = upload_tasks(e)) return rc;
     HIP_TRY(hipGetLastError());
     if (rows) *rows = best_rows;
-    if (variant) *variant = best_var;
+    if (variant) *variant = best_var[0];
     return 0;
 }
 
@@ -1042,7 +1082,7 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
         a.in = (double*)e->U;
         a.in2 = (double*)e->U;
         a.out = (double*)e->A;
-        e->k64.stage[e->variant](a, blocks, e->stream);
+        e->k64.stage[e->variant[0]](a, blocks, e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (host_out) HIP_TRY(hipMemcpy(host_out, buf_at<double>(e->A, D.base), n * 8, hipMemcpyDeviceToHost));
@@ -1059,7 +1099,7 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
         a.in = (float*)e->U;
         a.in2 = (float*)e->U;
         a.out = (float*)e->A;
-        e->k32.stage[e->variant](a, blocks, e->stream);
+        e->k32.stage[e->variant[0]](a, blocks, e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (host_out) {

@@ -19,6 +19,8 @@ int fail(int code, const std::string& msg);
 constexpr int kStageVariants = 3;  // register budgets for 1, 2, 3 waves per SIMD
 constexpr int kDefaultVariant = 1;
 inline int variant_waves(int v) { return 1 + v; }
+// Stage kind of a stage mode, for the per-kind register budget.
+inline int stage_kind(int mode) { return mode == MODE_RK3_1 ? 1 : (mode == MODE_RK3_2 ? 2 : 0); }
 
 template <class T>
 struct KernelSet {
@@ -201,7 +203,10 @@ struct alsvinn_hip_ensemble {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     long long max_vstride = 0;
-    int variant = alsvinn_hip::kDefaultVariant;  // stage kernel register budget
+    // stage kernel register budget per stage kind: [0] first stage (and the
+    // RK2 / RK4 evaluations), [1] second and [2] third SSP-RK3 stage
+    int variant[3] = {alsvinn_hip::kDefaultVariant, alsvinn_hip::kDefaultVariant,
+                      alsvinn_hip::kDefaultVariant};
     int band_rows = 0;  // rows per wave task; 0 = heuristic
     bool tuned = false;
     alsvinn_hip::KernelSet<double> k64;

@@ -178,11 +178,17 @@ int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int var
 int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
                                       int* ntasks);
 
-/* Autotuner: times a dry run of the step's stage kernels (all three for
- * SSP-RK3, the first otherwise; outputs into scratch volumes) on the current
- * state for every variant and a ladder of band heights, keeps the fastest
- * schedule. Synchronous; leaves the conserved state untouched. */
+/* Autotuner: times a dry run of the step's stage kernels (each of the three
+ * SSP-RK3 stages, the first stage otherwise; outputs into scratch volumes) on
+ * the current state for every variant and a ladder of band heights, twice in
+ * opposite orders; keeps the band height with the smallest total and, per
+ * stage kind, the fastest variant (*variant: the first stage's).
+ * Synchronous; leaves the conserved state untouched. set_schedule sets one
+ * variant for every stage kind. */
 int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* variant);
+/* The register-budget variant of each stage kind: [0] first stage (and the
+ * RK2/RK4 evaluations), [1] second and [2] third SSP-RK3 stage. */
+int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* variants);
 
 /* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
  * every sample that has not reached its end time. Asynchronous. */
