@@ -34,6 +34,19 @@ __device__ __forceinline__ T ref_fmax(T x, T y) {
     if (y != y) return x;
     return (x > y) ? x : y;
 }
+// fmin/fmax by the hardware (v_min_f64 / v_max_f64: IEEE minNum/maxNum, the
+// other operand when one is a quiet NaN). They agree with glibc's fmin/fmax
+// except possibly for the pair (+0, -0), where glibc returns its second
+// operand; used only where such a pair cannot occur or its sign cannot
+// reach the result (see the call sites).
+template <class T>
+__device__ __forceinline__ T hw_fmin(T x, T y) {
+    return __builtin_fmin(x, y);
+}
+template <class T>
+__device__ __forceinline__ T hw_fmax(T x, T y) {
+    return __builtin_fmax(x, y);
+}
 // std::max(a, b) = (a < b) ? b : a (keeps a when b is NaN).
 template <class T>
 __device__ __forceinline__ T std_max(T a, T b) {
@@ -232,10 +245,14 @@ struct FluxHLL3 {
         const T p = dvp(wL * L.p + wR * R.p, wsum);
         const T cfL = dsqrt(dvp(g * L.p, L.q[0]));
         const T cfR = dsqrt(dvp(g * R.p, R.q[0]));
-        const T corr = T(0.5) * ref_fmax(T(0), L.u[d] - R.u[d]);
+        // hw_fmin/hw_fmax: corr is a zero of either sign only when uL == uR
+        // or for uL = -0, uR = +0, and uL + corr, uR - corr then lose the
+        // zero's sign against cfL, cfR > 0; sL, sR compare two differences,
+        // each +0 when it vanishes (round to nearest), never a (+0, -0) pair
+        const T corr = T(0.5) * hw_fmax(T(0), L.u[d] - R.u[d]);
         const T cs = dsqrt(dvp(g * p, rho));
-        const T sL = ref_fmin(L.u[d] + corr - cfL, ud - cs);
-        const T sR = ref_fmax(R.u[d] - corr + cfR, ud + cs);
+        const T sL = hw_fmin(L.u[d] + corr - cfL, ud - cs);
+        const T sR = hw_fmax(R.u[d] - corr + cfR, ud + cs);
         if (sL == T(0)) {
 #pragma unroll
             for (int i = 0; i < nv; ++i) F[i] = T(0);
@@ -284,7 +301,7 @@ struct FluxHLL3 {
                 F[i] = fX + (mid[i] - qX) * sX;
             }
         }
-        return ref_fmax(dabs(sL), dabs(sR));
+        return hw_fmax(dabs(sL), dabs(sR));  // non-negative operands
     }
 };
 
