@@ -204,6 +204,50 @@ def test_full_size_kh_conserves_totals(hip):
     assert np.all(np.abs(t1 - t0) <= 1e-12 * scale * n * n / 1000)
 
 
+@pytest.mark.parametrize("case", ["kh2d_256x1024_fp64", "kh2d_1024_single_fp64",
+                                  "kh3d_256cubed_x64_fp32"])
+def test_largest_configurations_conserve_totals(hip, case):
+    """The benchmark configurations at full size: 1024 samples of 256^2
+    (FP64), one 1024^2 solve (FP64) and 64 samples of 256^3 (FP32 state,
+    FP64 totals), periodic: a few steps later the conserved totals of the
+    first and last samples are unchanged to roundoff and no sample fails."""
+    _case("largest " + case)
+    from alsvinn_amd import initial_data as ic
+    from alsvinn_amd import sampling
+    X = sampling.stl_mersenne_uniform(1, 40)[0]
+    if case == "kh2d_256x1024_fp64":
+        M, shape, prec, steps, tol = 1024, (256, 256, 1), hip.FP64, 6, 1e-12
+        u0 = ic.kelvin_helmholtz_2d(256, X)
+        eq = O.EQ_EULER2
+    elif case == "kh2d_1024_single_fp64":
+        M, shape, prec, steps, tol = 1, (1024, 1024, 1), hip.FP64, 12, 1e-12
+        u0 = ic.kelvin_helmholtz_2d(1024, X)
+        eq = O.EQ_EULER2
+    else:
+        M, shape, prec, steps, tol = 64, (256, 256, 256), hip.FP32, 3, 1e-6
+        u0 = ic.kelvin_helmholtz_3d(256, sampling.stl_mersenne_uniform(1, 400)[0])
+        eq = O.EQ_EULER3
+    top = (1, 1, 1 if shape[2] > 1 else 0)
+    scheme = hip.make_scheme(eq, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
+                             precision=prec)
+    samples = [hip.make_sample(shape, (0, 0, 0), top, 2.0, max_steps=steps) for _ in range(M)]
+    with hip.Ensemble(scheme, samples) as e:
+        for i in range(M):
+            e.upload(i, u0)
+        e.apply_boundary()
+        t0 = e.totals()
+        e.run(steps)
+        t1 = e.totals()
+        infos = [e.info(i) for i in (0, M - 1)]
+    assert all(inf["steps"] == steps and inf["status"] == 0 for inf in infos)
+    # FP64: sums of per-cell roundings (the same bound as the 256^2 x 64 test
+    # above); FP32 state: relative 1e-6 of the largest total
+    cells = shape[0] * shape[1] * shape[2]
+    scale = np.abs(t0).max(axis=0) + 1.0
+    bound = tol * scale * (cells / 1000 if prec == hip.FP64 else 1.0)
+    assert np.all(np.abs(t1 - t0)[[0, M - 1]] <= bound)
+
+
 def test_full_time_uq_run_bit_exact(hip):
     """The whole UQ run to T = 2 (reference sampler, KH WENO2+HLL3 SSP-RK3,
     periodic) on 48^2 x 4 samples: every final state, the step counts, meanvar

@@ -137,3 +137,18 @@ def test_poisson_request_stream():
             assert 4096 <= r.n <= 65536 and 0.25 <= r.hurst <= 0.75
         else:
             assert 256 <= r.n <= 1024 and r.states.shape == (4, 4)
+
+
+def test_empty_ensemble_and_pool_rejected_before_any_gpu_call():
+    """An ensemble with no samples, or a pool with no slots, is refused by
+    argument checks that run before any HIP call (this test runs without a
+    GPU)."""
+    import alsvinn_amd as hip
+    s = hip.make_scheme(hip.EQ_EULER2, hip.FLUX_HLL3, hip.RECON_WENO2, hip.INT_RK3,
+                        hip.BC_PERIODIC)
+    with pytest.raises(hip.HipError, match="bad ensemble arguments"):
+        hip.Ensemble(s, [])
+    L = hip._lib.load()
+    out = ctypes.c_void_p()
+    assert L.alsvinn_hip_pool_create(ctypes.byref(s), 0, 2, 1000, None, ctypes.byref(out)) != 0
+    assert b"bad pool arguments" in L.alsvinn_hip_last_error()
