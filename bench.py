@@ -462,10 +462,16 @@ def main():
     traffic = None
     prof = os.path.join(ROOT, "profiles", "stage_kernel_kh256x1024_fp64.md")
     if args.precision == "fp64" and M == 1024 and n == 256 and os.path.exists(prof):
-        # HBM bytes per stage launch from the calibrated FETCH_SIZE/WRITE_SIZE runs
+        # HBM bytes per stage launch from the calibrated FETCH_SIZE/WRITE_SIZE
+        # runs, for the band height the autotuner chose when it was profiled
+        by_rows = {}
         for ln in open(prof):
-            if ln.startswith("traffic_bytes_per_launch:"):
+            if ln.startswith("traffic_bytes_per_launch_rows"):
+                key, val = ln.split(":")
+                by_rows[int(key[len("traffic_bytes_per_launch_rows"):])] = float(val)
+            elif ln.startswith("traffic_bytes_per_launch:"):
                 traffic = float(ln.split(":")[1])
+        traffic = by_rows.get(sched.get("rows"), traffic)
     line = {
         "metric": "cell-updates/s + % HBM peak, batched vs plain (1 GPU); p50/p99 UQ-request latency",
         "value": value,
