@@ -115,13 +115,18 @@ __device__ __forceinline__ double sample_dt(const SampleDesc& D, const SampleSta
 }
 
 // Lane i receives the value of lane i-1 (lane_from_lower) or lane i+1
-// (lane_from_upper); the lane without a source keeps its own value. DPP
-// wave_shr:1 / wave_shl:1 moves, two 32-bit halves for a double.
+// (lane_from_upper); the lane without a source (lane 0, resp. 63) receives
+// 0. DPP wave_shr:1 / wave_shl:1 moves, two 32-bit halves for a double. Lanes
+// 0 and 63 are always halo lanes of their segment (every segment carries
+// ghost-width halo lanes on both sides), so no face or cell that is written
+// or reduced reads the value they receive; writing 0 there (bound_ctrl)
+// instead of keeping the lane's own value lets the move write a fresh
+// register without first copying its source into it.
 __device__ __forceinline__ int dpp_shr1(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int dpp_shl1(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ double lane_from_lower(double v) {
     return __hiloint2double(dpp_shr1(__double2hiint(v)), dpp_shr1(__double2loint(v)));
