@@ -62,7 +62,8 @@ __device__ __forceinline__ T std_max(T a, T b) {
 // 2^-969, exponents less than 768 apart) the scale steps return their
 // inputs, v_div_fmas is a plain FMA and v_div_fixup returns the quotient, so
 // the sequence below gives the same bits. A zero numerator gives the signed
-// zero x*r, the IEEE -0/+0 result: dv() selects it explicitly; dvp(), for
+// zero x*r, the IEEE -0/+0 result: dv() gives its result the sign of x*r
+// (the sign of every quotient, zero or not); dvp(), for
 // denominators that are positive in every valid state (densities, pressures,
 // sums of positive weights), gets it from the residual step itself (the
 // residual is formed as y*q - x and subtracted, which keeps the zero's sign
@@ -96,7 +97,9 @@ __device__ __forceinline__ double dv(double a, double b) {
     const double q = a * r;
     const double e = __builtin_fma(b, q, -a);
     const double x = __builtin_fma(-e, r, q);
-    return a == 0.0 ? q : x;
+    // sign(q) = sign(a) xor sign(b), the quotient's sign also when it is
+    // zero; for a nonzero quotient x has it already
+    return __builtin_copysign(x, q);
 }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float dvp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -113,7 +116,24 @@ __device__ __forceinline__ double dsqrt(double x) {
     g = __builtin_fma(d, h, g);
     return x == 0.0 ? x : g;
 }
+// The same sequence without the zero pass-through, for the square roots of
+// face densities: a zero density is also the divisor of the face's velocity,
+// so its flux is not finite either way (zero denominators give NaN here, as
+// above).
+__device__ __forceinline__ double dsqrt_rho(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double g = x * y0;
+    double h = y0 * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
 __device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float dsqrt_rho(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
@@ -200,8 +220,8 @@ struct FluxHLL {
                                                 const typename EulerEq<T, NSD>::All& R, T* F) {
         constexpr int nv = NSD + 2;
         const T g = eq.gamma;
-        const T wL = dsqrt(L.q[0]);
-        const T wR = dsqrt(R.q[0]);
+        const T wL = dsqrt_rho(L.q[0]);
+        const T wR = dsqrt_rho(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
         const T ud = dvp(L.u[d] * wL + R.u[d] * wR, wsum);
@@ -237,8 +257,8 @@ struct FluxHLL3 {
                                                 const typename EulerEq<T, NSD>::All& R, T* F) {
         constexpr int nv = NSD + 2;
         const T g = eq.gamma;
-        const T wL = dsqrt(L.q[0]);
-        const T wR = dsqrt(R.q[0]);
+        const T wL = dsqrt_rho(L.q[0]);
+        const T wR = dsqrt_rho(R.q[0]);
         const T rho = (L.q[0] + R.q[0]) / T(2);
         const T wsum = wL + wR;
         const T ud = dvp(L.u[d] * wL + R.u[d] * wR, wsum);
@@ -280,13 +300,24 @@ struct FluxHLL3 {
             }
             // The reference evaluates the two branches with the same formula
             // on mirrored operands; select the operands, then evaluate once.
+            // The side's point flux is evaluated on the selected state (the
+            // same function of the same values as selecting fL or fR; only
+            // the transverse momentum fluxes of both sides feed us[] above).
             const bool left = sm >= T(0);
+            typename EulerEq<T, NSD>::All X;
+#pragma unroll
+            for (int i = 0; i < nv; ++i) X.q[i] = left ? L.q[i] : R.q[i];
+#pragma unroll
+            for (int k = 0; k < NSD; ++k) X.u[k] = left ? L.u[k] : R.u[k];
+            X.p = left ? L.p : R.p;
+            T fX[nv];
+            eq.template pointFlux<d>(X, fX);
             const T sX = left ? sL : sR;
             const T udx = left ? udl : udr;
-            const T rhoX = left ? L.q[0] : R.q[0];
-            const T pX = left ? L.p : R.p;
-            const T uX = left ? L.u[d] : R.u[d];
-            const T EX = left ? L.q[nv - 1] : R.q[nv - 1];
+            const T rhoX = X.q[0];
+            const T pX = X.p;
+            const T uX = X.u[d];
+            const T EX = X.q[nv - 1];
             T mid[nv];
             const T smx = sm - sX;
             mid[0] = dv(rhoX * udx, smx);
@@ -295,11 +326,7 @@ struct FluxHLL3 {
             const T ps = pX + rhoX * (uX - sm) * udx;
             mid[nv - 1] = dv(udx * EX + pX * uX - ps * sm, smx);
 #pragma unroll
-            for (int i = 0; i < nv; ++i) {
-                const T fX = left ? fL[i] : fR[i];
-                const T qX = left ? L.q[i] : R.q[i];
-                F[i] = fX + (mid[i] - qX) * sX;
-            }
+            for (int i = 0; i < nv; ++i) F[i] = fX[i] + (mid[i] - X.q[i]) * sX;
         }
         return hw_fmax(dabs(sL), dabs(sR));  // non-negative operands
     }

@@ -46,7 +46,7 @@ def _dvp(a, b, est):
 
 
 def _div(a, b, est):
-    """dv(): either sign; a zero numerator returns the signed zero a*r."""
+    """dv(): either sign; the result takes the sign of a*r."""
     r = est
     t = _fma(-b, r, 1.0)
     r = _fma(r, t, r)
@@ -55,7 +55,7 @@ def _div(a, b, est):
     q = a * r
     e = _fma(b, q, -a)
     x = _fma(-e, r, q)
-    return q if a == 0.0 else x
+    return math.copysign(x, q)
 
 
 def _sqrt(x, est):
