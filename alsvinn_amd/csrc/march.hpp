@@ -172,17 +172,13 @@ __device__ __forceinline__ void load_cell(const T* __restrict__ in, const T* __r
 // forms the lazy combination U1 = L0*c + U0, so that the work in between
 // hides the loads' latency (forming it at the load makes the wave wait for
 // both loads at once: +12% for the second SSP-RK3 stage in 2D, +40-80% in 3D).
-// STREAM marks the first operand's loads non-temporal: read once per
-// kernel, they should not displace lines that are read again (the lazy
-// stage's U0, reloaded two rows later for the stage combination).
-template <class T, int NV, bool STREAM = false>
+template <class T, int NV>
 struct PendingCell {
     T a[NV], b[NV];
     __device__ __forceinline__ void issue(const T* __restrict__ in, const T* __restrict__ in2,
                                           bool lazy, long long i, long long vs) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            a[v] = STREAM ? __builtin_nontemporal_load(in + i + v * vs) : in[i + v * vs];
+        for (int v = 0; v < NV; ++v) a[v] = in[i + v * vs];
         if (lazy) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) b[v] = in2[i + v * vs];
@@ -249,7 +245,7 @@ __device__ __forceinline__ void axis_images(int c, int N, int G, int bc, int& a,
 
 // Writes one result cell to its padded position and to every ghost cell
 // that mirrors it (at most one image per axis side).
-template <class T, int NV, int DIM, bool STREAM = false>
+template <class T, int NV, int DIM>
 __device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDesc& D, int col,
                                            int row, int plane, const T* res) {
     int x[3], y[3], z[3];
@@ -271,12 +267,7 @@ __device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDes
                 if (x[ix] < 0) continue;
                 const long long o = ((long long)z[iz] * D.ty + y[iy]) * D.tx + x[ix];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    if (STREAM)
-                        __builtin_nontemporal_store(res[v], out + o + v * vs);
-                    else
-                        out[o + v * vs] = res[v];
-                }
+                for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
             }
         }
     }
@@ -360,7 +351,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             const int pr = r + gy;
             T qn[NV];
             // next row first: its latency overlaps the x sweep
-            PendingCell<T, NV, true> pn;
+            PendingCell<T, NV> pn;
             if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2), vs);
             // U^n of this row for the stage combination, also issued before
             // the sweep (loaded after it, its latency would stall the wave)
@@ -368,7 +359,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             T u0[NV];
             if (MODE != MODE_RHS && interior) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = __builtin_nontemporal_load(in2 + i0 + v * vs);
+                for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
             }
             T out_v[NV];
             // ---- x direction on row r (first: out = 0 + x divergence)
@@ -413,7 +404,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 T res[NV];
                 combine<T, NV>(MODE, c, out_v, yc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, DIM, true>(out, vs, D, col, r, 0, res);
+                store_cell<T, NV, DIM>(out, vs, D, col, r, 0, res);
             }
             if constexpr (DIM >= 2) {
 #pragma unroll
