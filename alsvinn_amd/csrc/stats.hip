@@ -206,6 +206,87 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict
     }
 }
 
+// 2D structure_cube on LDS tiles: a block of 256 threads owns a 16 x 16 tile
+// of cells of one sample (blockIdx.z) and variable (blockIdx.y) and first
+// stages the tile with a halo of R = H-1 cells on every side (the boundary's
+// wrap or clamp applied as it is loaded), so each of a cell's 8h shell
+// neighbours is an LDS read. Every thread sums its cell's shell terms in the
+// same order as structure_cube_kernel; the block total of each h (wave sums,
+// then the four waves in order) goes to partial[((k*nvar + var)*ntiles +
+// tile)*H + h], tiles numbered row-major.
+constexpr int kTile = 16;
+
+__host__ __device__ constexpr int tile_width(int H) { return kTile + 2 * (H - 1); }
+
+// |x|^PK for a positive integer PK, as pow_p evaluates it.
+template <int PK>
+__device__ __forceinline__ double pow_int(double x) {
+    if constexpr (PK == 1) return x;
+    if constexpr (PK == 2) return x * x;
+    if constexpr (PK == 3) return x * (x * x);
+    if constexpr (PK == 4) return x * (x * (x * x));
+    return x * (x * (x * (x * x)));
+}
+
+// PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
+// other p (device pow).
+template <class T, int PK>
+__global__ __launch_bounds__(256) void structure_cube_tiled_kernel(
+    const T* __restrict__ U, const long long* __restrict__ bases, StructGeom g, int H, double p,
+    int nvar, int tiles_x, double* partial) {
+    extern __shared__ double lds[];
+    const int W = tile_width(H);
+    const int R = H - 1;
+    double* tile = lds;               // W x W staged values
+    double* wsum = lds + W * W;       // [4][H] wave sums
+    const int k = blockIdx.z, var = blockIdx.y;
+    const int x0 = (blockIdx.x % tiles_x) * kTile;
+    const int y0 = (blockIdx.x / tiles_x) * kTile;
+    const T* u = U + bases[k] + (long long)var * g.vstride;
+    for (int idx = threadIdx.x; idx < W * W; idx += 256) {
+        const int lx = idx % W, ly = idx / W;
+        tile[idx] = value_at(u, g, x0 - R + lx, y0 - R + ly, 0);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const bool valid = x0 + tx < g.nx && y0 + ty < g.ny;
+    const double* c = tile + (ty + R) * W + (tx + R);  // this cell
+    const double uc = *c;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto term = [&](double v) { return PK > 0 ? pow_int<PK>(fabs(uc - v)) : pow(fabs(uc - v), p); };
+    for (int h = 1; h < H; ++h) {
+        double acc = 0;
+        if (valid) {
+            // one running sum per shell side (independent chains for the
+            // FP64 adder), added together at the end of the shell
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            // x sides: columns i-h and i+h, rows j-h+1 .. j+h-1
+            const double* cl = c - h + (1 - h) * W;
+            const double* cr = c + h + (1 - h) * W;
+            for (int y = 0; y < 2 * h - 1; ++y) {
+                a0 += term(cl[y * W]);
+                a1 += term(cr[y * W]);
+            }
+            // y sides: rows j-h and j+h, columns i-h .. i+h
+            const double* rb = c - h * W - h;
+            const double* rt = c + h * W - h;
+            for (int x = 0; x < 2 * h + 1; ++x) {
+                a2 += term(rb[x]);
+                a3 += term(rt[x]);
+            }
+            acc = (a0 + a1) + (a2 + a3);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) wsum[wave * H + h] = acc;
+    }
+    __syncthreads();
+    for (int h = 1 + (int)threadIdx.x; h < H; h += 256) {
+        const double tot = ((wsum[h] + wsum[H + h]) + wsum[2 * H + h]) + wsum[3 * H + h];
+        partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restrict__ U,
                                                              const long long* __restrict__ bases,
@@ -338,14 +419,56 @@ StructGeom geom_of(const alsvinn_hip_ensemble* e, const SampleDesc& D) {
     return g;
 }
 
+// Fixed-order total of the structure-function partials: thread h of block
+// var adds partial[((k*nvar + var)*nblocks + b)*H + h] over samples k in
+// list order and blocks b in index order, one running sum, the order a
+// sequential host loop over the partials would use.
+__global__ __launch_bounds__(64) void sum_partials_kernel(const double* __restrict__ part, int m,
+                                                          int nvar, int nblocks, int H,
+                                                          double* __restrict__ out) {
+    const int var = blockIdx.x;
+    for (int h = threadIdx.x; h < H; h += blockDim.x) {
+        double acc = 0;
+        for (int k = 0; k < m; ++k) {
+            const double* src = part + ((long long)k * nvar + var) * nblocks * H + h;
+            int b = 0;
+            for (; b + 8 <= nblocks; b += 8) {
+                double v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = src[(long long)(b + i) * H];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc += v[i];
+            }
+            for (; b < nblocks; ++b) acc += src[(long long)b * H];
+        }
+        out[(long long)var * H + h] = acc;
+    }
+}
+
+template <class T>
+void launch_tiled(int pk, dim3 grid, size_t lds, hipStream_t st, const T* U,
+                  const long long* bases, StructGeom g, int H, double p, int nvar, int tiles_x,
+                  double* part) {
+    switch (pk) {
+        case 1: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 1>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        case 2: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 2>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        case 3: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 3>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        case 4: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 4>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        case 5: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 5>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        default: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 0>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+    }
+}
+
+// nblocks_override > 0: the kernel's own block count per (sample, variable)
+// (the tiled kernel's tiles); otherwise one block per 256 cells.
 template <class Launch>
 int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, double* host_out,
-                  Launch launch) {
+                  Launch launch, int nblocks_override = 0) {
     std::vector<long long> bases;
     if (int rc = group_bases(e, samples, m, bases)) return rc;
     const SampleDesc& D = e->desc[samples[0]];
     const long long ncell = (long long)D.nx * D.ny * D.nz;
-    const int nblocks = int((ncell + 255) / 256);
+    const int nblocks = nblocks_override > 0 ? nblocks_override : int((ncell + 255) / 256);
     DeviceArray<long long> d_bases;
     DeviceArray<double> d_part;
     const size_t npart = (size_t)m * e->nvar * nblocks * H;
@@ -354,24 +477,27 @@ int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, dou
     HIP_TRY(d_part.alloc(npart));
     HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d_part.p, 0, sizeof(double) * npart));
+    DeviceArray<double> d_sum;
+    HIP_TRY(d_sum.alloc((size_t)e->nvar * H));
     launch(dim3(nblocks, e->nvar, m), d_bases.p, geom_of(e, D), d_part.p);
     hipError_t le = hipGetLastError();
+    // fixed-order reduction on the device: for each (variable, h), samples
+    // in list order, blocks in index order
+    if (le == hipSuccess) {
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(e->nvar), dim3(64), 0, e->stream, d_part.p, m,
+                           e->nvar, nblocks, H, d_sum.p);
+        le = hipGetLastError();
+    }
     hipError_t se = hipStreamSynchronize(e->stream);
     if (le == hipSuccess) le = se;
-    std::vector<double> part(npart);
-    hipError_t ce = hipMemcpy(part.data(), d_part.p, sizeof(double) * npart, hipMemcpyDeviceToHost);
+    std::vector<double> acc((size_t)e->nvar * H);
+    hipError_t ce = hipMemcpy(acc.data(), d_sum.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost);
     if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
     if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
-    // fixed-order reduction: samples in list order, blocks in index order
     const double N = double(ncell);
     for (int var = 0; var < e->nvar; ++var)
-        for (int h = 0; h < H; ++h) {
-            double acc = 0;
-            for (int k = 0; k < m; ++k)
-                for (int b = 0; b < nblocks; ++b)
-                    acc += part[(((size_t)k * e->nvar + var) * nblocks + b) * H + h];
-            host_out[(size_t)var * H + h] = (acc / N) / double(m);
-        }
+        for (int h = 0; h < H; ++h)
+            host_out[(size_t)var * H + h] = (acc[(size_t)var * H + h] / N) / double(m);
     return 0;
 }
 
@@ -477,6 +603,28 @@ int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samp
 int alsvinn_hip_stats_structure_cube(alsvinn_hip_ensemble* e, const int* samples, int m,
                                      int number_of_h, double p, double* host_out) {
     if (!e || number_of_h < 1 || !host_out) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const int H = number_of_h;
+    // 2D with a halo that fits in 64 KB of LDS (H <= 38): the tiled kernel
+    const size_t lds = (size_t(tile_width(H)) * tile_width(H) + 4 * size_t(H)) * sizeof(double);
+    if (e->dim == 2 && H >= 2 && lds <= 65536 && m > 0 && samples) {
+        if (samples[0] < 0 || samples[0] >= e->nsamples)
+            return fail(ALSVINN_HIP_E_INVALID, "sample index out of range");
+        const SampleDesc& D = e->desc[samples[0]];
+        const int tiles_x = (D.nx + kTile - 1) / kTile;
+        const int tiles = tiles_x * ((D.ny + kTile - 1) / kTile);
+        return run_structure(
+            e, samples, m, H, host_out,
+            [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
+                const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0) ? int(p) : 0;
+                if (e->esize == 8)
+                    launch_tiled<double>(pk, grid, lds, e->stream, (const double*)e->U, bases, g,
+                                         H, p, e->nvar, tiles_x, part);
+                else
+                    launch_tiled<float>(pk, grid, lds, e->stream, (const float*)e->U, bases, g, H,
+                                        p, e->nvar, tiles_x, part);
+            },
+            tiles);
+    }
     return run_structure(e, samples, m, number_of_h, host_out,
                          [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
                              if (e->esize == 8)

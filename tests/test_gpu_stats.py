@@ -109,6 +109,34 @@ def test_structure_functions_match_oracle(hip, bc):
     np.testing.assert_allclose(basic_y, ref_by, rtol=RTOL_REDUCTION, atol=0)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("bc", [O.BC_PERIODIC, O.BC_NEUMANN])
+def test_structure_cube_2d_tiles_and_wide_halos(hip, bc, prec):
+    """2D structure_cube at the edges of its LDS-tiled kernel: a halo wider
+    than the grid (H = 12 on 9 x 7: several periodic wraps or Neumann clamps
+    per side), partial tiles at the largest H of the statistics
+    configurations (H = 33 on 33 x 17), and H = 40, whose halo does not fit
+    in LDS (the per-cell kernel), for FP64 and FP32 states, against the
+    oracle on the downloaded volumes."""
+    _case("structure cube tiles bc=%d %s" % (bc, prec))
+    rng = np.random.default_rng(11)
+    precision = hip.FP64 if prec == "fp64" else hip.FP32
+    for (nx, ny), H, M in (((9, 7), 12, 3), ((33, 17), 33, 2), ((18, 20), 40, 2)):
+        scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, bc,
+                                 precision=precision)
+        samples = [hip.make_sample((nx, ny, 1), (0, 0, 0), (1, 1, 0), 1.0) for _ in range(M)]
+        with hip.Ensemble(scheme, samples) as e:
+            for i in range(M):
+                e.upload(i, rng.random(e.volume_shape(i)) + 1.0)
+            e.apply_boundary()
+            vols = [e.download(i) for i in range(M)]
+            cube = e.structure_cube(H, 2.0)
+        ref = sum(O.structure_cube(v, (nx, ny, 1), 2, H, 2.0, bc == O.BC_PERIODIC)
+                  for v in vols) / M
+        np.testing.assert_allclose(cube, ref, rtol=RTOL_REDUCTION, atol=0)
+        assert np.all(cube[:, 0] == 0.0)
+
+
 @pytest.mark.parametrize("dim", [1, 2])
 def test_onepoint_moment_and_bv_match_oracle(hip, dim):
     """OnePointMoment (p = 1, 2 bit for bit: u and u*u summed in sample order;
