@@ -210,10 +210,12 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict
 // of cells of one sample (blockIdx.z) and variable (blockIdx.y) and first
 // stages the tile with a halo of R = H-1 cells on every side (the boundary's
 // wrap or clamp applied as it is loaded), so each of a cell's 8h shell
-// neighbours is an LDS read. Every thread sums its cell's shell terms in the
-// same order as structure_cube_kernel; the block total of each h (wave sums,
-// then the four waves in order) goes to partial[((k*nvar + var)*ntiles +
-// tile)*H + h], tiles numbered row-major.
+// neighbours is an LDS read. Every thread keeps one running sum per shell
+// side (the same terms as structure_cube_kernel, in four independent chains
+// for the FP64 adder); the block total of each h (wave sums, then the four
+// waves in order) goes to partial[((k*nvar + var)*ntiles + tile)*H + h],
+// tiles numbered row-major. The total is a parallel sum either way, within
+// rounding of the reference's sequential loop.
 constexpr int kTile = 16;
 
 __host__ __device__ constexpr int tile_width(int H) { return kTile + 2 * (H - 1); }
