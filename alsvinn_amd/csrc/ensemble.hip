@@ -837,6 +837,13 @@ int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* 
     return 0;
 }
 
+int alsvinn_hip_ensemble_get_stage_waves(const alsvinn_hip_ensemble* e, int* waves) {
+    if (!e || !waves) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const int* w = e->esize == 8 ? e->k64.waves : e->k32.waves;
+    for (int s = 0; s < 3; ++s) waves[s] = w[e->variant[s]];
+    return 0;
+}
+
 int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
                                       int* ntasks) {
     if (!e) return fail(ALSVINN_HIP_E_INVALID, "ensemble is NULL");

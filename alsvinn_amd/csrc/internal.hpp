@@ -16,15 +16,15 @@ int fail(int code, const std::string& msg);
 
 // Launchers of the kernels one scheme (equation, reconstruction, flux, grid
 // dimension, precision) needs.
-constexpr int kStageVariants = 3;  // register budgets for 1, 2, 3 waves per SIMD
+constexpr int kStageVariants = 3;  // register budgets (KernelSet::waves gives each one's waves per SIMD)
 constexpr int kDefaultVariant = 1;
-inline int variant_waves(int v) { return 1 + v; }
 // Stage kind of a stage mode, for the per-kind register budget.
 inline int stage_kind(int mode) { return mode == MODE_RK3_1 ? 1 : (mode == MODE_RK3_2 ? 2 : 0); }
 
 template <class T>
 struct KernelSet {
     void (*stage[kStageVariants])(const StageArgs<T>& a, int blocks, hipStream_t s) = {};
+    int waves[kStageVariants] = {1, 2, 3};  // waves per SIMD of each variant's register budget
     void (*axpy)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                  const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
                  double gamma) = nullptr;
@@ -68,26 +68,25 @@ constexpr bool kAllBudgets =
     (std::is_same_v<Flux, FluxHLL> || std::is_same_v<Flux, FluxHLL3> ||
      std::is_same_v<Flux, FluxGodunov>);
 
-// Reconstructions wider than three cells run on the per-cell wide kernel
-// (one register budget).
-template <class T, class Eq, class Recon, class Flux, int DIM>
+// Reconstructions wider than three cells run on the per-cell wide kernel.
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS>
 void launch_stage_wide(const StageArgs<T>& a, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
     switch (a.mode) {
         case MODE_RHS:
-            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RHS>), dim3(blocks),
-                               dim3(256), 0, s, a);
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, WPS, MODE_RHS>),
+                               dim3(blocks), dim3(256), 0, s, a);
             break;
         case MODE_RK2_1:
-            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK2_1>),
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, WPS, MODE_RK2_1>),
                                dim3(blocks), dim3(256), 0, s, a);
             break;
         case MODE_RK3_1:
-            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK3_1>),
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, WPS, MODE_RK3_1>),
                                dim3(blocks), dim3(256), 0, s, a);
             break;
         case MODE_RK3_2:
-            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, MODE_RK3_2>),
+            hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, WPS, MODE_RK3_2>),
                                dim3(blocks), dim3(256), 0, s, a);
             break;
     }
@@ -96,13 +95,26 @@ void launch_stage_wide(const StageArgs<T>& a, int blocks, hipStream_t s) {
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
     if constexpr (recon_radius<Recon>::value > 1) {
-        k.stage[0] = k.stage[1] = k.stage[2] = &launch_stage_wide<T, Eq, Recon, Flux, DIM>;
+        // wide kernels wait on their cached stencil loads: the autotuner
+        // also tries the tighter register budgets for HLL3 (the Euler
+        // configurations); the others get 2 waves/SIMD
+        if constexpr (std::is_same_v<Flux, FluxHLL3>) {
+            k.stage[0] = &launch_stage_wide<T, Eq, Recon, Flux, DIM, 2>;
+            k.stage[1] = &launch_stage_wide<T, Eq, Recon, Flux, DIM, 3>;
+            k.stage[2] = &launch_stage_wide<T, Eq, Recon, Flux, DIM, 4>;
+            k.waves[0] = 2, k.waves[1] = 3, k.waves[2] = 4;
+        } else {
+            k.stage[0] = k.stage[1] = k.stage[2] = &launch_stage_wide<T, Eq, Recon, Flux, DIM, 2>;
+            k.waves[0] = k.waves[1] = k.waves[2] = 2;
+        }
     } else if constexpr (kAllBudgets<Recon, Flux>) {
         k.stage[0] = &launch_stage<T, Eq, Recon, Flux, DIM, 1>;
         k.stage[1] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
         k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 3>;
+        k.waves[0] = 1, k.waves[1] = 2, k.waves[2] = 3;
     } else {
         k.stage[0] = k.stage[1] = k.stage[2] = &launch_stage<T, Eq, Recon, Flux, DIM, 2>;
+        k.waves[0] = k.waves[1] = k.waves[2] = 2;
     }
 }
 

@@ -674,8 +674,8 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
 // once; in 3D the y faces of a row are evaluated by the row itself (twice
 // per face). The divergence is summed in the marching kernels' order:
 // ((0 + x part) + y part) + z part.
-template <class T, class Eq, class Recon, class Flux, int DIM, int MODE>
-__global__ __launch_bounds__(256) void stage_kernel_wide(StageArgs<T> a) {
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
+__global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;

@@ -125,10 +125,10 @@ class Ensemble:
         r, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(_lib.load().alsvinn_hip_ensemble_get_schedule(self.h, ctypes.byref(r), ctypes.byref(v),
                                                             ctypes.byref(n)))
-        per = (ctypes.c_int * 3)()
-        check(_lib.load().alsvinn_hip_ensemble_get_stage_variants(self.h, per))
-        return {"rows": r.value, "variant": v.value, "waves_per_simd": 1 + v.value,
-                "stage_waves_per_simd": [1 + x for x in per], "tasks": n.value}
+        waves = (ctypes.c_int * 3)()
+        check(_lib.load().alsvinn_hip_ensemble_get_stage_waves(self.h, waves))
+        return {"rows": r.value, "variant": v.value, "waves_per_simd": waves[0],
+                "stage_waves_per_simd": list(waves), "tasks": n.value}
 
     def autotune(self):
         r, v = ctypes.c_int(), ctypes.c_int()

@@ -189,6 +189,10 @@ int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* varia
 /* The register-budget variant of each stage kind: [0] first stage (and the
  * RK2/RK4 evaluations), [1] second and [2] third SSP-RK3 stage. */
 int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* variants);
+/* Waves per SIMD that the chosen register budget of each stage kind allows,
+ * in the same order (no counterpart in the reference: a report of the
+ * autotuned schedule). */
+int alsvinn_hip_ensemble_get_stage_waves(const alsvinn_hip_ensemble* e, int* waves);
 
 /* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
  * every sample that has not reached its end time. Asynchronous. */
