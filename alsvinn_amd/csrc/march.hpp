@@ -747,12 +747,50 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                     ws[MD] = std_max(ws[MD], s);
                 }
             }
+            // 1D/2D: the x stencil comes from the neighbouring lanes (DPP
+            // moves of the centre row, which every lane of a segment holds)
+            // and the marched stencil from a register window of the rows
+            // m+1-RAD .. m+1+RAD, so each row of the stage input is loaded
+            // once; 3D loads its stencils (cached).
+            constexpr bool WINDOW = DIM <= 2;
+            const int wcol = in_strip ? col + D.gx : D.gx;  // a column inside the volume
+            auto row_flat = [&](int m) -> long long {
+                int py = (DIM == 2 ? m : 0) + D.gy;
+                py = py < 0 ? 0 : (py >= D.ty ? D.ty - 1 : py);
+                return (long long)py * D.tx + wcol;
+            };
+            T Wn[2 * RAD + 1][NV];  // window rows m+1-RAD .. m+1+RAD (DIM == 2); row m at RAD-1
+            if constexpr (WINDOW) {
+                if constexpr (DIM == 2) {
+#pragma unroll
+                    for (int k = 0; k <= 2 * RAD; ++k)
+                        load_cell<T, NV>(in, in2, LAZY, c, row_flat(m0 + 1 - RAD + k), vs, Wn[k]);
+                } else {
+                    load_cell<T, NV>(in, in2, LAZY, c, row_flat(0), vs, Wn[RAD - 1]);
+                }
+            }
             for (int m = m0; m < m1; ++m) {  // uniform over the wavefront
                 const long long i0 = flat(m);
                 T out_v[NV], uin[NV];
+                PendingCell<T, NV> pn;
+                if constexpr (DIM == 2) pn.issue(in, in2, LAZY, row_flat(m + 2 + RAD), vs);
                 {  // x: own right face, left face from the lane below
                     T q[2 * RAD + 1][NV], Lx[NV], Rx[NV];
-                    if (xlane) {
+                    if constexpr (WINDOW) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            q[RAD][v] = Wn[RAD - 1][v];
+#pragma unroll
+                            for (int k = RAD - 1; k >= 0; --k) q[k][v] = lane_from_lower(q[k + 1][v]);
+#pragma unroll
+                            for (int k = RAD + 1; k <= 2 * RAD; ++k)
+                                q[k][v] = lane_from_upper(q[k - 1][v]);
+                            uin[v] = q[RAD][v];
+                        }
+                        // lanes outside [G-1, G+ncols] reconstruct from values
+                        // past their segment; no face that is used reads them
+                        recon_stencil<Recon, T, NV>(q, Lx, Rx, eq.gamma);
+                    } else if (xlane) {
                         recon_at(std::integral_constant<int, 0>(), i0, q, Lx, Rx);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) uin[v] = q[RAD][v];
@@ -795,8 +833,13 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                 if constexpr (DIM >= 2) {  // marched direction
                     T Gp[NV];
                     if (interior) {
-                        T q[2 * RAD + 1][NV], Ln[NV], Rn[NV];
-                        recon_at(std::integral_constant<int, MD>(), flat(m + 1), q, Ln, Rn);
+                        T Ln[NV], Rn[NV];
+                        if constexpr (WINDOW) {
+                            recon_stencil<Recon, T, NV>(Wn, Ln, Rn, eq.gamma);
+                        } else {
+                            T q[2 * RAD + 1][NV];
+                            recon_at(std::integral_constant<int, MD>(), flat(m + 1), q, Ln, Rn);
+                        }
                         const T s = face_flux<T, Eq, Flux, MD>(eq, Rprev, Ln, Gp);
                         ws[MD] = std_max(ws[MD], s);
 #pragma unroll
@@ -821,6 +864,13 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                     if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
                     store_cell<T, NV, DIM>(out, vs, D, col, DIM == 3 ? o : m, DIM == 3 ? m : 0,
                                            res);
+                }
+                if constexpr (DIM == 2) {  // advance the window by one row
+#pragma unroll
+                    for (int k = 0; k < 2 * RAD; ++k)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Wn[k][v] = Wn[k + 1][v];
+                    pn.finish(LAZY, c, Wn[2 * RAD]);
                 }
             }
         }

@@ -512,6 +512,9 @@ def test_wide_stencil_reconstructions_bit_exact(hip, recon, case):
                                                   (125, 2, 1)]),
     (O.EQ_EULER3, O.RECON_MC, O.BC_NEUMANN, [(2, 2, 2), (3, 2, 5), (61, 3, 2)]),
     (O.EQ_EULER1, O.RECON_ENO4, O.BC_PERIODIC, [(4, 1, 1), (5, 1, 1), (57, 1, 1), (200, 1, 1)]),
+    (O.EQ_EULER2, O.RECON_WENO3, O.BC_PERIODIC, [(3, 3, 1), (40, 9, 1), (59, 4, 1), (121, 6, 1),
+                                                 (59, 5, 1)]),
+    (O.EQ_BURGERS, O.RECON_ENO3, O.BC_NEUMANN, [(3, 4, 1), (57, 6, 1), (58, 3, 1), (130, 5, 1)]),
 ])
 def test_minimal_and_ragged_grids_mixed_end_times(hip, eq, recon, bc, sizes):
     """One ensemble of samples on the smallest grids the ghost layers allow,
