@@ -429,17 +429,31 @@ __global__ __launch_bounds__(64) void sum_partials_kernel(const double* __restri
                                                           int nvar, int nblocks, int H,
                                                           double* __restrict__ out) {
     const int var = blockIdx.x;
+    constexpr int kBatch = 32;  // loads in flight ahead of the additions
     for (int h = threadIdx.x; h < H; h += blockDim.x) {
         double acc = 0;
         for (int k = 0; k < m; ++k) {
             const double* src = part + ((long long)k * nvar + var) * nblocks * H + h;
             int b = 0;
-            for (; b + 8 <= nblocks; b += 8) {
-                double v[8];
+            double v[kBatch];
+            if (b + kBatch <= nblocks) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = src[(long long)(b + i) * H];
+                for (int i = 0; i < kBatch; ++i) v[i] = src[(long long)(b + i) * H];
+            }
+            for (; b + kBatch <= nblocks; b += kBatch) {
+                // the next batch's loads are issued before this batch is added
+                double w[kBatch];
+                const bool more = b + 2 * kBatch <= nblocks;
+                if (more) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) acc += v[i];
+                    for (int i = 0; i < kBatch; ++i) w[i] = src[(long long)(b + kBatch + i) * H];
+                }
+#pragma unroll
+                for (int i = 0; i < kBatch; ++i) acc += v[i];
+                if (more) {
+#pragma unroll
+                    for (int i = 0; i < kBatch; ++i) v[i] = w[i];
+                }
             }
             for (; b < nblocks; ++b) acc += src[(long long)b * H];
         }
