@@ -172,8 +172,10 @@ int alsvinn_hip_ensemble_apply_boundary(alsvinn_hip_ensemble* e);
 int alsvinn_hip_ensemble_reset_time(alsvinn_hip_ensemble* e);
 
 /* Work schedule of the stage kernels: rows per wavefront task (0 = built-in
- * heuristic) and the register-budget variant (0, 1, 2 = 1, 2, 3 waves per
- * SIMD). Results do not depend on the schedule, only the speed does. */
+ * heuristic) and the register-budget variant (0, 1, 2; the marching kernels'
+ * budgets allow 1, 2, 3 waves per SIMD, the wide-stencil kernel's with HLL3
+ * 2, 3, 4, as alsvinn_hip_ensemble_get_stage_waves reports). Results do not
+ * depend on the schedule, only the speed does. */
 int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int variant);
 int alsvinn_hip_ensemble_get_schedule(const alsvinn_hip_ensemble* e, int* rows, int* variant,
                                       int* ntasks);

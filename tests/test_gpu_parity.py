@@ -558,6 +558,52 @@ def test_minimal_and_ragged_grids_mixed_end_times(hip, eq, recon, bc, sizes):
         _assert_bit_exact(ref, got[i])
 
 
+@pytest.mark.parametrize("recon", [O.RECON_WENO2, O.RECON_ENO3])
+def test_schedule_does_not_change_results(hip, recon):
+    """Every band height and register budget the autotuner can choose gives
+    the same bits: three 2D Euler samples (two 48^2, one ragged 70 x 33),
+    HLL3 + SSP-RK3, eight steps under each schedule, against the oracle. The
+    reported waves per SIMD follow the budget: 1/2/3 on the marching kernel
+    (WENO2), 2/3/4 on the wide-stencil kernel (ENO3)."""
+    _print_case("schedules recon=%d" % recon)
+    rng = np.random.default_rng(61)
+    sizes = [(48, 48, 1), (48, 48, 1), (70, 33, 1)]
+    setups, u0s, refs = [], [], []
+    for n in sizes:
+        s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, recon, O.INT_RK3, O.BC_PERIODIC, n, (0, 0, 0),
+                         (1, 1, 0), cfl=0.475, end_time=2.0, max_steps=8)
+        shape = O.volume_shape(s)
+        u0 = np.zeros(shape)
+        u0[0] = 1.0 + 0.2 * rng.random(shape[1:])
+        u0[1] = 0.1 * rng.standard_normal(shape[1:])
+        u0[2] = 0.1 * rng.standard_normal(shape[1:])
+        u0[3] = 3.0 + 0.3 * rng.random(shape[1:])
+        rc, ref, t, steps, _ = O.run(s, u0)
+        assert rc == 0 and steps == 8
+        setups.append(s)
+        u0s.append(u0)
+        refs.append(ref)
+    waves = [2, 3, 4] if recon == O.RECON_ENO3 else [1, 2, 3]
+    scheme = hip.make_scheme(O.EQ_EULER2, O.FLUX_HLL3, recon, O.INT_RK3, O.BC_PERIODIC,
+                             gamma=1.4, cfl=0.475)
+    for rows in (0, 2, 16):
+        for variant in (0, 1, 2):
+            samples = [hip.make_sample(n, (0, 0, 0), (1, 1, 0), 2.0, max_steps=8) for n in sizes]
+            with hip.Ensemble(scheme, samples) as e:
+                for i, u in enumerate(u0s):
+                    e.upload(i, u)
+                e.apply_boundary()
+                e.set_schedule(rows, variant)
+                sch = e.schedule()
+                assert sch["stage_waves_per_simd"] == [waves[variant]] * 3, sch
+                e.run(4)
+                got = [e.download(i) for i in range(len(sizes))]
+                infos = [e.info(i) for i in range(len(sizes))]
+            for i in range(len(sizes)):
+                assert infos[i]["steps"] == 8, (rows, variant, i)
+                _assert_bit_exact(refs[i], got[i])
+
+
 @pytest.mark.parametrize("case", ["sod", "kh2d", "euler3", "burgers_eno3"])
 def test_rk4_bit_exact(hip, case):
     """The classical RK4 integrator (RungeKutta4.cpp:50-96): three stage
