@@ -117,7 +117,8 @@ def test_structure_cube_2d_tiles_and_wide_halos(hip, bc, prec):
     per side), partial tiles at the largest H of the statistics
     configurations (H = 33 on 33 x 17), and H = 40, whose halo does not fit
     in LDS (the per-cell kernel), for FP64 and FP32 states, against the
-    oracle on the downloaded volumes."""
+    oracle on the downloaded volumes; structure_basic in x and y on the same
+    volumes (shifts up to H-1, past the grid's width on 9 x 7)."""
     _case("structure cube tiles bc=%d %s" % (bc, prec))
     rng = np.random.default_rng(11)
     precision = hip.FP64 if prec == "fp64" else hip.FP32
@@ -131,10 +132,15 @@ def test_structure_cube_2d_tiles_and_wide_halos(hip, bc, prec):
             e.apply_boundary()
             vols = [e.download(i) for i in range(M)]
             cube = e.structure_cube(H, 2.0)
-        ref = sum(O.structure_cube(v, (nx, ny, 1), 2, H, 2.0, bc == O.BC_PERIODIC)
-                  for v in vols) / M
+            basic = [e.structure_basic(H, d, 2.0) for d in (0, 1)]
+        per = lambda f: sum(f(v) for v in vols) / M
+        ref = per(lambda v: O.structure_cube(v, (nx, ny, 1), 2, H, 2.0, bc == O.BC_PERIODIC))
         np.testing.assert_allclose(cube, ref, rtol=RTOL_REDUCTION, atol=0)
         assert np.all(cube[:, 0] == 0.0)
+        for d in (0, 1):
+            ref_b = per(lambda v: O.structure_basic(v, (nx, ny, 1), 2, H, 2.0, d,
+                                                    bc == O.BC_PERIODIC))
+            np.testing.assert_allclose(basic[d], ref_b, rtol=RTOL_REDUCTION, atol=0)
 
 
 @pytest.mark.parametrize("dim", [1, 2])
