@@ -273,6 +273,18 @@ __device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDes
     }
 }
 
+// Workgroups are dispatched round-robin over the GPU's 8 XCDs (each with its
+// own L2); this maps dispatch order to a logical block index so that each XCD
+// runs a contiguous range of logical blocks, i.e. of neighbouring wave tasks,
+// which share halo rows and columns in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+    constexpr int kXcd = 8;
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int q = nb / kXcd, r = nb % kXcd;
+    const int x = b % kXcd, k = b / kXcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // 1D and 2D stage kernel (DIM = active grid dimensions). WPS is the
 // register budget expressed as waves per SIMD (2: no spills for FP64 Euler
 // 2D; 3 and 4 trade spills for occupancy); the autotuner picks one.
@@ -282,7 +294,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
