@@ -466,7 +466,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     Eq eq;
     eq.gamma = T(a.gamma);
 
-    for (int t = blockIdx.x; t < a.ntasks; t += gridDim.x) {  // uniform over the workgroup
+    for (int t = xcd_block(); t < a.ntasks; t += gridDim.x) {  // uniform over the workgroup
         const WaveTask& tk = a.tasks[t];
         int seg = -1;
         for (int k = 0; k < tk.nseg; ++k)
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
     constexpr int RAD = recon_radius<Recon>::value;
     constexpr int MD = DIM == 3 ? 2 : 1;  // marched direction (DIM >= 2)
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
