@@ -50,7 +50,20 @@ __global__ __launch_bounds__(256) void meanvar_kernel(const T* __restrict__ U,
          i += (long long)gridDim.x * blockDim.x) {
         double a = 0, b = 0;
         if (!welford) {
-            for (int k = 0; k < m; ++k) {
+            // samples in order; batches of loads issued ahead of the sums
+            constexpr int kBatch = 16;
+            int k = 0;
+            for (; k + kBatch <= m; k += kBatch) {
+                double u[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) u[j] = double(U[bases[k + j] + i]);
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    a += u[j];
+                    b += u[j] * u[j];
+                }
+            }
+            for (; k < m; ++k) {
                 const double u = double(U[bases[k] + i]);
                 a += u;
                 b += u * u;
@@ -113,7 +126,16 @@ __global__ __launch_bounds__(256) void onepoint_kernel(const T* __restrict__ U,
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         double acc = 0;
-        for (int k = 0; k < m; ++k) acc += ref_pow(double(U[bases[k] + i]), p);
+        constexpr int kBatch = 16;  // loads issued ahead of the ordered sum
+        int k = 0;
+        for (; k + kBatch <= m; k += kBatch) {
+            double u[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) u[j] = double(U[bases[k + j] + i]);
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) acc += ref_pow(u[j], p);
+        }
+        for (; k < m; ++k) acc += ref_pow(double(U[bases[k] + i]), p);
         out[i] = acc / double(m);
     }
 }
