@@ -339,6 +339,10 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         T yc[NV], yp[NV];  // rows r, r+1 of the stage input
         T Ry[NV];          // right (r+1/2) reconstruction of row r in y
         T Gm[NV];          // y face flux at r-1/2
+        // reconstructions from primitive variables: those of rows r, r+1,
+        // computed once per row and shared by the x and y stencils
+        constexpr bool PRIM = recon_prim<Recon>::value;
+        T Pc[NV], Pp[NV];
         const int gy = D.gy;
         if constexpr (DIM >= 2) {
             const int pr = tk.y0 + gy;
@@ -347,21 +351,31 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), vs, ym);
             load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), vs, yc);
             load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), vs, yp);
-            T Lt[NV], Rt[NV];
-            Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt, eq.gamma);  // row r-1: keep R
-            T L0[NV], R0[NV];
-            Recon::template apply<T, NV>(ym, yc, yp, L0, R0, eq.gamma);  // row r
+            T Lt[NV], Rt[NV], L0[NV], R0[NV];
+            if constexpr (PRIM) {
+                T Pa[NV], Pm[NV];
+                Recon::template prim<T, NV>(qa, Pa, eq.gamma);
+                Recon::template prim<T, NV>(ym, Pm, eq.gamma);
+                Recon::template prim<T, NV>(yc, Pc, eq.gamma);
+                Recon::template prim<T, NV>(yp, Pp, eq.gamma);
+                Recon::template apply_prim<T, NV>(Pa, Pm, Pc, Lt, Rt, eq.gamma);  // row r-1
+                Recon::template apply_prim<T, NV>(Pm, Pc, Pp, L0, R0, eq.gamma);  // row r
+            } else {
+                Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt, eq.gamma);  // row r-1: keep R
+                Recon::template apply<T, NV>(ym, yc, yp, L0, R0, eq.gamma);  // row r
+            }
             const T s = face_flux<T, Eq, Flux, 1>(eq, Rt, L0, Gm);
             if (interior) wsy = std_max(wsy, s);
 #pragma unroll
             for (int v = 0; v < NV; ++v) Ry[v] = R0[v];
         } else {
             load_cell<T, NV>(in, in2, LAZY, c, row_index(0), vs, yc);
+            if constexpr (PRIM) Recon::template prim<T, NV>(yc, Pc, eq.gamma);
         }
 
         for (int r = tk.y0; r < tk.y1; ++r) {
             const int pr = r + gy;
-            T qn[NV];
+            T qn[NV], Pn[NV];
             // next row first: its latency overlaps the x sweep
             PendingCell<T, NV> pn;
             if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2), vs);
@@ -377,13 +391,17 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             // ---- x direction on row r (first: out = 0 + x divergence)
             {
                 T xm[NV], xp[NV];
+                const T* xc = PRIM ? Pc : yc;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    xm[v] = lane_from_lower(yc[v]);
-                    xp[v] = lane_from_upper(yc[v]);
+                    xm[v] = lane_from_lower(xc[v]);
+                    xp[v] = lane_from_upper(xc[v]);
                 }
                 T Lx[NV], Rx[NV];
-                Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx, eq.gamma);
+                if constexpr (PRIM)
+                    Recon::template apply_prim<T, NV>(xm, Pc, xp, Lx, Rx, eq.gamma);
+                else
+                    Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx, eq.gamma);
                 T Ln[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
@@ -401,7 +419,12 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             if constexpr (DIM >= 2) {
                 pn.finish(LAZY, c, qn);
                 T Ln[NV], Rn[NV];
-                Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn, eq.gamma);  // row r+1
+                if constexpr (PRIM) {
+                    Recon::template prim<T, NV>(qn, Pn, eq.gamma);
+                    Recon::template apply_prim<T, NV>(Pc, Pp, Pn, Ln, Rn, eq.gamma);  // row r+1
+                } else {
+                    Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn, eq.gamma);  // row r+1
+                }
                 T Gp[NV];
                 const T s = face_flux<T, Eq, Flux, 1>(eq, Ry, Ln, Gp);
                 if (interior) wsy = std_max(wsy, s);
@@ -423,6 +446,10 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 for (int v = 0; v < NV; ++v) {
                     yc[v] = yp[v];
                     yp[v] = qn[v];
+                    if constexpr (PRIM) {
+                        Pc[v] = Pp[v];
+                        Pp[v] = Pn[v];
+                    }
                 }
             }
         }

@@ -496,13 +496,17 @@ struct ReconWENOF2 {
         for (int k = 1; k < NV - 1; ++k) out[k] = P[k] * P[0];
         out[NV - 1] = dvp(P[NV - 1], gamma - T(1)) + T(0.5) * P[0] * dotu<T, NV>(P, P);
     }
+    // The reconstruction from the primitive variables of the three cells
+    // (P[0] is the density, the weights' input): apply() computes them;
+    // a caller that already holds them (each cell's primitives serve three
+    // stencils) calls this directly.
     template <class T, int NV>
-    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R, T gamma) {
+    __device__ __forceinline__ static void apply_prim(const T* Pm, const T* P0, const T* Pp, T* L,
+                                                      T* R, T gamma) {
         const T eps = T(1e-8);  // WENOCoefficients.hpp:21
-        const T i0 = q0[0];
-        const T d0 = qp[0] - i0;
-        const T d1 = i0 - qm[0];
+        const T i0 = P0[0];
+        const T d0 = Pp[0] - i0;
+        const T d1 = i0 - Pm[0];
         const T e0 = eps + d0 * d0;
         const T e1 = eps + d1 * d1;
         const T D0 = T(3) * e0 * e0;
@@ -516,10 +520,6 @@ struct ReconWENOF2 {
         const T w1L = dvp(a1L, a0L + a1L);
         const T w0R = dvp(a0R, a0R + a1R);
         const T w1R = dvp(a1R, a0R + a1R);
-        T Pm[NV], P0[NV], Pp[NV];
-        prim<T, NV>(qm, Pm, gamma);
-        prim<T, NV>(q0, P0, gamma);
-        prim<T, NV>(qp, Pp, gamma);
         T dL[NV], dR[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -530,6 +530,16 @@ struct ReconWENOF2 {
         side<T, NV>(P0, dL, Rr, T(-1), gamma, L);
         side<T, NV>(P0, dR, Rr, T(1), gamma, R);
     }
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R, T gamma) {
+        T Pm[NV], P0[NV], Pp[NV];
+        prim<T, NV>(qm, Pm, gamma);
+        prim<T, NV>(q0, P0, gamma);
+        prim<T, NV>(qp, Pp, gamma);
+        apply_prim<T, NV>(Pm, P0, Pp, L, R, gamma);
+    }
+    static constexpr bool uses_primitives = true;
 };
 
 struct ReconENO2 {
@@ -705,6 +715,11 @@ template <class R, class = void>
 struct recon_radius : std::integral_constant<int, 1> {};
 template <class R>
 struct recon_radius<R, std::void_t<decltype(R::radius)>> : std::integral_constant<int, R::radius> {};
+// Reconstructions computed from per-cell primitive variables (apply_prim).
+template <class R, class = void>
+struct recon_prim : std::false_type {};
+template <class R>
+struct recon_prim<R, std::void_t<decltype(R::uses_primitives)>> : std::true_type {};
 
 // Reconstruction of cell q[radius] of a stencil of 2*radius+1 cells.
 template <class Recon, class T, int NV>
