@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict
                                                             const long long* __restrict__ bases,
                                                             StructGeom g, int H, double p,
                                                             int nvar, double* partial) {
-    __shared__ double sh[256];
+    __shared__ double wsum[4][256];  // wave sums of up to 256 values of h
     const int k = blockIdx.z, var = blockIdx.y;
     const long long ncell = (long long)g.nx * g.ny * g.nz;
     const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -197,34 +197,45 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict
     const T* u = U + bases[k] + (long long)var * g.vstride;
     const double uc = value_at(u, g, i, j, l);
     const int dims = g.dims;
-    for (int h = 1; h < H; ++h) {
-        double acc = 0;
-        if (valid) {
-            for (int d = 0; d < dims; d++) {
-                for (int side = 0; side < 2; side++) {
-                    const bool zD = d == 2, yD = d == 1, xD = d == 0;
-                    const int zs = zD ? (side == 0 ? l - h : l + h) : (dims > 2 ? l - h : 0);
-                    const int ze = zD ? zs + 1 : (dims > 2 ? l + h + 1 : 1);
-                    const int ys = yD ? (side == 0 ? j - h : j + h) : (dims >= 2 ? j - h + 1 : 0);
-                    const int ye = yD ? ys + 1 : (dims >= 2 ? j + h : 1);
-                    int xs = xD ? (side == 0 ? i - h : i + h) : i - h;
-                    int xe = xD ? xs + 1 : i + h + 1;
-                    if (zD) {
-                        xs += 1;
-                        xe -= 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* out = partial + (((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H;
+    for (int h0 = 1; h0 < H; h0 += 256) {
+        const int h1 = H < h0 + 256 ? H : h0 + 256;
+        for (int h = h0; h < h1; ++h) {
+            double acc = 0;
+            if (valid) {
+                for (int d = 0; d < dims; d++) {
+                    for (int side = 0; side < 2; side++) {
+                        const bool zD = d == 2, yD = d == 1, xD = d == 0;
+                        const int zs = zD ? (side == 0 ? l - h : l + h) : (dims > 2 ? l - h : 0);
+                        const int ze = zD ? zs + 1 : (dims > 2 ? l + h + 1 : 1);
+                        const int ys = yD ? (side == 0 ? j - h : j + h) : (dims >= 2 ? j - h + 1 : 0);
+                        const int ye = yD ? ys + 1 : (dims >= 2 ? j + h : 1);
+                        int xs = xD ? (side == 0 ? i - h : i + h) : i - h;
+                        int xe = xD ? xs + 1 : i + h + 1;
+                        if (zD) {
+                            xs += 1;
+                            xe -= 1;
+                        }
+                        for (int z = zs; z < ze; z++)
+                            for (int y = ys; y < ye; y++)
+                                for (int x = xs; x < xe; x++) {
+                                    const double uh = value_at(u, g, x, y, z);
+                                    acc += pow_p(fabs(uc - uh), p);
+                                }
                     }
-                    for (int z = zs; z < ze; z++)
-                        for (int y = ys; y < ye; y++)
-                            for (int x = xs; x < xe; x++) {
-                                const double uh = value_at(u, g, x, y, z);
-                                acc += pow_p(fabs(uc - uh), p);
-                            }
                 }
             }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+            if (lane == 0) wsum[wave][h - h0] = acc;
         }
-        const double tot = block_sum(acc, sh);
-        if (threadIdx.x == 0)
-            partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+        __syncthreads();
+        for (int h = h0 + (int)threadIdx.x; h < h1; h += 256) {
+            const int t = h - h0;
+            out[h] = ((wsum[0][t] + wsum[1][t]) + wsum[2][t]) + wsum[3][t];
+        }
+        __syncthreads();
     }
 }
 
@@ -311,13 +322,17 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(
     }
 }
 
-template <class T>
+// PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
+// other p (device pow).
+template <class T, int PK>
 __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restrict__ U,
                                                              const long long* __restrict__ bases,
                                                              StructGeom g, int H, double p,
                                                              int direction, int nvar,
                                                              double* partial) {
-    __shared__ double sh[256];
+    // block totals per h: wave sums, then the four waves in order, for up to
+    // 256 values of h between two barriers
+    __shared__ double wsum[4][256];
     const int k = blockIdx.z, var = blockIdx.y;
     const long long ncell = (long long)g.nx * g.ny * g.nz;
     const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -327,16 +342,27 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restric
     const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
     const T* u = U + bases[k] + (long long)var * g.vstride;
     const double uc = value_at(u, g, i, j, l);
-    for (int h = 0; h < H; ++h) {
-        double acc = 0;
-        if (valid) {
-            const double uh = value_at(u, g, i + h * (direction == 0), j + h * (direction == 1),
-                                       l + h * (direction == 2));
-            acc = pow(fabs(uc - uh), p);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* out = partial + (((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H;
+    for (int h0 = 0; h0 < H; h0 += 256) {
+        const int h1 = H < h0 + 256 ? H : h0 + 256;
+        for (int h = h0; h < h1; ++h) {
+            double acc = 0;
+            if (valid) {
+                const double uh = value_at(u, g, i + h * (direction == 0),
+                                           j + h * (direction == 1), l + h * (direction == 2));
+                acc = PK > 0 ? pow_int<PK>(fabs(uc - uh)) : pow(fabs(uc - uh), p);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+            if (lane == 0) wsum[wave][h - h0] = acc;
         }
-        const double tot = block_sum(acc, sh);
-        if (threadIdx.x == 0)
-            partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+        __syncthreads();
+        for (int h = h0 + (int)threadIdx.x; h < h1; h += 256) {
+            const int t = h - h0;
+            out[h] = ((wsum[0][t] + wsum[1][t]) + wsum[2][t]) + wsum[3][t];
+        }
+        __syncthreads();
     }
 }
 
@@ -480,6 +506,19 @@ __global__ __launch_bounds__(64) void sum_partials_kernel(const double* __restri
             for (; b < nblocks; ++b) acc += src[(long long)b * H];
         }
         out[(long long)var * H + h] = acc;
+    }
+}
+
+template <class T>
+void launch_basic(int pk, dim3 grid, hipStream_t st, const T* U, const long long* bases,
+                  StructGeom g, int H, double p, int dir, int nvar, double* part) {
+    switch (pk) {
+        case 1: hipLaunchKernelGGL((structure_basic_kernel<T, 1>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        case 2: hipLaunchKernelGGL((structure_basic_kernel<T, 2>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        case 3: hipLaunchKernelGGL((structure_basic_kernel<T, 3>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        case 4: hipLaunchKernelGGL((structure_basic_kernel<T, 4>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        case 5: hipLaunchKernelGGL((structure_basic_kernel<T, 5>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        default: hipLaunchKernelGGL((structure_basic_kernel<T, 0>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
     }
 }
 
@@ -683,14 +722,14 @@ int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* sample
         return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
     return run_structure(e, samples, m, number_of_h, host_out,
                          [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
+                             const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0)
+                                                ? int(p) : 0;
                              if (e->esize == 8)
-                                 hipLaunchKernelGGL(structure_basic_kernel<double>, grid, dim3(256),
-                                                    0, e->stream, (const double*)e->U, bases, g,
-                                                    number_of_h, p, direction, e->nvar, part);
+                                 launch_basic<double>(pk, grid, e->stream, (const double*)e->U, bases,
+                                                      g, number_of_h, p, direction, e->nvar, part);
                              else
-                                 hipLaunchKernelGGL(structure_basic_kernel<float>, grid, dim3(256),
-                                                    0, e->stream, (const float*)e->U, bases, g,
-                                                    number_of_h, p, direction, e->nvar, part);
+                                 launch_basic<float>(pk, grid, e->stream, (const float*)e->U, bases,
+                                                     g, number_of_h, p, direction, e->nvar, part);
                          });
 }
 

@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--serving-tmin", type=float, default=0.1)
     ap.add_argument("--serving-tmax", type=float, default=1.0)
-    ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving", "full-uq", "kh3d"],
+    ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving", "full-uq", "kh3d",
+                                       "stats"],
                     default="all")
     ap.add_argument("--samples3d", type=int, default=64)
     ap.add_argument("--n3d", type=int, default=256)
@@ -289,6 +290,34 @@ def main():
                           "var_rho_max": float(var[0].max()),
                           "structure_cube_rho_h1_h32": [float(sf[0, 1]), float(sf[0, 32])],
                           "schedule": sched}), flush=True)
+        return
+
+    if args.only == "stats":
+        # the statistics kernels on the headline batch, after a few steps;
+        # each statistic called three times (the first pays the allocations)
+        M, n = args.samples, args.n
+        X = sampling.stl_mersenne_uniform(M, 40)
+        e = make_batch(hip, [n] * M, args.precision, X)
+        e.run_steps(PER_GRAPH, PER_GRAPH)
+        e.synchronize()
+        calls = [("meanvar", lambda: e.meanvar()),
+                 ("structure_cube_h33_p2", lambda: e.structure_cube(33, 2.0)),
+                 ("structure_basic_h33_x", lambda: e.structure_basic(33, 0, 2.0)),
+                 ("onepointmoment_p2", lambda: e.onepoint_moment(2)),
+                 ("onepointmoment_p3", lambda: e.onepoint_moment(3)),
+                 ("bv_p1", lambda: e.bounded_variation(1))]
+        out = {"workload": "statistics of KH %d^2 x %d, %s (device kernels + host copy of the "
+                           "result; ms per call, three calls)" % (n, M, args.precision)}
+        for name, f in calls:
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                f()
+                ts.append(1e3 * (time.perf_counter() - t0))
+            out[name + "_ms"] = ts
+            log("  %s: %s ms" % (name, ", ".join("%.2f" % t for t in ts)))
+        e.close()
+        print(json.dumps(out), flush=True)
         return
 
     if args.only == "kh3d":
