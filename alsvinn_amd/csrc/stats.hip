@@ -469,44 +469,32 @@ StructGeom geom_of(const alsvinn_hip_ensemble* e, const SampleDesc& D) {
     return g;
 }
 
-// Fixed-order total of the structure-function partials: thread h of block
-// var adds partial[((k*nvar + var)*nblocks + b)*H + h] over samples k in
-// list order and blocks b in index order, one running sum, the order a
-// sequential host loop over the partials would use.
-__global__ __launch_bounds__(64) void sum_partials_kernel(const double* __restrict__ part, int m,
-                                                          int nvar, int nblocks, int H,
-                                                          double* __restrict__ out) {
-    const int var = blockIdx.x;
-    constexpr int kBatch = 32;  // loads in flight ahead of the additions
-    for (int h = threadIdx.x; h < H; h += blockDim.x) {
-        double acc = 0;
-        for (int k = 0; k < m; ++k) {
-            const double* src = part + ((long long)k * nvar + var) * nblocks * H + h;
-            int b = 0;
-            double v[kBatch];
-            if (b + kBatch <= nblocks) {
+// Per-sample totals of the structure-function partials: thread (k, var, h)
+// adds partial[((k*nvar + var)*nblocks + b)*H + h] over the blocks b in index
+// order into sums[(k*nvar + var)*H + h]; the host then adds the samples in
+// list order (each sample's structure function, then their combination, as
+// the reference computes it per sample). The order is fixed, so repeated
+// calls give the same bits.
+__global__ __launch_bounds__(256) void sample_partials_kernel(const double* __restrict__ part,
+                                                             long long n, int nblocks, int H,
+                                                             double* __restrict__ sums) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long kv = t / H;
+    const int h = int(t % H);
+    const double* src = part + kv * nblocks * H + h;
+    double acc = 0;
+    constexpr int kBatch = 16;  // loads issued ahead of the ordered additions
+    int b = 0;
+    for (; b + kBatch <= nblocks; b += kBatch) {
+        double v[kBatch];
 #pragma unroll
-                for (int i = 0; i < kBatch; ++i) v[i] = src[(long long)(b + i) * H];
-            }
-            for (; b + kBatch <= nblocks; b += kBatch) {
-                // the next batch's loads are issued before this batch is added
-                double w[kBatch];
-                const bool more = b + 2 * kBatch <= nblocks;
-                if (more) {
+        for (int i = 0; i < kBatch; ++i) v[i] = src[(long long)(b + i) * H];
 #pragma unroll
-                    for (int i = 0; i < kBatch; ++i) w[i] = src[(long long)(b + kBatch + i) * H];
-                }
-#pragma unroll
-                for (int i = 0; i < kBatch; ++i) acc += v[i];
-                if (more) {
-#pragma unroll
-                    for (int i = 0; i < kBatch; ++i) v[i] = w[i];
-                }
-            }
-            for (; b < nblocks; ++b) acc += src[(long long)b * H];
-        }
-        out[(long long)var * H + h] = acc;
+        for (int i = 0; i < kBatch; ++i) acc += v[i];
     }
+    for (; b < nblocks; ++b) acc += src[(long long)b * H];
+    sums[t] = acc;
 }
 
 template <class T>
@@ -555,22 +543,26 @@ int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, dou
     HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d_part.p, 0, sizeof(double) * npart));
     DeviceArray<double> d_sum;
-    HIP_TRY(d_sum.alloc((size_t)e->nvar * H));
+    const long long nsum = (long long)m * e->nvar * H;
+    HIP_TRY(d_sum.alloc((size_t)nsum));
     launch(dim3(nblocks, e->nvar, m), d_bases.p, geom_of(e, D), d_part.p);
     hipError_t le = hipGetLastError();
-    // fixed-order reduction on the device: for each (variable, h), samples
-    // in list order, blocks in index order
+    // fixed-order reduction: each sample's blocks in index order on the
+    // device, then the samples in list order on the host
     if (le == hipSuccess) {
-        hipLaunchKernelGGL(sum_partials_kernel, dim3(e->nvar), dim3(64), 0, e->stream, d_part.p, m,
-                           e->nvar, nblocks, H, d_sum.p);
+        hipLaunchKernelGGL(sample_partials_kernel, dim3(unsigned((nsum + 255) / 256)), dim3(256), 0,
+                           e->stream, d_part.p, nsum, nblocks, H, d_sum.p);
         le = hipGetLastError();
     }
     hipError_t se = hipStreamSynchronize(e->stream);
     if (le == hipSuccess) le = se;
-    std::vector<double> acc((size_t)e->nvar * H);
-    hipError_t ce = hipMemcpy(acc.data(), d_sum.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost);
+    std::vector<double> sums((size_t)nsum);
+    hipError_t ce = hipMemcpy(sums.data(), d_sum.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost);
     if (le != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(le));
     if (ce != hipSuccess) return fail(ALSVINN_HIP_E_HIP, hipGetErrorString(ce));
+    std::vector<double> acc((size_t)e->nvar * H, 0.0);
+    for (int k = 0; k < m; ++k)
+        for (size_t i = 0; i < acc.size(); ++i) acc[i] += sums[(size_t)k * acc.size() + i];
     const double N = double(ncell);
     for (int var = 0; var < e->nvar; ++var)
         for (int h = 0; h < H; ++h)
