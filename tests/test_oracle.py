@@ -50,24 +50,225 @@ def test_sampler_moments_converge():
 
 # ---------------------------------------------------------------- fluxes
 
+def np_point_flux(q, d, gamma=GAMMA):
+    """Euler flux through a face of normal e_d, written from the primitive
+    variables in extended precision (Toro, Riemann Solvers, 3rd ed., eq. 3.60):
+    (rho u_d, rho u u_d + p e_d, (E + p) u_d) with p = (gamma-1)(E - rho|u|^2/2).
+    Independent of the oracle's Euler::pointFlux, which it pins."""
+    ld = np.longdouble
+    q = np.asarray(q, dtype=ld)
+    rho, E = q[0], q[-1]
+    u = q[1:-1] / rho
+    p = (ld(gamma) - 1) * (E - ld(0.5) * rho * np.dot(u, u))
+    F = np.empty(len(q), dtype=ld)
+    F[0] = rho * u[d]
+    F[1:-1] = rho * u * u[d]
+    F[1 + d] += p
+    F[-1] = (E + p) * u[d]
+    return F
+
+
+def _flux_scale(*arrays):
+    return max(float(np.max(np.abs(np.asarray(a, dtype=float)))) for a in arrays)
+
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+def test_point_flux_textbook(nsd):
+    """Euler.hpp:172-181 computePointFlux: the oracle's point flux equals the
+    textbook Euler flux (np_point_flux) in every direction, transverse
+    momentum components included."""
+    rng = np.random.default_rng(100 + nsd)
+    for _ in range(20):
+        rho, p = rng.uniform(0.2, 3.0, 2)
+        u = rng.uniform(-2, 2, nsd)
+        q = euler_state(rho, list(u), p)
+        for d in range(nsd):
+            f = O.point_flux(nsd, d, GAMMA, q)
+            expect = np_point_flux(q, d)
+            np.testing.assert_allclose(f, expect.astype(float), rtol=1e-13,
+                                       atol=1e-14 * _flux_scale(expect))
+
+
 @pytest.mark.parametrize("nsd", [1, 2, 3])
 @pytest.mark.parametrize("flux", [O.FLUX_HLL, O.FLUX_HLL3])
 def test_flux_consistency(nsd, flux):
-    """HLLTest.cpp:24-71: F(U, U) equals the point flux f(U) exactly, in
-    every direction (for states where the point-flux branch is taken and for
-    subsonic states where the HLL average of identical states reduces to f)."""
+    """HLLTest.cpp:24-71: F(U, U) equals the point flux f(U) in every
+    direction (for states where the point-flux branch is taken and for
+    subsonic states where the HLL average of identical states reduces to f);
+    f is the independent textbook flux."""
     states = [euler_state(1.0, [2.0, 3.0, 4.0][:nsd], 6.0),
               euler_state(0.7, [0.1, -0.2, 0.05][:nsd], 1.3),
               euler_state(2.0, [-3.5, 0.5, 1.0][:nsd], 0.4)]
     for q in states:
         for d in range(nsd):
             F, ws = O.face_flux(nsd, flux, d, GAMMA, q, q)
-            f = O.point_flux(nsd, d, GAMMA, q)
+            f = np_point_flux(q, d)
             u = q[1 + d] / q[0]
             p = (GAMMA - 1) * (q[-1] - 0.5 * np.dot(q[1:-1], q[1:-1]) / q[0])
             c = np.sqrt(GAMMA * p / q[0])
-            np.testing.assert_allclose(F, f, rtol=1e-13, atol=1e-13)
+            np.testing.assert_allclose(F, f.astype(float), rtol=1e-13,
+                                       atol=1e-14 * _flux_scale(f))
             assert ws == pytest.approx(abs(u) + c, rel=1e-14)
+
+
+def hllc_toro(qL, qR, d, gamma=GAMMA):
+    """HLLC flux of Toro, Riemann Solvers, 3rd ed., section 10.4, evaluated
+    in extended precision, with the reference's wave-speed estimates
+    (HLL3.hpp:118-146: Roe-type averages with weights sqrt(rho), the fast
+    speeds widened by 0.5*max(0, uL_d - uR_d)).
+
+    Contact speed S* from Toro eq. 10.37; star states from eq. 10.39; star
+    fluxes F*K = FK + SK (U*K - UK), eq. 10.38; the four-region selection of
+    eq. 10.26. The one departure from the textbook is the reference's: the
+    tangential velocity in both star states is that of the HLL intermediate
+    state U_hll = (SR UR - SL UL + FL - FR)/(SR - SL) (Toro eq. 10.13) instead
+    of each side's own; HLL3.hpp:80-82 forms it as (fR.m - fL.m - sR mR +
+    sL mL)/aa. Returns (F, ws, info) with the branch facts the tests assert."""
+    ld = np.longdouble
+    g = ld(gamma)
+    qL = np.asarray(qL, dtype=ld)
+    qR = np.asarray(qR, dtype=ld)
+
+    def prim(q):
+        rho = q[0]
+        u = q[1:-1] / rho
+        p = (g - 1) * (q[-1] - ld(0.5) * rho * np.dot(u, u))
+        return rho, u, p
+
+    rL, uL, pL = prim(qL)
+    rR, uR, pR = prim(qR)
+    wL, wR = np.sqrt(rL), np.sqrt(rR)
+    ubar = (wL * uL + wR * uR) / (wL + wR)
+    pbar = (wL * pL + wR * pR) / (wL + wR)
+    cbar = np.sqrt(g * pbar / ((rL + rR) / 2))
+    corr = ld(0.5) * max(ld(0), uL[d] - uR[d])
+    SL = min(uL[d] + corr - np.sqrt(g * pL / rL), ubar[d] - cbar)
+    SR = max(uR[d] - corr + np.sqrt(g * pR / rR), ubar[d] + cbar)
+    FL, FR = np_point_flux(qL, d, gamma), np_point_flux(qR, d, gamma)
+    Sstar = ((pR - pL + rL * uL[d] * (SL - uL[d]) - rR * uR[d] * (SR - uR[d]))
+             / (rL * (SL - uL[d]) - rR * (SR - uR[d])))
+    Uhll = (SR * qR - SL * qL + FL - FR) / (SR - SL)
+    vt = Uhll[1:-1] / Uhll[0]
+
+    def star(q, rho, u, p, S):
+        k = rho * (S - u[d]) / (S - Sstar)
+        vel = vt.copy()
+        vel[d] = Sstar
+        U = np.empty(len(q), dtype=ld)
+        U[0] = k
+        U[1:-1] = k * vel
+        U[-1] = k * (q[-1] / rho + (Sstar - u[d]) * (Sstar + p / (rho * (S - u[d]))))
+        return U
+
+    UsL = star(qL, rL, uL, pL, SL)
+    UsR = star(qR, rR, uR, pR, SR)
+    # the two star states average to the HLL state (Toro eq. 10.34's
+    # integral condition): a check of this function, not of the oracle
+    np.testing.assert_allclose((SR - Sstar) * UsR + (Sstar - SL) * UsL,
+                               (SR - SL) * Uhll, rtol=1e-15, atol=1e-15)
+    if SL >= 0:
+        F = FL
+    elif Sstar >= 0:
+        F = FL + SL * (UsL - qL)
+    elif SR > 0:
+        F = FR + SR * (UsR - qR)
+    else:
+        F = FR
+    info = {"subsonic": SL < 0 < SR, "sstar_nonneg": Sstar >= 0, "corr": corr,
+            "scale": _flux_scale(FL, FR, SL * qL, SR * qR)}
+    return F, max(abs(SL), abs(SR)), info
+
+
+# (rho, u_normal, u_t1, u_t2, p) of the left and right states; for face
+# direction d the normal velocity goes to component d and the tangential
+# ones fill the other components in order
+HLL3_PAIRS = [
+    ((1.0, 0.3, 0.2, -0.1, 1.0), (0.125, -0.1, -0.3, 0.4, 0.1)),     # uL > uR, S* > 0
+    ((0.125, 0.1, 0.5, 0.2, 0.1), (1.0, -0.3, -0.2, 0.3, 1.0)),      # uL > uR, S* < 0
+    ((1.0, -0.4, 0.7, 0.1, 0.8), (1.2, 0.5, -0.6, -0.2, 1.1)),       # uL < uR, corr = 0
+    ((2.0, 0.5, -0.5, 0.5, 2.5), (1.0, -0.5, 0.5, -0.5, 2.5)),       # shear + compression
+    ((0.6, 0.2, 0.0, -0.8, 0.4), (1.7, 0.25, 0.9, 0.0, 1.9)),        # pR > pL, S* < 0
+    ((1.5, 0.9, 0.3, 0.3, 1.2), (0.9, 0.1, -0.1, 0.6, 0.6)),         # strong compression, S* > 0
+    ((0.8, -0.6, 0.4, -0.3, 0.9), (0.5, -0.9, 0.2, 0.7, 0.3)),       # leftward flow, uL > uR
+]
+
+
+def _directed_state(spec, nsd, d):
+    rho, un, t1, t2, p = spec
+    vel = [0.0] * nsd
+    vel[d] = un
+    tang = iter([t1, t2])
+    for k in range(nsd):
+        if k != d:
+            vel[k] = next(tang)
+    return euler_state(rho, vel, p)
+
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+def test_hll3_hllc_textbook(nsd):
+    """HLL3.hpp:52-148 on distinct subsonic states equals the HLLC flux of
+    Toro section 10.4 (hllc_toro, extended precision) in every direction of
+    1D, 2D and 3D, with both signs of the contact speed, the wave-speed
+    correction active (uL_d > uR_d) and inactive, and non-zero tangential
+    momentum in every direction. A sign error in the correction, the left
+    star quantities in the right star branch or max(0, uR - uL) moves the
+    flux by O(0.1), far outside the tolerance."""
+    seen = {"pos": 0, "neg": 0, "corr": 0, "nocorr": 0}
+    for d in range(nsd):
+        for specL, specR in HLL3_PAIRS:
+            qL = _directed_state(specL, nsd, d)
+            qR = _directed_state(specR, nsd, d)
+            F, ws = O.face_flux(nsd, O.FLUX_HLL3, d, GAMMA, qL, qR)
+            expect, ws_expect, info = hllc_toro(qL, qR, d)
+            assert info["subsonic"]
+            seen["pos" if info["sstar_nonneg"] else "neg"] += 1
+            seen["corr" if info["corr"] > 0 else "nocorr"] += 1
+            np.testing.assert_allclose(F, expect.astype(float), rtol=1e-12,
+                                       atol=1e-13 * info["scale"])
+            assert ws == pytest.approx(float(ws_expect), rel=1e-14)
+    assert min(seen.values()) >= 2 * nsd, seen
+
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+@pytest.mark.parametrize("flux", [O.FLUX_HLL, O.FLUX_HLL3])
+def test_flux_mirror_symmetry(nsd, flux):
+    """Reflection invariance of the Euler equations: mirroring the axis d
+    (normal momentum negated) and swapping the sides negates the mass,
+    tangential-momentum and energy fluxes and keeps the normal-momentum flux
+    and the wave speed. The two star branches of HLL3 map onto each other,
+    so this checks the right-hand branch against the left-hand one."""
+    for d in range(nsd):
+        for specL, specR in HLL3_PAIRS:
+            qL = _directed_state(specL, nsd, d)
+            qR = _directed_state(specR, nsd, d)
+            mL, mR = qL.copy(), qR.copy()
+            mL[1 + d] = -mL[1 + d]
+            mR[1 + d] = -mR[1 + d]
+            F, ws = O.face_flux(nsd, flux, d, GAMMA, qL, qR)
+            Fm, wsm = O.face_flux(nsd, flux, d, GAMMA, mR, mL)
+            sign = -np.ones(len(F))
+            sign[1 + d] = 1
+            scale = _flux_scale(F, Fm, ws * qL, ws * qR)
+            np.testing.assert_allclose(F, sign * Fm, rtol=1e-12, atol=1e-13 * scale)
+            assert ws == pytest.approx(wsm, rel=1e-14)
+
+
+@pytest.mark.parametrize("flux", [O.FLUX_HLL, O.FLUX_HLL3])
+def test_flux_rotation_invariance(flux):
+    """Rotational invariance: the flux in direction d of a 3D state pair is
+    the direction-0 flux of the pair with axes d and 0 exchanged, with the
+    momentum components of the result exchanged back."""
+    for specL, specR in HLL3_PAIRS:
+        qL0 = _directed_state(specL, 3, 0)
+        qR0 = _directed_state(specR, 3, 0)
+        F0, ws0 = O.face_flux(3, flux, 0, GAMMA, qL0, qR0)
+        for d in (1, 2):
+            perm = [0, 1, 2, 3, 4]
+            perm[1], perm[1 + d] = perm[1 + d], perm[1]
+            F, ws = O.face_flux(3, flux, d, GAMMA, qL0[perm], qR0[perm])
+            scale = _flux_scale(F0, ws0 * qL0, ws0 * qR0)
+            np.testing.assert_allclose(F[perm], F0, rtol=1e-13, atol=1e-14 * scale)
+            assert ws == ws0
 
 
 def test_godunov_closed_form():
@@ -87,11 +288,11 @@ def test_hll_supersonic_upwinding():
     R = euler_state(0.5, [4.0], 0.8)
     for flux in (O.FLUX_HLL, O.FLUX_HLL3):
         F, _ = O.face_flux(1, flux, 0, GAMMA, L, R)
-        np.testing.assert_array_equal(F, O.point_flux(1, 0, GAMMA, L))
+        np.testing.assert_allclose(F, np_point_flux(L, 0).astype(float), rtol=1e-14)
         L2 = euler_state(1.0, [-5.0], 1.0)
         R2 = euler_state(0.5, [-6.0], 0.8)
         F2, _ = O.face_flux(1, flux, 0, GAMMA, L2, R2)
-        np.testing.assert_array_equal(F2, O.point_flux(1, 0, GAMMA, R2))
+        np.testing.assert_allclose(F2, np_point_flux(R2, 0).astype(float), rtol=1e-14)
 
 
 def test_hll_textbook_formula():
@@ -418,11 +619,11 @@ def test_burgers_fluxes_known_answers():
 
 @pytest.mark.parametrize("nsd", [1, 2, 3])
 def test_euler_central_flux(nsd):
-    """Central.hpp: F = 0.5*(f(L) + f(R)) with f the Euler point flux (pinned
-    by the HLL consistency tests), wave speed max(|u_d| + sqrt(gamma p/rho));
-    F(U, U) = f(U)."""
+    """Central.hpp: F = 0.5*(f(L) + f(R)) with f the textbook Euler flux
+    (np_point_flux), wave speed max(|u_d| + sqrt(gamma p/rho)); F(U, U) = f(U)."""
     rng = np.random.default_rng(nsd)
     g = 1.4
+    ld = np.longdouble
     for d in range(nsd):
         for _ in range(5):
             rho, p = rng.uniform(0.5, 2, 2)
@@ -430,14 +631,16 @@ def test_euler_central_flux(nsd):
             qL = euler_state(1.3 * rho, list(u[0]), p)
             qR = euler_state(rho, list(u[1]), 0.7 * p)
             F, ws = O.face_flux(nsd, O.FLUX_CENTRAL, d, g, qL, qR)
-            fL = O.point_flux(nsd, d, g, qL)
-            fR = O.point_flux(nsd, d, g, qR)
-            np.testing.assert_array_equal(F, 0.5 * (fL + fR))
+            fL = np_point_flux(qL, d, g)
+            fR = np_point_flux(qR, d, g)
+            expect = (ld(0.5) * (fL + fR)).astype(float)
+            np.testing.assert_allclose(F, expect, rtol=1e-13, atol=1e-14 * _flux_scale(fL, fR))
             cL = abs(u[0][d]) + np.sqrt(g * p / (1.3 * rho))
             cR = abs(u[1][d]) + np.sqrt(g * (0.7 * p) / rho)
             assert ws == pytest.approx(max(cL, cR), rel=1e-14)
             F2, _ = O.face_flux(nsd, O.FLUX_CENTRAL, d, g, qL, qL)
-            np.testing.assert_array_equal(F2, fL)
+            np.testing.assert_allclose(F2, fL.astype(float), rtol=1e-13,
+                                       atol=1e-14 * _flux_scale(fL))
 
 
 # ---------------------------------------------------------------- operator
