@@ -61,7 +61,23 @@ EXPORTS = [
     "alsvinn_hip_stats_structure_cube",
     "alsvinn_hip_stats_structure_basic",
     "alsvinn_hip_ensemble_totals",
+    "alsvinn_hip_operator_create",
+    "alsvinn_hip_operator_destroy",
+    "alsvinn_hip_numflux_compute",
+    "alsvinn_hip_reconstruction_compute",
+    "alsvinn_hip_boundary_apply",
+    "alsvinn_hip_statistics_create",
+    "alsvinn_hip_statistics_destroy",
+    "alsvinn_hip_statistics_compute",
+    "alsvinn_hip_statistics_combine",
+    "alsvinn_hip_statistics_finalize",
+    "alsvinn_hip_statistics_reset",
+    "alsvinn_hip_statistics_outputs",
+    "alsvinn_hip_statistics_result",
+    "alsvinn_hip_statistics_download",
 ]
+
+STAT_MEANVAR, STAT_ONEPOINTMOMENT, STAT_STRUCTURE_CUBE, STAT_STRUCTURE_BASIC, STAT_BV = 0, 1, 2, 3, 4
 
 
 class Scheme(ctypes.Structure):
@@ -89,6 +105,21 @@ class Sample(ctypes.Structure):
         ("interval", ctypes.c_double * 2),
         ("saved0", ctypes.c_longlong * 2),
         ("max_steps", ctypes.c_longlong),
+    ]
+
+
+class Volume(ctypes.Structure):
+    """alsvinn_hip_volume: a reference Volume as per-variable device pointers."""
+    _fields_ = [
+        ("nvar", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("gx", ctypes.c_int),
+        ("gy", ctypes.c_int),
+        ("gz", ctypes.c_int),
+        ("var", ctypes.c_void_p * 5),
     ]
 
 
@@ -159,6 +190,25 @@ def load():
     L.alsvinn_hip_stats_structure_basic.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_double, ctypes.c_int, dp]
     L.alsvinn_hip_ensemble_totals.argtypes = [vp, dp]
+    volp = ctypes.POINTER(Volume)
+    L.alsvinn_hip_operator_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, vp, ctypes.POINTER(vp)]
+    L.alsvinn_hip_operator_destroy.argtypes = [vp]
+    L.alsvinn_hip_numflux_compute.argtypes = [vp, volp, dp, ctypes.c_int, volp, ip, ip]
+    L.alsvinn_hip_reconstruction_compute.argtypes = [vp, volp, ctypes.c_int, ctypes.c_int, volp,
+                                                     volp, ip, ip]
+    L.alsvinn_hip_boundary_apply.argtypes = [ctypes.c_int, volp, vp]
+    L.alsvinn_hip_statistics_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, volp, vp,
+                                                ctypes.POINTER(vp)]
+    L.alsvinn_hip_statistics_destroy.argtypes = [vp]
+    L.alsvinn_hip_statistics_compute.argtypes = [vp, volp]
+    L.alsvinn_hip_statistics_combine.argtypes = [vp, ctypes.c_longlong]
+    L.alsvinn_hip_statistics_finalize.argtypes = [vp]
+    L.alsvinn_hip_statistics_reset.argtypes = [vp]
+    L.alsvinn_hip_statistics_outputs.argtypes = [vp, ip, llp]
+    L.alsvinn_hip_statistics_result.argtypes = [vp, ctypes.c_int, ctypes.POINTER(dp)]
+    L.alsvinn_hip_statistics_download.argtypes = [vp, ctypes.c_int, dp]
     _lib = L
     return L
 

@@ -15,14 +15,15 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libalsvinn_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = [
-    "ensemble.hip",
-    "kernels_burgers.hip",
-    "kernels_euler1.hip",
-    "kernels_euler2.hip",
-    "kernels_euler3.hip",
-    "stats.hip",
-]
+# (source, object, extra defines): the equation units are compiled once per
+# state precision, which halves the longest compile and runs the two halves
+# in parallel; the longest jobs are listed first.
+SOURCES = []
+for _eq in ("euler3", "euler2", "burgers", "euler1"):
+    SOURCES.append(("kernels_%s.hip" % _eq, "kernels_%s_f64" % _eq, ["-DALSVINN_HIP_FP64_ONLY"]))
+    SOURCES.append(("kernels_%s.hip" % _eq, "kernels_%s_f32" % _eq, ["-DALSVINN_HIP_FP32_ONLY"]))
+SOURCES += [("ensemble.hip", "ensemble", []), ("operators.hip", "operators", []),
+            ("stats.hip", "stats", [])]
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -36,9 +37,10 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(job):
+    src, name, defines = job
+    obj = os.path.join(BUILD, name + ".o")
+    cmd = [HIPCC] + FLAGS + defines + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))

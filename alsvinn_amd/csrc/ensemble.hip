@@ -8,6 +8,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -81,7 +82,7 @@ __global__ void count_active_kernel(const SampleState* __restrict__ state, int n
     } while (0)
 
 
-namespace {
+namespace alsvinn_hip {
 
 int validate(const alsvinn_hip_scheme* s) {
     if (!s) return fail(ALSVINN_HIP_E_INVALID, "scheme is NULL");
@@ -115,6 +116,10 @@ int ghost_cells(const alsvinn_hip_scheme& s) {
     if (s.reconstruction == ALSVINN_HIP_RECON_WENO3) return 3;
     return 2;
 }
+
+}  // namespace alsvinn_hip
+
+namespace {
 
 template <class T>
 void* buf_at(void* base, long long off) {
@@ -861,10 +866,47 @@ int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* varia
     for (const auto& D : e->desc) maxny = std::max(maxny, e->dim == 3 ? D.nz : D.ny);
     std::vector<int> cand;
     if (e->dim >= 2) {
+        // powers-of-two halvings of the tallest grid, plus the band heights
+        // that fill the GPU's wave slots: for w = 1..3 waves per SIMD and
+        // one or two dispatch rounds, the lowest band whose work list still
+        // fits in w * SIMDs * rounds waves (an almost full last round, where
+        // a power of two can leave the second round nearly empty); grids too
+        // small to fill the GPU with 8-row bands also try 1..7 rows
+        std::set<int> c;
         for (int r = maxny; r >= 8; r = (r + 1) / 2) {
-            cand.push_back(r);
+            c.insert(r);
             if (r == 8) break;
         }
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const long long simds = 4LL * std::max(1, cus);
+        const int saved = e->band_rows;
+        auto waves_of = [&](int h) {
+            std::vector<WaveTask> t;
+            e->band_rows = h;
+            build_tasks(e, t);
+            return (long long)t.size() * (e->dim == 3 ? kGroupRows : 1);
+        };
+        for (int w = 1; w <= 3; ++w)
+            for (int rounds = 1; rounds <= 2; ++rounds) {
+                const long long slots = simds * w * rounds;
+                if (waves_of(maxny) > slots) {
+                    int lo = 1, hi = maxny;  // waves_of decreases with h: smallest h that fits
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (waves_of(mid) <= slots)
+                            hi = mid;
+                        else
+                            lo = mid + 1;
+                    }
+                    c.insert(lo);
+                }
+            }
+        if (waves_of(std::min(8, maxny)) < simds)
+            for (int h = 1; h < std::min(8, maxny); ++h) c.insert(h);
+        e->band_rows = saved;
+        cand.assign(c.rbegin(), c.rend());
     } else {
         cand.push_back(1);
     }

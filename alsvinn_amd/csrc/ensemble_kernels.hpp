@@ -137,4 +137,25 @@ __global__ __launch_bounds__(256) void ghost_fill_kernel(const SampleDesc* __res
     }
 }
 
+// Boundary::applyBoundaryConditions on one reference Volume (per-variable
+// memories; BoundaryCPU.cpp:30-121): the same ghost sources as
+// ghost_fill_kernel. D describes the volume (base 0).
+template <class T>
+__global__ __launch_bounds__(256) void ghost_fill_volume_kernel(PerVarVol<T> u, int nvar,
+                                                               SampleDesc D) {
+    const long long n = D.vstride;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int px = int(i % D.tx);
+        const int py = int((i / D.tx) % D.ty);
+        const int pz = int(i / ((long long)D.tx * D.ty));
+        const int sx = ghost_source(px, D.nx, D.gx, D.bc);
+        const int sy = ghost_source(py, D.ny, D.gy, D.bc);
+        const int sz = ghost_source(pz, D.nz, D.gz, D.bc);
+        if (sx == px && sy == py && sz == pz) continue;
+        const long long j = ((long long)sz * D.ty + sy) * D.tx + sx;
+        for (int v = 0; v < nvar; ++v) u(v, i) = u(v, j);
+    }
+}
+
 }  // namespace alsvinn_hip

@@ -13,6 +13,11 @@
 namespace alsvinn_hip {
 
 int fail(int code, const std::string& msg);
+// Checks a scheme's option values and their combination (0 = valid).
+int validate(const alsvinn_hip_scheme* s);
+// Ghost layers of a scheme: max(flux stencil - 1, reconstruction ghosts)
+// (NumericalFluxCPU.cpp:226-231).
+int ghost_cells(const alsvinn_hip_scheme& s);
 
 // Launchers of the kernels one scheme (equation, reconstruction, flux, grid
 // dimension, precision) needs.
@@ -31,6 +36,12 @@ struct KernelSet {
     void (*rk4)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                 const unsigned long long* ws, int kind, const T* X, const T* Y, const T* Z,
                 const T* W, const T* V, T* out, int check, double gamma) = nullptr;
+    // volume operators: NumericalFlux::computeFlux on a reference Volume
+    // (the first-stage kernel on per-variable memories) and
+    // Reconstruction::performReconstruction
+    void (*op_flux)(const StageArgs<T>& a, int blocks, hipStream_t s) = nullptr;
+    void (*op_recon)(const ReconArgs<T>& a, int blocks, hipStream_t s) = nullptr;
+    int recon_radius = 1;
 };
 
 // Each stage kind (first stage, RK2 second stage, SSP-RK3 second and third
@@ -92,8 +103,33 @@ void launch_stage_wide(const StageArgs<T>& a, int blocks, hipStream_t s) {
     }
 }
 
+// computeFlux on a reference Volume: the first-stage kernel with per-variable
+// memories, writing only the cells of the region (2 waves/SIMD budget).
+template <class T, class Eq, class Recon, class Flux, int DIM>
+void launch_op_flux(const StageArgs<T>& a, int blocks, hipStream_t s) {
+    if (blocks <= 0) return;
+    if constexpr (recon_radius<Recon>::value > 1)
+        hipLaunchKernelGGL((stage_kernel_wide<T, Eq, Recon, Flux, DIM, 2, MODE_RHS, 1>), dim3(blocks),
+                           dim3(256), 0, s, a);
+    else if constexpr (DIM == 3)
+        hipLaunchKernelGGL((stage_kernel_3<T, Eq, Recon, Flux, 2, MODE_RHS, 1>), dim3(a.ntasks),
+                           dim3(64 * kGroupRows), 0, s, a);
+    else
+        hipLaunchKernelGGL((stage_kernel_12<T, Eq, Recon, Flux, DIM, 2, MODE_RHS, 1>), dim3(blocks),
+                           dim3(256), 0, s, a);
+}
+
+template <class T, class Eq, class Recon>
+void launch_op_recon(const ReconArgs<T>& a, int blocks, hipStream_t s) {
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL((recon_volume_kernel<T, Eq, Recon>), dim3(blocks), dim3(256), 0, s, a);
+}
+
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
+    k.op_flux = &launch_op_flux<T, Eq, Recon, Flux, DIM>;
+    k.op_recon = &launch_op_recon<T, Eq, Recon>;
+    k.recon_radius = recon_radius<Recon>::value;
     if constexpr (recon_radius<Recon>::value > 1) {
         // wide kernels wait on their cached stencil loads: the autotuner
         // also tries the tighter register budgets for HLL3 (the Euler

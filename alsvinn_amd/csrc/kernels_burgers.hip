@@ -1,6 +1,9 @@
 // Kernel instantiations for the Burgers equation (Godunov flux), 1D and 2D.
 #include "internal.hpp"
 
+// Built twice (alsvinn_amd/build.py): once with ALSVINN_HIP_FP64_ONLY for the
+// FP64-state kernels and once with ALSVINN_HIP_FP32_ONLY for the FP32-state ones.
+
 namespace alsvinn_hip {
 
 template <class T>
@@ -24,7 +27,11 @@ int select_burgers(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
     return fail(ALSVINN_HIP_E_INVALID, "flux not available for burgers");
 }
 
+#ifndef ALSVINN_HIP_FP32_ONLY
 template int select_burgers<double>(const alsvinn_hip_scheme&, int, KernelSet<double>&);
+#endif
+#ifndef ALSVINN_HIP_FP64_ONLY
 template int select_burgers<float>(const alsvinn_hip_scheme&, int, KernelSet<float>&);
+#endif
 
 }  // namespace alsvinn_hip

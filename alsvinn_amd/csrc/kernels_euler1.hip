@@ -1,6 +1,9 @@
 // Kernel instantiations for the 1D Euler equations (HLL, HLL3).
 #include "internal.hpp"
 
+// Built twice (alsvinn_amd/build.py): once with ALSVINN_HIP_FP64_ONLY for the
+// FP64-state kernels and once with ALSVINN_HIP_FP32_ONLY for the FP32-state ones.
+
 namespace alsvinn_hip {
 
 template <class T>
@@ -11,7 +14,11 @@ int select_euler1(const alsvinn_hip_scheme& sc, int dim, KernelSet<T>& k) {
     return select_recon<T, EulerEq<T, 1>, FluxHLL3, 1>(sc, k);
 }
 
+#ifndef ALSVINN_HIP_FP32_ONLY
 template int select_euler1<double>(const alsvinn_hip_scheme&, int, KernelSet<double>&);
+#endif
+#ifndef ALSVINN_HIP_FP64_ONLY
 template int select_euler1<float>(const alsvinn_hip_scheme&, int, KernelSet<float>&);
+#endif
 
 }  // namespace alsvinn_hip

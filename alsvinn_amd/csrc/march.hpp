@@ -90,7 +90,52 @@ struct StageArgs {
     const T* in2;  // U0
     T* out;
     double gamma;
+    // volume operators (OP kernels): one device pointer per conserved
+    // variable of the input and output volumes (a reference Volume's
+    // per-variable memories), used instead of in / out
+    const T* vin[5];
+    T* vout[5];
 };
+
+// Memory of one volume: variable v of padded cell i. The ensemble's buffers
+// hold a sample's variables vstride elements apart; a reference Volume has
+// one memory per variable (Volume.cpp:44-60).
+template <class T>
+struct StridedVol {
+    T* p;
+    long long vs;
+    __device__ __forceinline__ T& operator()(int v, long long i) const { return p[i + v * vs]; }
+};
+template <class T>
+struct PerVarVol {
+    T* p[5];
+    __device__ __forceinline__ T& operator()(int v, long long i) const { return p[v][i]; }
+};
+
+// Input, second input and output accessors of a stage kernel for sample D:
+// OP = 0 the ensemble's buffers, OP = 1 the volume operators' pointers.
+template <int OP, class T>
+__device__ __forceinline__ auto stage_in(const StageArgs<T>& a, const SampleDesc& D, bool second) {
+    if constexpr (OP == 0) {
+        return StridedVol<const T>{(second ? a.in2 : a.in) + D.base, D.vstride};
+    } else {
+        PerVarVol<const T> r;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) r.p[v] = a.vin[v];
+        return r;
+    }
+}
+template <int OP, class T>
+__device__ __forceinline__ auto stage_out(const StageArgs<T>& a, const SampleDesc& D) {
+    if constexpr (OP == 0) {
+        return StridedVol<T>{a.out + D.base, D.vstride};
+    } else {
+        PerVarVol<T> r;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) r.p[v] = a.vout[v];
+        return r;
+    }
+}
 
 // Integrator::computeTimestep + FixedInterval adjusters (Integrator.cpp:20-66,
 // FixedIntervalWriter.cpp:45-53, FixedIntervalStatistics.cpp:33-41), in FP64.
@@ -155,16 +200,31 @@ __device__ __forceinline__ T segment_max(T v, int seg) {
     return v;
 }
 
-// Stage input at flat index i (variable stride vs): plain or lazy U1 = L0*c + U0.
+// Stage input at flat index i (variable stride vs): plain or lazy
+// U1 = L0*c + U0.
 template <class T, int NV>
-__device__ __forceinline__ void load_cell(const T* __restrict__ in, const T* __restrict__ in2,
-                                          bool lazy, T c, long long i, long long vs, T* q) {
+__device__ __forceinline__ void load_strided(const T* __restrict__ in, const T* __restrict__ in2,
+                                             bool lazy, T c, long long i, long long vs, T* q) {
     if (lazy) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = in[i + v * vs] * c + in2[i + v * vs];
     } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = in[i + v * vs];
+    }
+}
+
+template <class T, int NV, class Acc>
+__device__ __forceinline__ void load_cell(const Acc& in, const Acc& in2, bool lazy, T c,
+                                          long long i, T* q) {
+    if constexpr (std::is_same_v<Acc, StridedVol<const T>>) {
+        load_strided<T, NV>(in.p, in2.p, lazy, c, i, in.vs, q);
+    } else if (lazy) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = in(v, i) * c + in2(v, i);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = in(v, i);
     }
 }
 
@@ -175,13 +235,27 @@ __device__ __forceinline__ void load_cell(const T* __restrict__ in, const T* __r
 template <class T, int NV>
 struct PendingCell {
     T a[NV], b[NV];
-    __device__ __forceinline__ void issue(const T* __restrict__ in, const T* __restrict__ in2,
-                                          bool lazy, long long i, long long vs) {
+    __device__ __forceinline__ void issue_strided(const T* __restrict__ in,
+                                                  const T* __restrict__ in2, bool lazy,
+                                                  long long i, long long vs) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a[v] = in[i + v * vs];
         if (lazy) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) b[v] = in2[i + v * vs];
+        }
+    }
+    template <class Acc>
+    __device__ __forceinline__ void issue(const Acc& in, const Acc& in2, bool lazy, long long i) {
+        if constexpr (std::is_same_v<Acc, StridedVol<const T>>) {
+            issue_strided(in.p, in2.p, lazy, i, in.vs);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) a[v] = in(v, i);
+            if (lazy) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) b[v] = in2(v, i);
+            }
         }
     }
     __device__ __forceinline__ void finish(bool lazy, T c, T* q) const {
@@ -243,11 +317,18 @@ __device__ __forceinline__ void axis_images(int c, int N, int G, int bc, int& a,
     }
 }
 
-// Writes one result cell to its padded position and to every ghost cell
-// that mirrors it (at most one image per axis side).
-template <class T, int NV, int DIM>
-__device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDesc& D, int col,
-                                           int row, int plane, const T* res) {
+// Writes one result cell to its padded position and, with IMAGES, to every
+// ghost cell that mirrors it (at most one image per axis side). The volume
+// operators (computeFlux) write the cell only, as the reference does.
+template <class T, int NV, int DIM, bool IMAGES = true, class Acc>
+__device__ __forceinline__ void store_cell(const Acc& out, const SampleDesc& D, int col, int row,
+                                           int plane, const T* res) {
+    if constexpr (!IMAGES) {
+        const long long o = ((long long)(plane + D.gz) * D.ty + (row + D.gy)) * D.tx + (col + D.gx);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out(v, o) = res[v];
+        return;
+    }
     int x[3], y[3], z[3];
     axis_images(col, D.nx, D.gx, D.bc, x[0], x[1], x[2]);
     y[0] = 0;
@@ -267,7 +348,7 @@ __device__ __forceinline__ void store_cell(T* out, long long vs, const SampleDes
                 if (x[ix] < 0) continue;
                 const long long o = ((long long)z[iz] * D.ty + y[iy]) * D.tx + x[ix];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) out[o + v * vs] = res[v];
+                for (int v = 0; v < NV; ++v) out(v, o) = res[v];
             }
         }
     }
@@ -287,8 +368,10 @@ __device__ __forceinline__ int xcd_block() {
 
 // 1D and 2D stage kernel (DIM = active grid dimensions). WPS is the
 // register budget expressed as waves per SIMD (2: no spills for FP64 Euler
-// 2D; 3 and 4 trade spills for occupancy); the autotuner picks one.
-template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
+// 2D; 3 and 4 trade spills for occupancy); the autotuner picks one. OP = 1
+// is the volume operator (computeFlux on a reference Volume's per-variable
+// memories over a region; the result cells only, no ghost images).
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
@@ -322,10 +405,9 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         const bool interior = active && li >= G && li < G + ncols;
         const bool xface = active && li >= G - 1 && li < G + ncols;  // face col+1/2
         const int px = in_strip ? col + D.gx : D.gx;                 // padded column
-        const long long vs = D.vstride;
-        const T* in = a.in + D.base;
-        const T* in2 = a.in2 + D.base;
-        T* out = a.out + D.base;
+        const auto in = stage_in<OP>(a, D, false);
+        const auto in2 = stage_in<OP>(a, D, true);
+        const auto out = stage_out<OP>(a, D);
 
         T wsx = T(0), wsy = T(0);
 
@@ -347,10 +429,10 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         if constexpr (DIM >= 2) {
             const int pr = tk.y0 + gy;
             T qa[NV], ym[NV];
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 2), vs, qa);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), vs, ym);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), vs, yc);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), vs, yp);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 2), qa);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), ym);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), yc);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), yp);
             T Lt[NV], Rt[NV], L0[NV], R0[NV];
             if constexpr (PRIM) {
                 T Pa[NV], Pm[NV];
@@ -369,7 +451,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) Ry[v] = R0[v];
         } else {
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(0), vs, yc);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(0), yc);
             if constexpr (PRIM) Recon::template prim<T, NV>(yc, Pc, eq.gamma);
         }
 
@@ -378,14 +460,14 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             T qn[NV], Pn[NV];
             // next row first: its latency overlaps the x sweep
             PendingCell<T, NV> pn;
-            if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2), vs);
+            if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2));
             // U^n of this row for the stage combination, also issued before
             // the sweep (loaded after it, its latency would stall the wave)
             const long long i0 = (long long)pr * D.tx + px;
             T u0[NV];
             if (MODE != MODE_RHS && interior) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
             }
             T out_v[NV];
             // ---- x direction on row r (first: out = 0 + x divergence)
@@ -439,7 +521,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                 T res[NV];
                 combine<T, NV>(MODE, c, out_v, yc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, DIM>(out, vs, D, col, r, 0, res);
+                store_cell<T, NV, DIM, OP == 0>(out, D, col, r, 0, res);
             }
             if constexpr (DIM >= 2) {
 #pragma unroll
@@ -479,7 +561,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
 // instead of per row.
 constexpr int kGroupRows = 4;
 
-template <class T, class Eq, class Recon, class Flux, int WPS, int MODE>
+template <class T, class Eq, class Recon, class Flux, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
@@ -520,11 +602,10 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         const bool xface = active && row_in && li >= G - 1 && li < G + ncols;
         const int px = in_strip ? col + D.gx : D.gx;
         const int py = row + D.gy;
-        const long long vs = D.vstride;
         const long long plane_stride = (long long)D.tx * D.ty;
-        const T* in = a.in + D.base;
-        const T* in2 = a.in2 + D.base;
-        T* out = a.out + D.base;
+        const auto in = stage_in<OP>(a, D, false);
+        const auto in2 = stage_in<OP>(a, D, true);
+        const auto out = stage_out<OP>(a, D);
 
         T wsx = T(0), wsy = T(0), wsz = T(0);
         auto idx = [&](int pyy, int pz) -> long long {
@@ -541,10 +622,10 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         {
             const int pz = tk.z0 + D.gz;
             T qa[NV], zm[NV];
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 2), vs, qa);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 1), vs, zm);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), vs, zc);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), vs, zp);
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 2), qa);
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 1), zm);
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), zc);
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), zp);
             T Lt[NV], Rt[NV], L0[NV], R0[NV];
             Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt, eq.gamma);
             Recon::template apply<T, NV>(zm, zc, zp, L0, R0, eq.gamma);
@@ -558,16 +639,16 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             const int pz = zz + D.gz;
             T qn[NV];
             PendingCell<T, NV> pq, ph0, ph1;
-            pq.issue(in, in2, LAZY, idx(py, pz + 2), vs);
+            pq.issue(in, in2, LAZY, idx(py, pz + 2));
             if (lo_halo || hi_halo) {
-                ph0.issue(in, in2, LAZY, idx(h0, pz), vs);
-                ph1.issue(in, in2, LAZY, idx(h0 + 1, pz), vs);
+                ph0.issue(in, in2, LAZY, idx(h0, pz));
+                ph1.issue(in, in2, LAZY, idx(h0 + 1, pz));
             }
             const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
             T u0[NV];
             if (MODE != MODE_RHS && interior) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
             }
             T out_v[NV];
             {  // x
@@ -679,7 +760,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
                 T res[NV];
                 combine<T, NV>(MODE, c, out_v, zc, u0, res);
                 if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, 3>(out, vs, D, col, row, zz, res);
+                store_cell<T, NV, 3, OP == 0>(out, D, col, row, zz, res);
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -713,7 +794,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
 // once; in 3D the y faces of a row are evaluated by the row itself (twice
 // per face). The divergence is summed in the marching kernels' order:
 // ((0 + x part) + y part) + z part.
-template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE>
+template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr int NV = Eq::nvar;
@@ -747,10 +828,9 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
         const bool interior = active && li >= G && li < G + ncols;
         const bool xlane = active && li >= G - 1 && li <= G + ncols;  // reconstructs in x
         const bool xface = active && li >= G - 1 && li < G + ncols;   // face col+1/2
-        const long long vs = D.vstride;
-        const T* in = a.in + D.base;
-        const T* in2 = a.in2 + D.base;
-        T* out = a.out + D.base;
+        const auto in = stage_in<OP>(a, D, false);
+        const auto in2 = stage_in<OP>(a, D, true);
+        const auto out = stage_out<OP>(a, D);
         const long long sxd[3] = {1, (long long)D.tx, (long long)D.tx * D.ty};
         T ws[3] = {T(0), T(0), T(0)};
 
@@ -760,7 +840,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
             constexpr int d = decltype(dc)::value;
 #pragma unroll
             for (int k = 0; k <= 2 * RAD; ++k)
-                load_cell<T, NV>(in, in2, LAZY, c, i + (k - RAD) * sxd[d], vs, q[k]);
+                load_cell<T, NV>(in, in2, LAZY, c, i + (k - RAD) * sxd[d], q[k]);
             recon_stencil<Recon, T, NV>(q, L, R, eq.gamma);
         };
 
@@ -803,16 +883,16 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                 if constexpr (DIM == 2) {
 #pragma unroll
                     for (int k = 0; k <= 2 * RAD; ++k)
-                        load_cell<T, NV>(in, in2, LAZY, c, row_flat(m0 + 1 - RAD + k), vs, Wn[k]);
+                        load_cell<T, NV>(in, in2, LAZY, c, row_flat(m0 + 1 - RAD + k), Wn[k]);
                 } else {
-                    load_cell<T, NV>(in, in2, LAZY, c, row_flat(0), vs, Wn[RAD - 1]);
+                    load_cell<T, NV>(in, in2, LAZY, c, row_flat(0), Wn[RAD - 1]);
                 }
             }
             for (int m = m0; m < m1; ++m) {  // uniform over the wavefront
                 const long long i0 = flat(m);
                 T out_v[NV], uin[NV];
                 PendingCell<T, NV> pn;
-                if constexpr (DIM == 2) pn.issue(in, in2, LAZY, row_flat(m + 2 + RAD), vs);
+                if constexpr (DIM == 2) pn.issue(in, in2, LAZY, row_flat(m + 2 + RAD));
                 {  // x: own right face, left face from the lane below
                     T q[2 * RAD + 1][NV], Lx[NV], Rx[NV];
                     if constexpr (WINDOW) {
@@ -897,12 +977,12 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                     T u0[NV], res[NV];
                     if (MODE != MODE_RHS) {
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) u0[v] = in2[i0 + v * vs];
+                        for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
                     }
                     combine<T, NV>(MODE, c, out_v, uin, u0, res);
                     if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                    store_cell<T, NV, DIM>(out, vs, D, col, DIM == 3 ? o : m, DIM == 3 ? m : 0,
-                                           res);
+                    store_cell<T, NV, DIM, OP == 0>(out, D, col, DIM == 3 ? o : m,
+                                                    DIM == 3 ? m : 0, res);
                 }
                 if constexpr (DIM == 2) {  // advance the window by one row
 #pragma unroll
@@ -921,6 +1001,57 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                 if (in_strip && li == 0 && runnable(*Sp))
                     atomicMax(a.ws + 3 * sample + d, (unsigned long long)__double_as_longlong(double(m)));
             }
+        }
+    }
+}
+
+// ---- Reconstruction::performReconstruction on a reference Volume
+// (ReconstructionCPU.cpp:38-97, ENOCPU.cpp:41-176, WENOCPU.cpp:32-186,
+// NoReconstruction.cpp:42-73): the left (i-1/2) and right (i+1/2) face
+// values of every cell of the padded range [lo, hi) along `direction`, from
+// the same reconstruction arithmetic as the stage kernels. One thread per
+// cell; the host checks that every stencil stays inside the volume.
+template <class T>
+struct ReconArgs {
+    const T* vin[5];
+    T* vleft[5];
+    T* vright[5];
+    int tx, ty, tz;      // padded extents
+    int lo[3], hi[3];    // padded cell range per axis
+    int direction;
+    double gamma;
+};
+
+template <class T, class Eq, class Recon>
+__global__ __launch_bounds__(256) void recon_volume_kernel(ReconArgs<T> a) {
+    constexpr int NV = Eq::nvar;
+    constexpr int RAD = recon_radius<Recon>::value;
+    const int ex = a.hi[0] - a.lo[0], ey = a.hi[1] - a.lo[1], ez = a.hi[2] - a.lo[2];
+    const long long n = (long long)ex * ey * ez;
+    const long long st = a.direction == 0 ? 1LL
+                         : (a.direction == 1 ? (long long)a.tx : (long long)a.tx * a.ty);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = a.lo[0] + int(c % ex);
+        const int y = a.lo[1] + int((c / ex) % ey);
+        const int z = a.lo[2] + int(c / ((long long)ex * ey));
+        const long long i = ((long long)z * a.ty + y) * a.tx + x;
+        T L[NV], R[NV];
+        if constexpr (std::is_same_v<Recon, ReconNone>) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) L[v] = R[v] = a.vin[v][i];
+        } else {
+            T q[2 * RAD + 1][NV];
+#pragma unroll
+            for (int k = 0; k <= 2 * RAD; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) q[k][v] = a.vin[v][i + (k - RAD) * st];
+            recon_stencil<Recon, T, NV>(q, L, R, T(a.gamma));
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            a.vleft[v][i] = L[v];
+            a.vright[v][i] = R[v];
         }
     }
 }

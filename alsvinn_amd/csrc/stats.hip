@@ -22,6 +22,7 @@
 // Accumulation is FP64 whatever the storage precision.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -30,6 +31,28 @@
 #include "internal.hpp"
 
 using namespace alsvinn_hip;
+
+// One statistic fed one sample at a time (include/alsvinn_hip.h,
+// alsvinn_hip_statistics_*): FP64 accumulators of one snapshot plus the
+// scratch of the structure-function partial sums and the bv terms.
+struct alsvinn_hip_statistics {
+    int kind = 0;
+    double p = 1;
+    int H = 1, direction = 0, periodic = 0;
+    int nvar = 0, precision = 0, dims = 1;
+    int n[3] = {1, 1, 1}, g[3] = {0, 0, 0};
+    long long vstride = 0;   // padded cells per variable
+    long long elements = 0;  // FP64 values per output
+    int noutputs = 1;
+    double* out[2] = {nullptr, nullptr};
+    double* part = nullptr;  // structure functions: nvar x nblocks x H partial sums
+    double* tv = nullptr;    // bv: nvar total variations of the last sample
+    int nblocks = 0, tiles_x = 0;
+    bool tiled = false;
+    size_t lds = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
 
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \
@@ -158,6 +181,24 @@ struct StructGeom {
     long long vstride;
 };
 
+// Where the statistics kernels find variable `var` of sample k: the
+// ensemble's buffer (a sample's volume at bases[k], variables vstride apart)
+// or one reference Volume's per-variable memories (k = 0).
+template <class T>
+struct EnsembleSrc {
+    const T* U;
+    const long long* bases;
+    long long vstride;
+    __device__ __forceinline__ const T* at(int k, int var) const {
+        return U + bases[k] + (long long)var * vstride;
+    }
+};
+template <class T>
+struct VolumeSrc {
+    const T* p[5];
+    __device__ __forceinline__ const T* at(int, int var) const { return p[var]; }
+};
+
 template <class T>
 __device__ __forceinline__ double value_at(const T* u, const StructGeom& g, int x, int y, int z) {
     const int X = wrap_index(x, g.nx, g.periodic) + g.gx;
@@ -181,10 +222,8 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // One block = 256 interior cells of one sample (blockIdx.z) and one variable
 // (blockIdx.y); writes partial[((k*nvar + var)*nblocks + block)*H + h].
-template <class T>
-__global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict__ U,
-                                                            const long long* __restrict__ bases,
-                                                            StructGeom g, int H, double p,
+template <class T, class Src>
+__global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom g, int H, double p,
                                                             int nvar, double* partial) {
     __shared__ double wsum[4][256];  // wave sums of up to 256 values of h
     const int k = blockIdx.z, var = blockIdx.y;
@@ -194,7 +233,7 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(const T* __restrict
     const int i = valid ? int(cell % g.nx) : 0;
     const int j = valid ? int((cell / g.nx) % g.ny) : 0;
     const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
-    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const T* u = src.at(k, var);
     const double uc = value_at(u, g, i, j, l);
     const int dims = g.dims;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -265,10 +304,10 @@ __device__ __forceinline__ double pow_int(double x) {
 
 // PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
 // other p (device pow).
-template <class T, int PK>
-__global__ __launch_bounds__(256) void structure_cube_tiled_kernel(
-    const T* __restrict__ U, const long long* __restrict__ bases, StructGeom g, int H, double p,
-    int nvar, int tiles_x, double* partial) {
+template <class T, int PK, class Src>
+__global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, StructGeom g, int H,
+                                                                  double p, int nvar, int tiles_x,
+                                                                  double* partial) {
     extern __shared__ double lds[];
     const int W = tile_width(H);
     const int R = H - 1;
@@ -277,7 +316,7 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(
     const int k = blockIdx.z, var = blockIdx.y;
     const int x0 = (blockIdx.x % tiles_x) * kTile;
     const int y0 = (blockIdx.x / tiles_x) * kTile;
-    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const T* u = src.at(k, var);
     for (int idx = threadIdx.x; idx < W * W; idx += 256) {
         const int lx = idx % W, ly = idx / W;
         tile[idx] = value_at(u, g, x0 - R + lx, y0 - R + ly, 0);
@@ -324,10 +363,8 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(
 
 // PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
 // other p (device pow).
-template <class T, int PK>
-__global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restrict__ U,
-                                                             const long long* __restrict__ bases,
-                                                             StructGeom g, int H, double p,
+template <class T, int PK, class Src>
+__global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeom g, int H, double p,
                                                              int direction, int nvar,
                                                              double* partial) {
     // block totals per h: wave sums, then the four waves in order, for up to
@@ -340,7 +377,7 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restric
     const int i = valid ? int(cell % g.nx) : 0;
     const int j = valid ? int((cell / g.nx) % g.ny) : 0;
     const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
-    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const T* u = src.at(k, var);
     const double uc = value_at(u, g, i, j, l);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double* out = partial + (((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H;
@@ -374,17 +411,16 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(const T* __restric
 // 64 terms in lane order into one FP64 accumulator, so the sum is the
 // reference's sequential sum, bit for bit. The next batch's loads are issued
 // before the current batch is summed. out[k * nvar + var].
-template <class T>
-__global__ __launch_bounds__(64) void bv_kernel(const T* __restrict__ U,
-                                               const long long* __restrict__ bases, StructGeom g,
-                                               int tz, double p, int nvar, double* out) {
+template <class T, class Src>
+__global__ __launch_bounds__(64) void bv_kernel(Src src, StructGeom g, int tz, double p, int nvar,
+                                               double* out) {
     const int var = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
     const int x0 = g.gx + 1, x1 = g.tx - g.gx;
     const int y0 = g.gy + (g.ty > 1 ? 1 : 0), y1 = g.ty - g.gy;
     const int z1 = tz - g.gz;
     const long long wx = x1 > x0 ? x1 - x0 : 0, wy = y1 > y0 ? y1 - y0 : 0;
     const long long ncell = wx * wy * (z1 > 0 ? z1 : 0);
-    const T* u = U + bases[k] + (long long)var * g.vstride;
+    const T* u = src.at(k, var);
     auto load = [&](long long c, double& uc, double& ub, double& ul) {
         uc = ub = ul = 0;
         if (c < ncell) {
@@ -475,6 +511,7 @@ StructGeom geom_of(const alsvinn_hip_ensemble* e, const SampleDesc& D) {
 // list order (each sample's structure function, then their combination, as
 // the reference computes it per sample). The order is fixed, so repeated
 // calls give the same bits.
+template <bool ACCUMULATE>
 __global__ __launch_bounds__(256) void sample_partials_kernel(const double* __restrict__ part,
                                                              long long n, int nblocks, int H,
                                                              double* __restrict__ sums) {
@@ -494,33 +531,35 @@ __global__ __launch_bounds__(256) void sample_partials_kernel(const double* __re
         for (int i = 0; i < kBatch; ++i) acc += v[i];
     }
     for (; b < nblocks; ++b) acc += src[(long long)b * H];
-    sums[t] = acc;
+    if (ACCUMULATE)
+        sums[t] += acc;  // the per-sample statistics: the samples added in call order
+    else
+        sums[t] = acc;
 }
 
-template <class T>
-void launch_basic(int pk, dim3 grid, hipStream_t st, const T* U, const long long* bases,
-                  StructGeom g, int H, double p, int dir, int nvar, double* part) {
+template <class T, class Src>
+void launch_basic(int pk, dim3 grid, hipStream_t st, Src src, StructGeom g, int H, double p, int dir,
+                  int nvar, double* part) {
     switch (pk) {
-        case 1: hipLaunchKernelGGL((structure_basic_kernel<T, 1>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
-        case 2: hipLaunchKernelGGL((structure_basic_kernel<T, 2>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
-        case 3: hipLaunchKernelGGL((structure_basic_kernel<T, 3>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
-        case 4: hipLaunchKernelGGL((structure_basic_kernel<T, 4>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
-        case 5: hipLaunchKernelGGL((structure_basic_kernel<T, 5>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
-        default: hipLaunchKernelGGL((structure_basic_kernel<T, 0>), grid, dim3(256), 0, st, U, bases, g, H, p, dir, nvar, part); break;
+        case 1: hipLaunchKernelGGL((structure_basic_kernel<T, 1, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
+        case 2: hipLaunchKernelGGL((structure_basic_kernel<T, 2, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
+        case 3: hipLaunchKernelGGL((structure_basic_kernel<T, 3, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
+        case 4: hipLaunchKernelGGL((structure_basic_kernel<T, 4, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
+        case 5: hipLaunchKernelGGL((structure_basic_kernel<T, 5, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
+        default: hipLaunchKernelGGL((structure_basic_kernel<T, 0, Src>), grid, dim3(256), 0, st, src, g, H, p, dir, nvar, part); break;
     }
 }
 
-template <class T>
-void launch_tiled(int pk, dim3 grid, size_t lds, hipStream_t st, const T* U,
-                  const long long* bases, StructGeom g, int H, double p, int nvar, int tiles_x,
-                  double* part) {
+template <class T, class Src>
+void launch_tiled(int pk, dim3 grid, size_t lds, hipStream_t st, Src src, StructGeom g, int H,
+                  double p, int nvar, int tiles_x, double* part) {
     switch (pk) {
-        case 1: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 1>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
-        case 2: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 2>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
-        case 3: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 3>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
-        case 4: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 4>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
-        case 5: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 5>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
-        default: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 0>), grid, dim3(256), lds, st, U, bases, g, H, p, nvar, tiles_x, part); break;
+        case 1: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 1, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 2: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 2, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 3: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 3, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 4: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 4, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 5: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 5, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        default: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 0, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
     }
 }
 
@@ -550,7 +589,7 @@ int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, dou
     // fixed-order reduction: each sample's blocks in index order on the
     // device, then the samples in list order on the host
     if (le == hipSuccess) {
-        hipLaunchKernelGGL(sample_partials_kernel, dim3(unsigned((nsum + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(sample_partials_kernel<false>, dim3(unsigned((nsum + 255) / 256)), dim3(256), 0,
                            e->stream, d_part.p, nsum, nblocks, H, d_sum.p);
         le = hipGetLastError();
     }
@@ -569,6 +608,60 @@ int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, dou
             host_out[(size_t)var * H + h] = (acc[(size_t)var * H + h] / N) / double(m);
     return 0;
 }
+
+// ---- statistics fed one sample at a time (a reference Volume per call)
+
+// meanvar: a1 += u, a2 += u*u; onepointmoment: a1 += pow(u, p); every
+// padded cell of every variable (index var*n + i), as the ensemble kernels
+// add the samples in order.
+template <class T>
+__global__ __launch_bounds__(256) void sample_moments_kernel(VolumeSrc<T> src, long long n, int nvar,
+                                                            int meanvar, double p,
+                                                            double* __restrict__ a1,
+                                                            double* __restrict__ a2) {
+    const long long total = n * nvar;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int var = int(idx / n);
+        const double u = double(src.p[var][idx - (long long)var * n]);
+        if (meanvar) {
+            a1[idx] += u;
+            a2[idx] += u * u;
+        } else {
+            a1[idx] += ref_pow(u, p);
+        }
+    }
+}
+
+__global__ void add_vec_kernel(double* __restrict__ acc, const double* __restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] += x[i];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double* x, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = 0;
+}
+
+// combineStatistics: x / M, or (x / N) / M for the structure functions' sums
+// over N interior cells (the ensemble entry points' order of operations).
+__global__ __launch_bounds__(256) void combine_kernel(double* x, long long n, double N, double M,
+                                                     int per_cell) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = per_cell ? (x[i] / N) / M : x[i] / M;
+}
+
+// MeanVariance::finalizeStatistics: variance -= pow(mean, 2).
+__global__ __launch_bounds__(256) void variance_kernel(const double* __restrict__ mean,
+                                                      double* __restrict__ var, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        var[i] = var[i] - mean[i] * mean[i];
+}
+
+int grid_blocks(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 65536)); }
 
 }  // namespace
 
@@ -647,12 +740,15 @@ int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samp
     HIP_TRY(d_tv.alloc(np));
     HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
     const dim3 grid(e->nvar, m);
+    const StructGeom gd = geom_of(e, D);
     if (e->esize == 8)
-        hipLaunchKernelGGL(bv_kernel<double>, grid, dim3(64), 0, e->stream, (const double*)e->U,
-                           d_bases.p, geom_of(e, D), D.tz, p, e->nvar, d_tv.p);
+        hipLaunchKernelGGL((bv_kernel<double, EnsembleSrc<double>>), grid, dim3(64), 0, e->stream,
+                           EnsembleSrc<double>{(const double*)e->U, d_bases.p, gd.vstride}, gd, D.tz,
+                           p, e->nvar, d_tv.p);
     else
-        hipLaunchKernelGGL(bv_kernel<float>, grid, dim3(64), 0, e->stream, (const float*)e->U,
-                           d_bases.p, geom_of(e, D), D.tz, p, e->nvar, d_tv.p);
+        hipLaunchKernelGGL((bv_kernel<float, EnsembleSrc<float>>), grid, dim3(64), 0, e->stream,
+                           EnsembleSrc<float>{(const float*)e->U, d_bases.p, gd.vstride}, gd, D.tz,
+                           p, e->nvar, d_tv.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
     std::vector<double> tv(np);
@@ -686,24 +782,28 @@ int alsvinn_hip_stats_structure_cube(alsvinn_hip_ensemble* e, const int* samples
             [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
                 const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0) ? int(p) : 0;
                 if (e->esize == 8)
-                    launch_tiled<double>(pk, grid, lds, e->stream, (const double*)e->U, bases, g,
-                                         H, p, e->nvar, tiles_x, part);
+                    launch_tiled<double>(pk, grid, lds, e->stream,
+                                         EnsembleSrc<double>{(const double*)e->U, bases, g.vstride},
+                                         g, H, p, e->nvar, tiles_x, part);
                 else
-                    launch_tiled<float>(pk, grid, lds, e->stream, (const float*)e->U, bases, g, H,
-                                        p, e->nvar, tiles_x, part);
+                    launch_tiled<float>(pk, grid, lds, e->stream,
+                                        EnsembleSrc<float>{(const float*)e->U, bases, g.vstride}, g,
+                                        H, p, e->nvar, tiles_x, part);
             },
             tiles);
     }
     return run_structure(e, samples, m, number_of_h, host_out,
                          [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
                              if (e->esize == 8)
-                                 hipLaunchKernelGGL(structure_cube_kernel<double>, grid, dim3(256), 0,
-                                                    e->stream, (const double*)e->U, bases, g,
-                                                    number_of_h, p, e->nvar, part);
+                                 hipLaunchKernelGGL((structure_cube_kernel<double, EnsembleSrc<double>>),
+                                                    grid, dim3(256), 0, e->stream,
+                                                    EnsembleSrc<double>{(const double*)e->U, bases, g.vstride},
+                                                    g, number_of_h, p, e->nvar, part);
                              else
-                                 hipLaunchKernelGGL(structure_cube_kernel<float>, grid, dim3(256), 0,
-                                                    e->stream, (const float*)e->U, bases, g,
-                                                    number_of_h, p, e->nvar, part);
+                                 hipLaunchKernelGGL((structure_cube_kernel<float, EnsembleSrc<float>>),
+                                                    grid, dim3(256), 0, e->stream,
+                                                    EnsembleSrc<float>{(const float*)e->U, bases, g.vstride},
+                                                    g, number_of_h, p, e->nvar, part);
                          });
 }
 
@@ -717,10 +817,12 @@ int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* sample
                              const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0)
                                                 ? int(p) : 0;
                              if (e->esize == 8)
-                                 launch_basic<double>(pk, grid, e->stream, (const double*)e->U, bases,
+                                 launch_basic<double>(pk, grid, e->stream,
+                                                      EnsembleSrc<double>{(const double*)e->U, bases, g.vstride},
                                                       g, number_of_h, p, direction, e->nvar, part);
                              else
-                                 launch_basic<float>(pk, grid, e->stream, (const float*)e->U, bases,
+                                 launch_basic<float>(pk, grid, e->stream,
+                                                     EnsembleSrc<float>{(const float*)e->U, bases, g.vstride},
                                                      g, number_of_h, p, direction, e->nvar, part);
                          });
 }
@@ -751,6 +853,231 @@ int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host) {
             for (int b = 0; b < blocks; ++b) acc += part[((size_t)s * e->nvar + v) * blocks + b];
             host[(size_t)s * e->nvar + v] = acc;
         }
+    return 0;
+}
+
+int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direction, int boundary,
+                                  const alsvinn_hip_volume* shape, void* stream,
+                                  alsvinn_hip_statistics** out) {
+    if (!out) return fail(ALSVINN_HIP_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (kind < ALSVINN_HIP_STAT_MEANVAR || kind > ALSVINN_HIP_STAT_BV)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown statistics kind " + std::to_string(kind));
+    if (!shape || shape->nvar < 1 || shape->nvar > 5 || shape->nx < 1 || shape->ny < 1 ||
+        shape->nz < 1 || shape->gx < 0 || shape->gy < 0 || shape->gz < 0 ||
+        (shape->precision != ALSVINN_HIP_FP64 && shape->precision != ALSVINN_HIP_FP32))
+        return fail(ALSVINN_HIP_E_INVALID, "bad sample shape");
+    if (boundary != ALSVINN_HIP_BC_NEUMANN && boundary != ALSVINN_HIP_BC_PERIODIC)
+        return fail(ALSVINN_HIP_E_INVALID, "unknown boundary " + std::to_string(boundary));
+    const bool structure = kind == ALSVINN_HIP_STAT_STRUCTURE_CUBE || kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC;
+    const int dims = 1 + (shape->ny > 1) + (shape->nz > 1);
+    if (structure && number_of_h < 1) return fail(ALSVINN_HIP_E_INVALID, "numberOfH must be >= 1");
+    if (kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC && (direction < 0 || direction >= dims))
+        return fail(ALSVINN_HIP_E_INVALID, "structure_basic direction must be an active axis");
+    if (kind == ALSVINN_HIP_STAT_BV && dims > 2)
+        return fail(ALSVINN_HIP_E_INVALID, "bv: not supported for 3d (as in the reference)");
+    auto* st = new alsvinn_hip_statistics();
+    st->kind = kind;
+    st->p = p;
+    st->H = structure ? number_of_h : 1;
+    st->direction = direction;
+    st->periodic = boundary == ALSVINN_HIP_BC_PERIODIC;
+    st->nvar = shape->nvar;
+    st->precision = shape->precision;
+    st->dims = dims;
+    st->n[0] = shape->nx;
+    st->n[1] = shape->ny;
+    st->n[2] = shape->nz;
+    st->g[0] = shape->gx;
+    st->g[1] = shape->gy;
+    st->g[2] = shape->gz;
+    st->vstride = (long long)(shape->nx + 2 * shape->gx) * (shape->ny + 2 * shape->gy) *
+                  (shape->nz + 2 * shape->gz);
+    const long long ncell = (long long)shape->nx * shape->ny * shape->nz;
+    if (kind == ALSVINN_HIP_STAT_MEANVAR || kind == ALSVINN_HIP_STAT_ONEPOINTMOMENT) {
+        st->elements = st->vstride * st->nvar;
+        st->noutputs = kind == ALSVINN_HIP_STAT_MEANVAR ? 2 : 1;
+    } else if (structure) {
+        st->elements = (long long)st->nvar * st->H;
+        const int H = st->H;
+        st->lds = (size_t(tile_width(H)) * tile_width(H) + 4 * size_t(H)) * sizeof(double);
+        st->tiled = kind == ALSVINN_HIP_STAT_STRUCTURE_CUBE && dims == 2 && H >= 2 && st->lds <= 65536;
+        if (st->tiled) {
+            st->tiles_x = (shape->nx + kTile - 1) / kTile;
+            st->nblocks = st->tiles_x * ((shape->ny + kTile - 1) / kTile);
+        } else {
+            st->nblocks = int((ncell + 255) / 256);
+        }
+    } else {
+        st->elements = st->nvar;
+    }
+    auto cleanup = [&](hipError_t e, const char* what) {
+        alsvinn_hip_statistics_destroy(st);
+        return fail(ALSVINN_HIP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e = hipSuccess;
+    if (stream) {
+        st->stream = (hipStream_t)stream;
+    } else {
+        if ((e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking)) != hipSuccess)
+            return cleanup(e, "hipStreamCreateWithFlags");
+        st->own_stream = true;
+    }
+    for (int k = 0; k < st->noutputs; ++k)
+        if ((e = hipMalloc(&st->out[k], sizeof(double) * st->elements)) != hipSuccess)
+            return cleanup(e, "hipMalloc");
+    if (structure) {
+        const size_t np = (size_t)st->nvar * st->nblocks * st->H;
+        if ((e = hipMalloc(&st->part, sizeof(double) * np)) != hipSuccess) return cleanup(e, "hipMalloc");
+        // entries a kernel never writes (structure_cube's h = 0) stay zero
+        if ((e = hipMemset(st->part, 0, sizeof(double) * np)) != hipSuccess) return cleanup(e, "hipMemset");
+    }
+    if (kind == ALSVINN_HIP_STAT_BV)
+        if ((e = hipMalloc(&st->tv, sizeof(double) * st->nvar)) != hipSuccess) return cleanup(e, "hipMalloc");
+    if (int rc = alsvinn_hip_statistics_reset(st)) {
+        alsvinn_hip_statistics_destroy(st);
+        return rc;
+    }
+    if ((e = hipStreamSynchronize(st->stream)) != hipSuccess) return cleanup(e, "hipStreamSynchronize");
+    *out = st;
+    return 0;
+}
+
+int alsvinn_hip_statistics_destroy(alsvinn_hip_statistics* st) {
+    if (!st) return 0;
+    if (st->stream) (void)hipStreamSynchronize(st->stream);
+    for (int k = 0; k < 2; ++k) (void)hipFree(st->out[k]);
+    (void)hipFree(st->part);
+    (void)hipFree(st->tv);
+    if (st->own_stream && st->stream) (void)hipStreamDestroy(st->stream);
+    delete st;
+    return 0;
+}
+
+int alsvinn_hip_statistics_reset(alsvinn_hip_statistics* st) {
+    if (!st) return fail(ALSVINN_HIP_E_INVALID, "statistics is NULL");
+    for (int k = 0; k < st->noutputs; ++k)
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_blocks(st->elements)), dim3(256), 0, st->stream,
+                           st->out[k], st->elements);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"
+
+template <class T>
+static void statistics_compute(alsvinn_hip_statistics* st, const alsvinn_hip_volume& v) {
+    VolumeSrc<T> src;
+    for (int k = 0; k < 5; ++k) src.p[k] = k < st->nvar ? (const T*)v.var[k] : nullptr;
+    StructGeom g;
+    g.nx = st->n[0];
+    g.ny = st->n[1];
+    g.nz = st->n[2];
+    g.gx = st->g[0];
+    g.gy = st->g[1];
+    g.gz = st->g[2];
+    g.tx = st->n[0] + 2 * st->g[0];
+    g.ty = st->n[1] + 2 * st->g[1];
+    g.dims = st->dims;
+    g.periodic = st->periodic;
+    g.vstride = st->vstride;
+    const int pk = (st->p == 1.0 || st->p == 2.0 || st->p == 3.0 || st->p == 4.0 || st->p == 5.0)
+                       ? int(st->p) : 0;
+    switch (st->kind) {
+        case ALSVINN_HIP_STAT_MEANVAR:
+        case ALSVINN_HIP_STAT_ONEPOINTMOMENT:
+            hipLaunchKernelGGL(sample_moments_kernel<T>, dim3(grid_blocks(st->elements)), dim3(256), 0,
+                               st->stream, src, st->vstride, st->nvar,
+                               st->kind == ALSVINN_HIP_STAT_MEANVAR ? 1 : 0, st->p, st->out[0],
+                               st->out[1]);
+            break;
+        case ALSVINN_HIP_STAT_STRUCTURE_CUBE:
+        case ALSVINN_HIP_STAT_STRUCTURE_BASIC: {
+            const dim3 grid(st->nblocks, st->nvar, 1);
+            if (st->kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC)
+                launch_basic<T>(pk, grid, st->stream, src, g, st->H, st->p, st->direction, st->nvar,
+                                st->part);
+            else if (st->tiled)
+                launch_tiled<T>(pk, grid, st->lds, st->stream, src, g, st->H, st->p, st->nvar,
+                                st->tiles_x, st->part);
+            else
+                hipLaunchKernelGGL((structure_cube_kernel<T, VolumeSrc<T>>), grid, dim3(256), 0,
+                                   st->stream, src, g, st->H, st->p, st->nvar, st->part);
+            const long long nsum = st->elements;
+            hipLaunchKernelGGL(sample_partials_kernel<true>, dim3(unsigned((nsum + 255) / 256)), dim3(256),
+                               0, st->stream, st->part, nsum, st->nblocks, st->H, st->out[0]);
+            break;
+        }
+        case ALSVINN_HIP_STAT_BV:
+            hipLaunchKernelGGL((bv_kernel<T, VolumeSrc<T>>), dim3(st->nvar, 1), dim3(64), 0,
+                               st->stream, src, g, st->n[2] + 2 * st->g[2], st->p, st->nvar, st->tv);
+            hipLaunchKernelGGL(add_vec_kernel, dim3(1), dim3(64), 0, st->stream, st->out[0], st->tv,
+                               st->nvar);
+            break;
+    }
+}
+
+extern "C" {
+
+int alsvinn_hip_statistics_compute(alsvinn_hip_statistics* st, const alsvinn_hip_volume* sample) {
+    if (!st || !sample) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    if (sample->nvar != st->nvar || sample->precision != st->precision || sample->nx != st->n[0] ||
+        sample->ny != st->n[1] || sample->nz != st->n[2] || sample->gx != st->g[0] ||
+        sample->gy != st->g[1] || sample->gz != st->g[2])
+        return fail(ALSVINN_HIP_E_INVALID, "sample volume does not match the statistics' shape");
+    for (int k = 0; k < st->nvar; ++k)
+        if (!sample->var[k]) return fail(ALSVINN_HIP_E_INVALID, "sample variable pointer is NULL");
+    if (st->precision == ALSVINN_HIP_FP64)
+        statistics_compute<double>(st, *sample);
+    else
+        statistics_compute<float>(st, *sample);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_statistics_combine(alsvinn_hip_statistics* st, long long samples) {
+    if (!st || samples < 1) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const bool per_cell = st->kind == ALSVINN_HIP_STAT_STRUCTURE_CUBE ||
+                          st->kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC;
+    const double N = double((long long)st->n[0] * st->n[1] * st->n[2]);
+    for (int k = 0; k < st->noutputs; ++k)
+        hipLaunchKernelGGL(combine_kernel, dim3(grid_blocks(st->elements)), dim3(256), 0, st->stream,
+                           st->out[k], st->elements, N, double(samples), per_cell ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_statistics_finalize(alsvinn_hip_statistics* st) {
+    if (!st) return fail(ALSVINN_HIP_E_INVALID, "statistics is NULL");
+    if (st->kind == ALSVINN_HIP_STAT_MEANVAR) {
+        hipLaunchKernelGGL(variance_kernel, dim3(grid_blocks(st->elements)), dim3(256), 0, st->stream,
+                           st->out[0], st->out[1], st->elements);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int alsvinn_hip_statistics_outputs(const alsvinn_hip_statistics* st, int* count,
+                                   long long* elements) {
+    if (!st) return fail(ALSVINN_HIP_E_INVALID, "statistics is NULL");
+    if (count) *count = st->noutputs;
+    if (elements) *elements = st->elements;
+    return 0;
+}
+
+int alsvinn_hip_statistics_result(alsvinn_hip_statistics* st, int index, double** device) {
+    if (!st || !device || index < 0 || index >= st->noutputs)
+        return fail(ALSVINN_HIP_E_INVALID, "bad output index");
+    *device = st->out[index];
+    return 0;
+}
+
+int alsvinn_hip_statistics_download(alsvinn_hip_statistics* st, int index, double* host) {
+    if (!st || !host || index < 0 || index >= st->noutputs)
+        return fail(ALSVINN_HIP_E_INVALID, "bad output index");
+    HIP_TRY(hipMemcpyAsync(host, st->out[index], sizeof(double) * st->elements, hipMemcpyDeviceToHost,
+                           st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
     return 0;
 }
 

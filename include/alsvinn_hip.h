@@ -298,6 +298,117 @@ int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* sample
  * host[sample*nvar + var]. */
 int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host);
 
+/* ---- Operators on a reference Volume (device memory, one area per variable)
+ *
+ * The reference's platform operators take a volume::Volume whose conserved
+ * variables are separate memory areas of the padded extents
+ * (alsfvm/src/volume/Volume.cpp:44-60, View.hpp:116-118). These entry points
+ * take such a volume as device pointers, one per variable, and never copy it
+ * to the host. Calls are asynchronous on the operator's stream except where
+ * a result goes to the host. */
+typedef struct alsvinn_hip_volume {
+    int nvar;        /* conserved variables: 1 (Burgers), 3/4/5 (Euler 1D/2D/3D) */
+    int precision;   /* ALSVINN_HIP_FP64 or ALSVINN_HIP_FP32: element type of var[] */
+    int nx, ny, nz;  /* interior cells per axis (1 on inactive axes) */
+    int gx, gy, gz;  /* ghost layers per axis (0 on inactive axes) */
+    void* var[5];    /* device memory of variable v: (nx+2gx)(ny+2gy)(nz+2gz) elements, x fastest */
+} alsvinn_hip_volume;
+
+typedef struct alsvinn_hip_operator alsvinn_hip_operator;
+
+/* A numerical flux with its reconstruction for grids of nx x ny x nz interior
+ * cells: the object NumericalFluxFactory::createNumericalFlux(grid) returns
+ * (NumericalFluxFactory.cpp:159-200; ReconstructionFactory.cpp:41-355 for the
+ * reconstruction it owns). Only the scheme's equation, flux, reconstruction,
+ * precision and gamma are used. stream may be NULL (the library creates one). */
+int alsvinn_hip_operator_create(const alsvinn_hip_scheme* scheme, int nx, int ny, int nz,
+                                void* stream, alsvinn_hip_operator** out);
+int alsvinn_hip_operator_destroy(alsvinn_hip_operator* op);
+
+/* NumericalFlux::computeFlux(conservedVariables, waveSpeed, computeWaveSpeed,
+ * output, start, end) (NumericalFlux.hpp:44-52, NumericalFluxCPU.cpp:170-221):
+ * for every cell of the region [g+start, N+g+end) on each active axis (start
+ * >= 0, end <= 0, in cells), output = sum over directions of
+ * (-F(i+1/2)) + F(i-1/2), the unscaled divergence of the face fluxes; other
+ * cells of output, ghosts included, keep their contents. wave_speeds[d] gets
+ * the max wave speed over the faces of direction d of the region (every
+ * active direction; the reference computes it whatever compute_wave_speed
+ * says); the entry synchronises the operator's stream to return it. The
+ * ConservedSystem of the reference calls this once for the inner region and
+ * twice per direction for the sides (ConservedSystem.cpp:31-103). start and
+ * end may be NULL (whole interior). */
+int alsvinn_hip_numflux_compute(alsvinn_hip_operator* op, const alsvinn_hip_volume* conserved,
+                                double wave_speeds[3], int compute_wave_speed,
+                                const alsvinn_hip_volume* output, const int start[3],
+                                const int end[3]);
+
+/* Reconstruction::performReconstruction(input, direction, indicatorVariable,
+ * left, right, start, end) (Reconstruction.hpp:46-58) with the reference's
+ * index ranges: ReconstructionCPU.cpp:71-77 for weno2, wenof2, mc and mm
+ * (ghost 2), ENOCPU.cpp:92-98 for eno2-4 (ghost = order), WENOCPU.cpp:101-107
+ * for weno3 (start and end unused, every row of the other axes),
+ * NoReconstruction.cpp:42-64 for none (every cell). left[i] is the value at
+ * the cell's lower face, right[i] at its upper face. indicator_variable must
+ * be 0 (the reference's numerical fluxes pass 0). Asynchronous. */
+int alsvinn_hip_reconstruction_compute(alsvinn_hip_operator* op, const alsvinn_hip_volume* input,
+                                       int direction, int indicator_variable,
+                                       const alsvinn_hip_volume* left,
+                                       const alsvinn_hip_volume* right, const int start[3],
+                                       const int end[3]);
+
+/* Boundary::applyBoundaryConditions(volume, grid) (Boundary.hpp:30-31,
+ * BoundaryCPU.cpp:30-121 with Neumann.hpp:24-35 / Periodic.hpp:24-42): fills
+ * every ghost layer of the volume (corners included) from the interior.
+ * boundary: ALSVINN_HIP_BC_NEUMANN or ALSVINN_HIP_BC_PERIODIC. stream may be
+ * NULL (the default stream). Asynchronous. */
+int alsvinn_hip_boundary_apply(int boundary, const alsvinn_hip_volume* volume, void* stream);
+
+/* ---- Statistics fed one sample at a time (alsuq::stats::Statistics,
+ * Statistics.hpp:25-62), as the reference's Runner feeds them
+ * (Runner.cpp:42-87): computeStatistics for every sample, then
+ * combineStatistics (StatisticsHelper.cpp:35-79: / number of samples) and
+ * finalizeStatistics. Accumulators are FP64 device memory owned by the
+ * object, whatever the sample precision. One object is one snapshot (one
+ * save time); REGISTER_STATISTICS(hip, ...) classes keep one per time. */
+enum {
+    ALSVINN_HIP_STAT_MEANVAR = 0,          /* MeanVariance.cpp:30-67: outputs mean, variance */
+    ALSVINN_HIP_STAT_ONEPOINTMOMENT = 1,   /* OnePointMoment.cpp:30-45: output m_p */
+    ALSVINN_HIP_STAT_STRUCTURE_CUBE = 2,   /* StructureCube.cpp:37-76 */
+    ALSVINN_HIP_STAT_STRUCTURE_BASIC = 3,  /* StructureBasic.cpp:67-120 */
+    ALSVINN_HIP_STAT_BV = 4                /* BoundedVariation.cpp:38-53 (1D, 2D) */
+};
+
+typedef struct alsvinn_hip_statistics alsvinn_hip_statistics;
+
+/* kind: ALSVINN_HIP_STAT_*; p: the exponent (onepointmoment, structure
+ * functions, bv); number_of_h and direction: structure functions (direction
+ * for structure_basic only); boundary: ALSVINN_HIP_BC_* of the grid (the
+ * structure functions' ValueAtBoundary, ValueAtBoundary.hpp:20-98). shape
+ * gives nvar, precision, extents and ghosts of the samples (its pointers are
+ * not used). Outputs: meanvar 2 volumes (mean, variance) of nvar x padded
+ * cells; onepointmoment 1 such volume; structure functions nvar x number_of_h
+ * values (out[var*H + h]); bv nvar values. */
+int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direction, int boundary,
+                                  const alsvinn_hip_volume* shape, void* stream,
+                                  alsvinn_hip_statistics** out);
+int alsvinn_hip_statistics_destroy(alsvinn_hip_statistics* st);
+/* computeStatistics: adds one sample's contribution. Asynchronous. */
+int alsvinn_hip_statistics_compute(alsvinn_hip_statistics* st, const alsvinn_hip_volume* sample);
+/* combineStatistics on one rank: every output divided by samples (the
+ * structure functions' sums first by the number of interior cells). */
+int alsvinn_hip_statistics_combine(alsvinn_hip_statistics* st, long long samples);
+/* finalizeStatistics: meanvar variance -= mean^2; nothing for the others. */
+int alsvinn_hip_statistics_finalize(alsvinn_hip_statistics* st);
+/* Zeroes the accumulators (a new snapshot). */
+int alsvinn_hip_statistics_reset(alsvinn_hip_statistics* st);
+/* Number of outputs and FP64 elements per output. */
+int alsvinn_hip_statistics_outputs(const alsvinn_hip_statistics* st, int* count,
+                                   long long* elements);
+/* Device pointer of output `index` (FP64), valid until the object is destroyed. */
+int alsvinn_hip_statistics_result(alsvinn_hip_statistics* st, int index, double** device);
+/* Host copy of output `index`. Synchronous. */
+int alsvinn_hip_statistics_download(alsvinn_hip_statistics* st, int index, double* host);
+
 #ifdef __cplusplus
 }
 #endif

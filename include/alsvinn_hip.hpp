@@ -7,20 +7,30 @@
 //   alsfvm::hip::Simulator        alsfvm/include/alsfvm/simulator/Simulator.hpp
 //                                 (setInitialValue, performStep, atEnd,
 //                                 getCurrentTime, getEndTime; Simulator.cpp:77-214)
-//   alsfvm::hip::NumericalFlux    numflux::NumericalFlux::computeFlux over all
-//                                 regions (NumericalFlux.hpp:44-52, ConservedSystem.cpp:31-103)
+//   alsfvm::hip::NumericalFlux    numflux::NumericalFlux::computeFlux on a device
+//                                 Volume over a region (NumericalFlux.hpp:44-52)
+//   alsfvm::hip::Reconstruction   reconstruction::Reconstruction::performReconstruction
+//                                 (Reconstruction.hpp:46-58)
 //   alsfvm::hip::Boundary         boundary::Boundary::applyBoundaryConditions (Boundary.hpp:30-31)
+//   alsfvm::hip::ConservedSystem  simulator::ConservedSystem::operator() from computeFlux
+//                                 regions (ConservedSystem.cpp:31-103)
 //   alsuq::hip::Runner            alsuq::run::Runner::run (Runner.cpp:39-90): all samples
 //                                 of the rank advance together
 //   alsuq::hip::MeanVariance, StructureCube, StructureBasic, OnePointMoment,
 //   BoundedVariation              alsuq::stats (MeanVariance.cpp:30-67, StructureCube.cpp:37-76,
 //                                 StructureBasic.cpp:67-120, OnePointMoment.cpp:30-45,
-//                                 BoundedVariation.cpp:38-53)
+//                                 BoundedVariation.cpp:38-53) over a Runner's samples
+//   alsuq::hip::Statistics        alsuq::stats::Statistics fed one device Volume at a
+//                                 time (Statistics.hpp:25-62)
 //
-// Volumes cross the boundary as host vectors in the reference's layout
-// (variable-major, z*tx*ty + y*tx + x over padded extents).
+// The operators take device Volumes (alsvinn_hip_volume: one device pointer
+// per conserved variable, the reference Volume's layout, Volume.cpp:44-60)
+// and never copy them to the host. The Simulator and Runner take initial
+// data as host vectors in the same layout (variable-major, z*tx*ty + y*tx +
+// x over padded extents), as the reference evaluates initial data on the host.
 #pragma once
 
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -157,38 +167,111 @@ private:
     alsvinn_hip_cxx::EnsembleHandle h;
 };
 
-// numflux::NumericalFlux on the device: computeFlux over the inner and side
-// regions of a conserved volume (the reference's ConservedSystem), returning
-// the divergence and the per-direction max wave speeds.
+// numflux::NumericalFlux of the hip platform with its reconstruction, for one
+// grid: computeFlux on a device Volume over a region [start, end) of
+// interior cells (start >= 0, end <= 0), as ConservedSystem calls it.
 class NumericalFlux {
 public:
-    NumericalFlux(const alsvinn_hip_scheme& scheme, const Grid& grid) : sim(scheme, grid, 1.0, 1) {}
+    NumericalFlux(const alsvinn_hip_scheme& scheme, const Grid& grid, void* stream = nullptr)
+        : scheme(scheme) {
+        check(alsvinn_hip_operator_create(&scheme, grid.n[0], grid.n[1], grid.n[2], stream, &op),
+              "NumericalFlux");
+    }
+    ~NumericalFlux() { alsvinn_hip_operator_destroy(op); }
+    NumericalFlux(const NumericalFlux&) = delete;
+    NumericalFlux& operator=(const NumericalFlux&) = delete;
 
-    void computeFlux(const std::vector<double>& conservedVariables, double waveSpeed[3],
-                     bool computeWaveSpeed, std::vector<double>& output) {
-        check(alsvinn_hip_ensemble_upload(sim.handle(), 0, conservedVariables.data()), "computeFlux");
-        output.resize(conservedVariables.size());
-        double ws[3];
-        check(alsvinn_hip_ensemble_rhs(sim.handle(), 0, output.data(), ws), "computeFlux");
-        if (computeWaveSpeed)
-            for (int k = 0; k < 3; ++k) waveSpeed[k] = ws[k];
+    void computeFlux(const alsvinn_hip_volume& conservedVariables, double waveSpeed[3],
+                     bool computeWaveSpeed, const alsvinn_hip_volume& output,
+                     const int start[3] = nullptr, const int end[3] = nullptr) {
+        check(alsvinn_hip_numflux_compute(op, &conservedVariables, waveSpeed, computeWaveSpeed ? 1 : 0,
+                                          &output, start, end),
+              "computeFlux");
     }
 
+    size_t getNumberOfGhostCells() const { return (size_t)alsvinn_hip_ghost_cells(&scheme); }
+    alsvinn_hip_operator* handle() { return op; }
+
 private:
-    Simulator sim;
+    alsvinn_hip_scheme scheme;
+    alsvinn_hip_operator* op = nullptr;
 };
 
-// boundary::Boundary on the device.
-class Boundary {
+// reconstruction::Reconstruction of the hip platform (the scheme's
+// reconstruction), with the reference's index ranges per reconstruction.
+class Reconstruction {
 public:
-    Boundary(const alsvinn_hip_scheme& scheme, const Grid& grid) : sim(scheme, grid, 1.0, 1) {}
-    void applyBoundaryConditions(std::vector<double>& volume) {
-        sim.setInitialValue(volume);
-        volume = sim.getConservedVolume();
+    Reconstruction(const alsvinn_hip_scheme& scheme, const Grid& grid, void* stream = nullptr)
+        : flux(scheme, grid, stream), scheme(scheme) {}
+
+    void performReconstruction(const alsvinn_hip_volume& inputVariables, size_t direction,
+                               size_t indicatorVariable, const alsvinn_hip_volume& leftOut,
+                               const alsvinn_hip_volume& rightOut, const int start[3] = nullptr,
+                               const int end[3] = nullptr) {
+        check(alsvinn_hip_reconstruction_compute(flux.handle(), &inputVariables, (int)direction,
+                                                 (int)indicatorVariable, &leftOut, &rightOut, start,
+                                                 end),
+              "performReconstruction");
+    }
+
+    size_t getNumberOfGhostCells() const {
+        switch (scheme.reconstruction) {
+            case ALSVINN_HIP_RECON_NONE: return 1;
+            case ALSVINN_HIP_RECON_ENO3:
+            case ALSVINN_HIP_RECON_WENO3: return 3;
+            case ALSVINN_HIP_RECON_ENO4: return 4;
+            default: return 2;
+        }
     }
 
 private:
-    Simulator sim;
+    NumericalFlux flux;
+    alsvinn_hip_scheme scheme;
+};
+
+// boundary::Boundary of the hip platform: fills the ghost layers of a device
+// Volume in place (BoundaryFactory.cpp:38-75 names: "neumann", "periodic").
+class Boundary {
+public:
+    explicit Boundary(int boundary, void* stream = nullptr) : bc(boundary), stream(stream) {}
+    void applyBoundaryConditions(const alsvinn_hip_volume& volume) {
+        check(alsvinn_hip_boundary_apply(bc, &volume, stream), "applyBoundaryConditions");
+    }
+
+private:
+    int bc;
+    void* stream;
+};
+
+// simulator::ConservedSystem::operator() (ConservedSystem.cpp:31-103) on one
+// rank: computeFlux on the inner region, then on both sides of every active
+// direction; the wave speeds are the max over the regions.
+class ConservedSystem {
+public:
+    explicit ConservedSystem(NumericalFlux& flux) : flux(flux) {}
+    void operator()(const alsvinn_hip_volume& conservedVariables, double waveSpeed[3],
+                    bool computeWaveSpeed, const alsvinn_hip_volume& output) {
+        const int g[3] = {output.gx, output.gy, output.gz};
+        const int size[3] = {output.nx + 2 * output.gx, output.ny + 2 * output.gy,
+                             output.nz + 2 * output.gz};
+        const int dims = 1 + (output.ny > 1) + (output.nz > 1);
+        const int negg[3] = {-g[0], -g[1], -g[2]};
+        flux.computeFlux(conservedVariables, waveSpeed, computeWaveSpeed, output, g, negg);
+        double sides[3] = {0, 0, 0};
+        for (int d = 0; d < dims; ++d) {
+            int sb[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, st[3] = {0, 0, 0}, et[3] = {0, 0, 0};
+            eb[d] = -size[d] + 3 * g[d];
+            st[d] = size[d] - 3 * g[d];
+            double wb[3] = {0, 0, 0}, wt[3] = {0, 0, 0};
+            flux.computeFlux(conservedVariables, wb, computeWaveSpeed, output, sb, eb);
+            flux.computeFlux(conservedVariables, wt, computeWaveSpeed, output, st, et);
+            for (int k = 0; k < 3; ++k) sides[k] = std::max(sides[k], std::max(wb[k], wt[k]));
+        }
+        for (int k = 0; k < 3; ++k) waveSpeed[k] = std::max(waveSpeed[k], sides[k]);
+    }
+
+private:
+    NumericalFlux& flux;
 };
 
 }  // namespace hip
@@ -380,6 +463,50 @@ public:
 private:
     Runner& runner;
     int p;
+};
+
+// alsuq::stats::Statistics of the hip platform fed one device Volume at a
+// time, as the reference's Runner feeds its statistics (Runner.cpp:42-87):
+// computeStatistics per sample, then combineStatistics (one rank: divide by
+// the number of samples) and finalizeStatistics. kind: ALSVINN_HIP_STAT_*.
+class Statistics {
+public:
+    Statistics(int kind, const alsvinn_hip_volume& shape, double p = 2.0, int numberOfH = 1,
+               int direction = 0, int boundary = ALSVINN_HIP_BC_PERIODIC, void* stream = nullptr) {
+        check(alsvinn_hip_statistics_create(kind, p, numberOfH, direction, boundary, &shape, stream,
+                                            &st),
+              "Statistics");
+        check(alsvinn_hip_statistics_outputs(st, &outputs, &elements), "Statistics");
+    }
+    ~Statistics() { alsvinn_hip_statistics_destroy(st); }
+    Statistics(const Statistics&) = delete;
+    Statistics& operator=(const Statistics&) = delete;
+
+    void computeStatistics(const alsvinn_hip_volume& conservedVariables) {
+        check(alsvinn_hip_statistics_compute(st, &conservedVariables), "computeStatistics");
+    }
+    void combineStatistics(long long samples) {
+        check(alsvinn_hip_statistics_combine(st, samples), "combineStatistics");
+    }
+    void finalizeStatistics() { check(alsvinn_hip_statistics_finalize(st), "finalizeStatistics"); }
+    // Host copy of output i (meanvar: 0 mean, 1 variance).
+    std::vector<double> getStatistics(int i = 0) {
+        std::vector<double> out((size_t)elements);
+        check(alsvinn_hip_statistics_download(st, i, out.data()), "getStatistics");
+        return out;
+    }
+    // Device pointer of output i (FP64), for a writer that reads device memory.
+    double* getDevicePointer(int i = 0) {
+        double* p = nullptr;
+        check(alsvinn_hip_statistics_result(st, i, &p), "getDevicePointer");
+        return p;
+    }
+    int getNumberOfOutputs() const { return outputs; }
+
+private:
+    alsvinn_hip_statistics* st = nullptr;
+    int outputs = 0;
+    long long elements = 0;
 };
 
 }  // namespace hip

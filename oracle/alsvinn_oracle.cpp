@@ -781,6 +781,11 @@ struct SystemBase {
     virtual ~SystemBase() {}
     virtual void rhs(const Vol& U, real ws[3], Vol& out) = 0;
     virtual bool obeys(const Vol& U) = 0;
+    // NumericalFlux::computeFlux over one region
+    virtual void fluxRegion(const Vol& U, real ws[3], Vol& out, ivec3 s, ivec3 e) = 0;
+    // Reconstruction::performReconstruction into L, R (prior contents kept
+    // outside the reconstruction's range)
+    virtual void reconstructInto(const Vol& U, int d, ivec3 s, ivec3 e, Vol& L, Vol& R) = 0;
 };
 
 template <class Eq, class Flux>
@@ -831,6 +836,18 @@ struct System : SystemBase {
                 netFlux<Eq, Flux, 2>(eq, left, right, out, temp, ws[2], s, e);
             }
         }
+    }
+
+    void fluxRegion(const Vol& U, real ws[3], Vol& out, ivec3 s, ivec3 e) override {
+        computeFlux(U, ws, out, s, e);
+    }
+
+    void reconstructInto(const Vol& U, int d, ivec3 s, ivec3 e, Vol& L, Vol& R) override {
+        left.buf = L.buf;
+        right.buf = R.buf;
+        reconstruct(U, d, s, e);
+        L.buf = left.buf;
+        R.buf = right.buf;
     }
 
     // ConservedSystem::operator() (ConservedSystem.cpp:31-103): inner region
@@ -1122,6 +1139,53 @@ int oracle_system_rhs(const oracle_setup* setup, const double* u, double* out, d
     delete sys;
     std::memcpy(out, O.buf.data(), O.buf.size() * sizeof(double));
     for (int k = 0; k < 3; ++k) ws3[k] = ws[k];
+    return 0;
+}
+
+// One NumericalFlux::computeFlux(U, ws, computeWaveSpeed, out, start, end)
+// call (NumericalFluxCPU.cpp:170-221). `out` holds the prior contents of the
+// output volume; ws3 gets the wave speeds of the active directions (0 for
+// the others).
+int oracle_compute_flux(const oracle_setup* setup, const double* u, const int* start,
+                        const int* end, double* out, double* ws3) {
+    if (!orc::validSetup(*setup)) return 1;
+    const int ng = orc::systemGhosts(*setup);
+    const int nv = orc::nvarOf(setup->equation);
+    orc::Vol U(nv, setup->nx, setup->ny, setup->nz, ng);
+    orc::Vol O(nv, setup->nx, setup->ny, setup->nz, ng);
+    std::memcpy(U.buf.data(), u, U.buf.size() * sizeof(double));
+    std::memcpy(O.buf.data(), out, O.buf.size() * sizeof(double));
+    orc::SystemBase* sys = orc::makeSystem(*setup, U);
+    if (!sys) return 1;
+    double ws[3] = {0, 0, 0};
+    sys->fluxRegion(U, ws, O, orc::iv(start[0], start[1], start[2]), orc::iv(end[0], end[1], end[2]));
+    delete sys;
+    std::memcpy(out, O.buf.data(), O.buf.size() * sizeof(double));
+    for (int k = 0; k < 3; ++k) ws3[k] = ws[k];
+    return 0;
+}
+
+// One Reconstruction::performReconstruction(U, direction, 0, left, right,
+// start, end) call of the setup's reconstruction on a fresh reconstruction
+// object; left/right hold their prior contents.
+int oracle_reconstruct(const oracle_setup* setup, const double* u, int direction, const int* start,
+                       const int* end, double* left, double* right) {
+    if (!orc::validSetup(*setup)) return 1;
+    const int ng = orc::systemGhosts(*setup);
+    const int nv = orc::nvarOf(setup->equation);
+    orc::Vol U(nv, setup->nx, setup->ny, setup->nz, ng);
+    orc::Vol L(nv, setup->nx, setup->ny, setup->nz, ng);
+    orc::Vol R(nv, setup->nx, setup->ny, setup->nz, ng);
+    std::memcpy(U.buf.data(), u, U.buf.size() * sizeof(double));
+    std::memcpy(L.buf.data(), left, L.buf.size() * sizeof(double));
+    std::memcpy(R.buf.data(), right, R.buf.size() * sizeof(double));
+    orc::SystemBase* sys = orc::makeSystem(*setup, U);
+    if (!sys) return 1;
+    sys->reconstructInto(U, direction, orc::iv(start[0], start[1], start[2]),
+                         orc::iv(end[0], end[1], end[2]), L, R);
+    delete sys;
+    std::memcpy(left, L.buf.data(), L.buf.size() * sizeof(double));
+    std::memcpy(right, R.buf.data(), R.buf.size() * sizeof(double));
     return 0;
 }
 

@@ -20,6 +20,10 @@ int oracle_ghost_cells(const oracle_setup* s);
 int oracle_run(const oracle_setup* setup, const double* u0, double* uout, double* t_out,
                long* steps_out, double* dt_hist, long dt_cap);
 int oracle_system_rhs(const oracle_setup* setup, const double* u, double* out, double* ws3);
+int oracle_compute_flux(const oracle_setup* setup, const double* u, const int* start,
+                        const int* end, double* out, double* ws3);
+int oracle_reconstruct(const oracle_setup* setup, const double* u, int direction, const int* start,
+                       const int* end, double* left, double* right);
 int oracle_apply_boundary(const oracle_setup* setup, double* u);
 void oracle_meanvar(long nsamples, long len, const double* samples, double* mean, double* var);
 void oracle_onepoint_moment(long nsamples, long len, const double* samples, double p,

@@ -12,14 +12,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
 
 
-def gpu_available():
-    try:
-        import torch  # noqa: F401  (only used to ask whether a GPU is visible)
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def hip():
     """The product library; GPU tests fail loudly when it is not built."""

@@ -1,13 +1,16 @@
 // The C++ host layer (include/alsvinn_hip.hpp) against the oracle, on the GPU:
-// the reference-style Simulator loop, NumericalFlux::computeFlux, the batched
+// the reference-style Simulator loop, ConservedSystem from NumericalFlux::computeFlux
+// regions on device Volumes (hipMalloc'd per variable), Boundary, the batched
 // Runner with the STLMersenne sampler, MeanVariance, OnePointMoment,
-// BoundedVariation and StructureBasic. Prints one line per
-// case and exits non-zero on the first mismatch.
+// BoundedVariation, StructureBasic and the per-sample Statistics. Prints one
+// line per case and exits non-zero on the first mismatch.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "../../include/alsvinn_hip.hpp"
 #include "../../oracle/oracle.h"
@@ -27,6 +30,45 @@ static int failures = 0;
 static bool same_bits(const std::vector<double>& a, const std::vector<double>& b) {
     return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * 8) == 0;
 }
+
+// A device Volume: one hipMalloc per variable (as a Memory<real> per
+// variable of the reference's Volume).
+struct DeviceVolume {
+    alsvinn_hip_volume v;
+    size_t per_var = 0;
+    DeviceVolume(int nvar, int nx, int ny, int nz, int g) {
+        std::memset(&v, 0, sizeof(v));
+        v.nvar = nvar;
+        v.precision = ALSVINN_HIP_FP64;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.gx = g;
+        v.gy = ny > 1 ? g : 0;
+        v.gz = nz > 1 ? g : 0;
+        per_var = (size_t)(nx + 2 * v.gx) * (ny + 2 * v.gy) * (nz + 2 * v.gz);
+        for (int k = 0; k < nvar; ++k)
+            if (hipMalloc(&v.var[k], per_var * 8) != hipSuccess) throw std::runtime_error("hipMalloc");
+    }
+    ~DeviceVolume() {
+        for (int k = 0; k < v.nvar; ++k) (void)hipFree(v.var[k]);
+    }
+    void upload(const std::vector<double>& host) {
+        for (int k = 0; k < v.nvar; ++k)
+            if (hipMemcpy(v.var[k], host.data() + k * per_var, per_var * 8, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                throw std::runtime_error("hipMemcpy");
+    }
+    std::vector<double> download() {
+        std::vector<double> host(per_var * v.nvar);
+        if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize");
+        for (int k = 0; k < v.nvar; ++k)
+            if (hipMemcpy(host.data() + k * per_var, v.var[k], per_var * 8, hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                throw std::runtime_error("hipMemcpy");
+        return host;
+    }
+};
 
 static oracle_setup setup_of(const alsvinn_hip_scheme& s, const alsvinn_hip_cxx::Grid& g, double T) {
     oracle_setup o;
@@ -170,27 +212,46 @@ int main() {
             sok = std::fabs(sb[i] - sbref[i] / 3) <= 1e-12 * std::fabs(sbref[i] / 3);
         EXPECT(sok, "structure_basic within 1e-12");
     }
-    // 3. NumericalFlux::computeFlux vs ConservedSystem
+    // 3. ConservedSystem from NumericalFlux::computeFlux regions on device
+    //    Volumes, and Boundary, vs the oracle
     {
         std::vector<double> X(40);
         oracle_stl_mersenne_uniform(5, 1, 40, 0.0, 1.0, X.data());
         auto u = kh(32, X);
         for (size_t i = 0; i < u.size() / 4; ++i) u[i] *= 1.0 + 0.01 * std::sin(0.37 * i);
         oracle_setup o = setup_of(kh_s, g2, 1.0);
-        oracle_apply_boundary(&o, u.data());
-        std::vector<double> ref(u.size(), 0.0), out;
-        double wsr[3], ws[3];
-        oracle_system_rhs(&o, u.data(), ref.data(), wsr);
+        auto ub = u;
+        oracle_apply_boundary(&o, ub.data());
+        DeviceVolume U(4, 32, 32, 1, 2), O(4, 32, 32, 1, 2);
+        U.upload(u);
+        alsfvm::hip::Boundary(ALSVINN_HIP_BC_PERIODIC).applyBoundaryConditions(U.v);
+        EXPECT(same_bits(U.download(), ub), "Boundary::applyBoundaryConditions bit-identical");
+        std::vector<double> ref(u.size(), 0.0);
+        O.upload(ref);
+        double wsr[3], ws[3] = {0, 0, 0};
+        oracle_system_rhs(&o, ub.data(), ref.data(), wsr);
         alsfvm::hip::NumericalFlux flux(kh_s, g2);
-        flux.computeFlux(u, ws, true, out);
-        bool ok = ws[0] == wsr[0] && ws[1] == wsr[1];
-        for (int v = 0; v < 4; ++v)
-            for (int j = 2; j < 34; ++j)
-                for (int i = 2; i < 34; ++i) {
-                    const size_t c = (size_t)v * 36 * 36 + (size_t)j * 36 + i;
-                    ok = ok && std::memcmp(&out[c], &ref[c], 8) == 0;
-                }
-        EXPECT(ok, "computeFlux divergence and wave speeds bit-identical");
+        alsfvm::hip::ConservedSystem system(flux);
+        system(U.v, ws, true, O.v);
+        EXPECT(flux.getNumberOfGhostCells() == 2, "getNumberOfGhostCells");
+        EXPECT(ws[0] == wsr[0] && ws[1] == wsr[1], "ConservedSystem wave speeds bit-identical");
+        EXPECT(same_bits(O.download(), ref), "ConservedSystem divergence bit-identical (every cell)");
+        // per-sample Statistics over two device samples vs the oracle's meanvar
+        DeviceVolume V(4, 32, 32, 1, 2);
+        auto u2 = ub;
+        for (auto& x : u2) x *= 1.25;
+        V.upload(u2);
+        alsuq::hip::Statistics mv(ALSVINN_HIP_STAT_MEANVAR, U.v);
+        mv.computeStatistics(U.v);
+        mv.computeStatistics(V.v);
+        mv.combineStatistics(2);
+        mv.finalizeStatistics();
+        std::vector<double> both(ub);
+        both.insert(both.end(), u2.begin(), u2.end());
+        std::vector<double> mref(ub.size()), vref(ub.size());
+        oracle_meanvar(2, (long)ub.size(), both.data(), mref.data(), vref.data());
+        EXPECT(same_bits(mv.getStatistics(0), mref) && same_bits(mv.getStatistics(1), vref),
+               "per-sample Statistics meanvar bit-identical");
     }
     // 4. error behaviour: an unknown flux throws
     {

@@ -58,6 +58,9 @@ def lib():
         L.oracle_run.argtypes = [sp, dp, dp, dp, lp, dp, ctypes.c_long]
         L.oracle_system_rhs.argtypes = [sp, dp, dp, dp]
         L.oracle_apply_boundary.argtypes = [sp, dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_compute_flux.argtypes = [sp, dp, ip, ip, dp, dp]
+        L.oracle_reconstruct.argtypes = [sp, dp, ctypes.c_int, ip, ip, dp, dp]
         L.oracle_obeys_constraints.argtypes = [sp, dp]
         L.oracle_face_flux.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, dp, dp, dp]
         L.oracle_face_flux.restype = ctypes.c_double
@@ -140,6 +143,34 @@ def system_rhs(setup, u, out_init=None):
     rc = lib().oracle_system_rhs(ctypes.byref(setup), _dp(u), _dp(out), _dp(ws))
     assert rc == 0
     return out, ws
+
+
+def _ivec(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def compute_flux(setup, u, start=(0, 0, 0), end=(0, 0, 0), out=None):
+    """NumericalFlux::computeFlux over the region (start, end); `out` holds
+    the output volume's prior contents (zeros by default). Returns (out, ws)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u) if out is None else np.array(out, dtype=np.float64, copy=True)
+    ws = np.zeros(3)
+    rc = lib().oracle_compute_flux(ctypes.byref(setup), _dp(u), _ivec(start), _ivec(end), _dp(out),
+                                   _dp(ws))
+    assert rc == 0
+    return out, ws
+
+
+def reconstruct(setup, u, direction, start=(0, 0, 0), end=(0, 0, 0), left=None, right=None):
+    """Reconstruction::performReconstruction of the setup's reconstruction;
+    left/right hold the prior contents (zeros by default)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    left = np.zeros_like(u) if left is None else np.array(left, dtype=np.float64, copy=True)
+    right = np.zeros_like(u) if right is None else np.array(right, dtype=np.float64, copy=True)
+    rc = lib().oracle_reconstruct(ctypes.byref(setup), _dp(u), direction, _ivec(start), _ivec(end),
+                                  _dp(left), _dp(right))
+    assert rc == 0
+    return left, right
 
 
 def apply_boundary(setup, u):
