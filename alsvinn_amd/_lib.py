@@ -31,6 +31,12 @@ EXPORTS = [
     "alsvinn_hip_pool_create",
     "alsvinn_hip_pool_assign",
     "alsvinn_hip_pool_release",
+    "alsvinn_hip_pool_upload",
+    "alsvinn_hip_pool_apply_boundary",
+    "alsvinn_hip_pool_download",
+    "alsvinn_hip_pool_snapshot",
+    "alsvinn_hip_pool_poll",
+    "alsvinn_hip_device_memory",
     "alsvinn_hip_ensemble_states",
     "alsvinn_hip_ensemble_layout",
     "alsvinn_hip_ensemble_upload",
@@ -55,6 +61,7 @@ EXPORTS = [
     "alsvinn_hip_ensemble_dt_history",
     "alsvinn_hip_ensemble_rhs",
     "alsvinn_hip_ensemble_last_elapsed_ms",
+    "alsvinn_hip_ensemble_interval_ms",
     "alsvinn_hip_stats_meanvar",
     "alsvinn_hip_stats_onepoint_moment",
     "alsvinn_hip_stats_bounded_variation",
@@ -156,10 +163,16 @@ def load():
                                              ctypes.POINTER(Sample), llp]
     L.alsvinn_hip_ensemble_create_bound.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int,
                                                     ctypes.POINTER(Sample), vp, vp, ctypes.POINTER(vp)]
-    L.alsvinn_hip_pool_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int,
+    L.alsvinn_hip_pool_create.argtypes = [ctypes.POINTER(Scheme), ctypes.c_int, ip,
                                           ctypes.c_longlong, vp, ctypes.POINTER(vp)]
     L.alsvinn_hip_pool_assign.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Sample), ctypes.c_longlong]
     L.alsvinn_hip_pool_release.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_pool_upload.argtypes = [vp, ctypes.c_int, dp]
+    L.alsvinn_hip_pool_apply_boundary.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_pool_download.argtypes = [vp, ctypes.c_int, dp]
+    L.alsvinn_hip_pool_snapshot.argtypes = [vp, llp]
+    L.alsvinn_hip_pool_poll.argtypes = [vp, ctypes.c_longlong, ctypes.c_int, dp, llp, ip, ip, ip]
+    L.alsvinn_hip_device_memory.argtypes = [llp, llp]
     L.alsvinn_hip_ensemble_states.argtypes = [vp, dp, llp, ip, ip]
     L.alsvinn_hip_ensemble_layout.argtypes = [vp, ctypes.c_int, ip, llp, llp]
     L.alsvinn_hip_ensemble_upload.argtypes = [vp, ctypes.c_int, dp]
@@ -182,6 +195,7 @@ def load():
     L.alsvinn_hip_ensemble_dt_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_longlong]
     L.alsvinn_hip_ensemble_rhs.argtypes = [vp, ctypes.c_int, dp, dp]
     L.alsvinn_hip_ensemble_last_elapsed_ms.argtypes = [vp, dp]
+    L.alsvinn_hip_ensemble_interval_ms.argtypes = [vp, vp, dp, dp]
     L.alsvinn_hip_stats_meanvar.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, dp, dp]
     L.alsvinn_hip_stats_onepoint_moment.argtypes = [vp, ip, ctypes.c_int, ctypes.c_double, dp]
     L.alsvinn_hip_stats_bounded_variation.argtypes = [vp, ip, ctypes.c_int, ctypes.c_double, dp]

@@ -54,6 +54,18 @@ __global__ void bookkeeping_kernel(const SampleDesc* __restrict__ desc, SampleSt
     state[s] = S;
 }
 
+// Pool slot (re)assignment in stream order: descriptor, fresh state and
+// cleared wave-speed slots, passed by value.
+__global__ void slot_update_kernel(SampleDesc* desc, SampleState* state, unsigned long long* ws,
+                                   int slot, SampleDesc D, SampleState S) {
+    if (threadIdx.x != 0) return;
+    desc[slot] = D;
+    state[slot] = S;
+    ws[3 * slot + 0] = 0;
+    ws[3 * slot + 1] = 0;
+    ws[3 * slot + 2] = 0;
+}
+
 __global__ void set_hold_kernel(SampleState* state, int nsamples, int k, long long limit) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsamples) state[s].hold[k] = limit;
@@ -291,6 +303,22 @@ struct KernelTimer {
     }
 };
 
+// Work list of a stage launch: an ensemble launches one wave per task; a
+// pool launches its fixed grid, which reads the current number of tasks
+// from the device (a.ntasks is then the 3D kernel's grid size).
+template <class T>
+int set_tasks(const alsvinn_hip_ensemble* e, StageArgs<T>& a) {
+    a.tasks = e->d_tasks;
+    if (e->pool) {
+        a.ntasks_dev = (const int*)e->d_taskbuf;
+        a.ntasks = e->pool_blocks;
+        return e->pool_blocks;
+    }
+    a.ntasks_dev = nullptr;
+    a.ntasks = e->ntasks;
+    return (e->ntasks + 3) / 4;
+}
+
 template <class T>
 int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm = nullptr) {
     StageArgs<T> a;
@@ -298,10 +326,8 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     a.desc = e->d_desc;
     a.state = e->d_state;
     a.ws = e->d_ws;
-    a.tasks = e->d_tasks;
-    a.ntasks = e->ntasks;
     a.gamma = e->scheme.gamma;
-    const int blocks = (e->ntasks + 3) / 4;
+    const int blocks = set_tasks(e, a);
     T* U = (T*)e->U;
     T* A = (T*)e->A;
     T* B = (T*)e->B;
@@ -416,9 +442,52 @@ void destroy_graph(alsvinn_hip_ensemble* e) {
     e->graph_steps = 0;
 }
 
+// Device work list of a pool, capacity `cap` tasks: an int count (16 bytes
+// with padding) followed by the tasks, and its pinned staging buffers.
+int alloc_pool_tasks(alsvinn_hip_ensemble* e, long long cap) {
+    for (int j = 0; j < alsvinn_hip_ensemble::kRing; ++j) {
+        if (e->stage_ev[j]) HIP_TRY(hipEventSynchronize(e->stage_ev[j]));
+        if (e->h_stage[j]) (void)hipHostFree(e->h_stage[j]);
+        e->h_stage[j] = nullptr;
+    }
+    (void)hipFree(e->d_taskbuf);
+    e->d_taskbuf = nullptr;
+    e->d_tasks = nullptr;
+    const size_t bytes = 16 + sizeof(WaveTask) * (size_t)std::max<long long>(1, cap);
+    HIP_TRY(hipMalloc(&e->d_taskbuf, bytes));
+    HIP_TRY(hipMemset(e->d_taskbuf, 0, 16));
+    for (int j = 0; j < alsvinn_hip_ensemble::kRing; ++j)
+        HIP_TRY(hipHostMalloc(&e->h_stage[j], bytes));
+    e->d_tasks = (WaveTask*)(e->d_taskbuf + 16);
+    e->task_cap = cap;
+    e->pool_blocks = (int)std::max<long long>(
+        1, std::min<long long>(1024, e->dim == 3 ? cap : (cap + 3) / 4));
+    return 0;
+}
+
 int upload_tasks(alsvinn_hip_ensemble* e) {
     std::vector<WaveTask> tasks;
     build_tasks(e, tasks);
+    if (e->pool) {
+        // asynchronous, in stream order: steps enqueued before see the old
+        // list, steps enqueued after the new one
+        if ((long long)tasks.size() > e->task_cap) {  // a schedule change outgrew the buffer
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            destroy_graph(e);
+            if (int rc = alloc_pool_tasks(e, 2 * (long long)tasks.size())) return rc;
+        }
+        const int j = e->stage_next;
+        e->stage_next = (j + 1) % alsvinn_hip_ensemble::kRing;
+        HIP_TRY(hipEventSynchronize(e->stage_ev[j]));  // its previous copy has run
+        const int count = (int)tasks.size();
+        std::memcpy(e->h_stage[j], &count, sizeof(int));
+        if (count) std::memcpy(e->h_stage[j] + 16, tasks.data(), sizeof(WaveTask) * tasks.size());
+        HIP_TRY(hipMemcpyAsync(e->d_taskbuf, e->h_stage[j], 16 + sizeof(WaveTask) * tasks.size(),
+                               hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipEventRecord(e->stage_ev[j], e->stream));
+        e->ntasks = count;
+        return 0;
+    }
     hipFree(e->d_tasks);
     e->d_tasks = nullptr;
     e->ntasks = (int)tasks.size();
@@ -445,10 +514,8 @@ void time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps, doubl
     a.desc = e->d_desc;
     a.state = e->d_state;
     a.ws = e->d_ws;
-    a.tasks = e->d_tasks;
-    a.ntasks = e->ntasks;
     a.gamma = e->scheme.gamma;
-    const int blocks = (e->ntasks + 3) / 4;
+    const int blocks = set_tasks(e, a);
     T* U = (T*)e->U;
     T* A = (T*)e->A;
     T* B = (T*)e->B;
@@ -539,13 +606,23 @@ static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* sche
     HIP_TRY_C(hipMalloc(&e->d_count, sizeof(int)));
     HIP_TRY_C(hipHostMalloc(&e->h_count, sizeof(int)));
     HIP_TRY_C(hipMemcpy(e->d_desc, e->desc.data(), sizeof(SampleDesc) * nsamples, hipMemcpyHostToDevice));
-    std::vector<WaveTask> tasks;
-    build_tasks(e, tasks);
-    e->ntasks = (int)tasks.size();
-    if (!tasks.empty()) {
-        HIP_TRY_C(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
-        HIP_TRY_C(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(),
-                            hipMemcpyHostToDevice));
+    if (e->pool) {
+        for (int j = 0; j < alsvinn_hip_ensemble::kRing; ++j) {
+            HIP_TRY_C(hipEventCreateWithFlags(&e->stage_ev[j], hipEventDisableTiming));
+            HIP_TRY_C(hipEventCreateWithFlags(&e->snap_ev[j], hipEventDisableTiming));
+            HIP_TRY_C(hipHostMalloc(&e->h_snap[j], sizeof(SampleState) * nsamples));
+        }
+        if (int r = alloc_pool_tasks(e, e->task_cap)) return cleanup(r);
+        e->ntasks = 0;
+    } else {
+        std::vector<WaveTask> tasks;
+        build_tasks(e, tasks);
+        e->ntasks = (int)tasks.size();
+        if (!tasks.empty()) {
+            HIP_TRY_C(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
+            HIP_TRY_C(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(),
+                                hipMemcpyHostToDevice));
+        }
     }
     HIP_TRY_C(hipEventCreate(&e->ev0));
     HIP_TRY_C(hipEventCreate(&e->ev1));
@@ -649,7 +726,17 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     hipFree(e->d_desc);
     hipFree(e->d_state);
     hipFree(e->d_ws);
-    hipFree(e->d_tasks);
+    if (e->pool) {
+        (void)hipFree(e->d_taskbuf);
+        for (int j = 0; j < alsvinn_hip_ensemble::kRing; ++j) {
+            if (e->h_stage[j]) (void)hipHostFree(e->h_stage[j]);
+            if (e->h_snap[j]) (void)hipHostFree(e->h_snap[j]);
+            if (e->stage_ev[j]) (void)hipEventDestroy(e->stage_ev[j]);
+            if (e->snap_ev[j]) (void)hipEventDestroy(e->snap_ev[j]);
+        }
+    } else {
+        hipFree(e->d_tasks);
+    }
     hipFree(e->d_count);
     hipFree(e->d_dt_hist);
     if (e->h_count) hipHostFree(e->h_count);
@@ -660,10 +747,11 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     return 0;
 }
 
-int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int dim,
+int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, const int max_n[3],
                             long long arena_elements, void* stream, alsvinn_hip_ensemble** out) {
     if (int rc = validate(scheme)) return rc;
-    if (!out || nslots < 1 || dim < 1 || dim > 3 || arena_elements < 1)
+    if (!out || nslots < 1 || !max_n || max_n[0] < 1 || max_n[1] < 1 || max_n[2] < 1 ||
+        (max_n[2] > 1 && max_n[1] < 2) || arena_elements < 1)
         return fail(ALSVINN_HIP_E_INVALID, "bad pool arguments");
     *out = nullptr;
     auto* e = new alsvinn_hip_ensemble();
@@ -672,7 +760,22 @@ int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int di
     e->nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
     e->ghosts = ghost_cells(*scheme);
     e->esize = scheme->precision == ALSVINN_HIP_FP64 ? 8 : 4;
-    e->dim = dim;
+    e->dim = 1 + (max_n[1] > 1) + (max_n[2] > 1);
+    e->pool = true;
+    for (int a = 0; a < 3; ++a) e->max_n[a] = max_n[a];
+    // fixed band heights (rows per 2D task, planes per 3D task), so that the
+    // work list of every slot is bounded by that of the largest grid
+    e->band_rows = e->dim == 1 ? 1 : 16;
+    const int G = e->ghosts, W = 64 - 2 * G;
+    long long per_slot = (max_n[0] + W - 1) / W;
+    if (e->dim == 2) per_slot *= (max_n[1] + e->band_rows - 1) / e->band_rows;
+    if (e->dim == 3)
+        per_slot *= (long long)((max_n[1] + kGroupRows - 1) / kGroupRows) *
+                    ((max_n[2] + e->band_rows - 1) / e->band_rows);
+    e->task_cap = per_slot * nslots;
+    // the elementwise kernels' grid is sized once, for a sample the size of
+    // the arena's share of one variable
+    e->max_vstride = std::max<long long>(1, arena_elements / e->nvar);
     alsvinn_hip_sample blank;
     std::memset(&blank, 0, sizeof(blank));
     SampleDesc D;
@@ -688,11 +791,15 @@ int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int di
 
 int alsvinn_hip_pool_assign(alsvinn_hip_ensemble* e, int slot, const alsvinn_hip_sample* p,
                             long long offset) {
-    if (!e || !p || slot < 0 || slot >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad slot");
+    if (!e || !e->pool || !p || slot < 0 || slot >= e->nsamples)
+        return fail(ALSVINN_HIP_E_INVALID, "bad slot");
+    if (e->assigned[slot]) return fail(ALSVINN_HIP_E_INVALID, "slot is in use");
     const int G = e->ghosts;
     const int dim = 1 + (p->ny > 1) + (p->nz > 1);
     if (dim != e->dim || p->nx < G || (p->ny > 1 && p->ny < G) || (p->nz > 1 && p->nz < G))
         return fail(ALSVINN_HIP_E_INVALID, "grid does not fit the pool's dimension or ghost cells");
+    if (p->nx > e->max_n[0] || p->ny > e->max_n[1] || p->nz > e->max_n[2])
+        return fail(ALSVINN_HIP_E_INVALID, "grid larger than the pool's largest grid");
     if (p->num_adjusters < 0 || p->num_adjusters > 2)
         return fail(ALSVINN_HIP_E_INVALID, "num_adjusters must be 0..2");
     const SampleDesc D = make_desc(e->scheme, G, *p, offset);
@@ -704,38 +811,123 @@ int alsvinn_hip_pool_assign(alsvinn_hip_ensemble* e, int slot, const alsvinn_hip
         if (offset < a1 && offset + D.vstride * e->nvar > a0)
             return fail(ALSVINN_HIP_E_INVALID, "arena range overlaps slot " + std::to_string(s));
     }
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    destroy_graph(e);
     e->desc[slot] = D;
     e->spec[slot] = *p;
     e->assigned[slot] = 1;
-    e->max_vstride = 0;
-    for (int s = 0; s < e->nsamples; ++s)
-        if (e->assigned[s]) e->max_vstride = std::max(e->max_vstride, e->desc[s].vstride);
-    const SampleState S = fresh_state(*p);
-    const unsigned long long z[3] = {0, 0, 0};
-    HIP_TRY(hipMemcpy(e->d_desc + slot, &D, sizeof(D), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->d_state + slot, &S, sizeof(S), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->d_ws + 3 * slot, z, sizeof(z), hipMemcpyHostToDevice));
+    // in stream order: steps already enqueued still see the slot unassigned
+    hipLaunchKernelGGL(slot_update_kernel, dim3(1), dim3(64), 0, e->stream, e->d_desc, e->d_state,
+                       e->d_ws, slot, D, fresh_state(*p));
+    HIP_TRY(hipGetLastError());
     return upload_tasks(e);
 }
 
 int alsvinn_hip_pool_release(alsvinn_hip_ensemble* e, int slot) {
-    if (!e || slot < 0 || slot >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad slot");
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    destroy_graph(e);
+    if (!e || !e->pool || slot < 0 || slot >= e->nsamples) return fail(ALSVINN_HIP_E_INVALID, "bad slot");
     e->assigned[slot] = 0;
     SampleState S;
     std::memset(&S, 0, sizeof(S));
     SampleDesc D;
     std::memset(&D, 0, sizeof(D));
     e->desc[slot] = D;
-    HIP_TRY(hipMemcpy(e->d_state + slot, &S, sizeof(S), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->d_desc + slot, &D, sizeof(D), hipMemcpyHostToDevice));
-    e->max_vstride = 0;
-    for (int s = 0; s < e->nsamples; ++s)
-        if (e->assigned[s]) e->max_vstride = std::max(e->max_vstride, e->desc[s].vstride);
+    hipLaunchKernelGGL(slot_update_kernel, dim3(1), dim3(64), 0, e->stream, e->d_desc, e->d_state,
+                       e->d_ws, slot, D, S);
+    HIP_TRY(hipGetLastError());
     return upload_tasks(e);
+}
+
+int alsvinn_hip_pool_upload(alsvinn_hip_ensemble* e, int slot, const double* host) {
+    if (!e || !e->pool || !host || slot < 0 || slot >= e->nsamples || !e->assigned[slot])
+        return fail(ALSVINN_HIP_E_INVALID, "bad upload arguments");
+    const SampleDesc& D = e->desc[slot];
+    const size_t n = (size_t)D.vstride * e->nvar;
+    // a synchronous copy that does not wait for the pool's stream: no step
+    // enqueued before the slot's assignment reads or writes its memory
+    if (e->esize == 8) {
+        HIP_TRY(hipMemcpy(buf_at<double>(e->U, D.base), host, n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+        HIP_TRY(hipMemcpy(buf_at<float>(e->U, D.base), tmp.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int alsvinn_hip_pool_download(alsvinn_hip_ensemble* e, int slot, double* host) {
+    if (!e || !e->pool || !host || slot < 0 || slot >= e->nsamples || !e->assigned[slot])
+        return fail(ALSVINN_HIP_E_INVALID, "bad download arguments");
+    const SampleDesc& D = e->desc[slot];
+    const size_t n = (size_t)D.vstride * e->nvar;
+    // does not wait for the pool's stream: the slot has finished (a snapshot
+    // showed it inactive), and no step writes an inactive slot
+    if (e->esize == 8) {
+        HIP_TRY(hipMemcpy(host, buf_at<double>(e->U, D.base), n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIP_TRY(hipMemcpy(tmp.data(), buf_at<float>(e->U, D.base), n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+    }
+    return 0;
+}
+
+int alsvinn_hip_pool_apply_boundary(alsvinn_hip_ensemble* e, int slot) {
+    if (!e || !e->pool || slot < 0 || slot >= e->nsamples || !e->assigned[slot])
+        return fail(ALSVINN_HIP_E_INVALID, "bad slot");
+    const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->desc[slot].vstride + 1023) / 1024));
+    if (e->esize == 8)
+        hipLaunchKernelGGL(ghost_fill_kernel<double>, dim3(bx, 1), dim3(256), 0, e->stream, e->d_desc,
+                           (double*)e->U, e->nvar, slot);
+    else
+        hipLaunchKernelGGL(ghost_fill_kernel<float>, dim3(bx, 1), dim3(256), 0, e->stream, e->d_desc,
+                           (float*)e->U, e->nvar, slot);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_pool_snapshot(alsvinn_hip_ensemble* e, long long* ticket) {
+    if (!e || !e->pool || !ticket) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const long long tk = e->next_ticket++;
+    const int j = int(tk % alsvinn_hip_ensemble::kRing);
+    HIP_TRY(hipEventSynchronize(e->snap_ev[j]));  // the copy of ticket tk - kRing has run
+    HIP_TRY(hipMemcpyAsync(e->h_snap[j], e->d_state, sizeof(SampleState) * e->nsamples,
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipEventRecord(e->snap_ev[j], e->stream));
+    e->snap_ticket[j] = tk;
+    *ticket = tk;
+    return 0;
+}
+
+int alsvinn_hip_pool_poll(alsvinn_hip_ensemble* e, long long ticket, int wait, double* t,
+                          long long* steps, int* active, int* status, int* ready) {
+    if (!e || !e->pool || !ready || ticket < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    const int j = int(ticket % alsvinn_hip_ensemble::kRing);
+    if (e->snap_ticket[j] != ticket)
+        return fail(ALSVINN_HIP_E_INVALID, "snapshot ticket " + std::to_string(ticket) +
+                                               " was overwritten (at most 3 are kept)");
+    *ready = 0;
+    if (wait) {
+        HIP_TRY(hipEventSynchronize(e->snap_ev[j]));
+    } else {
+        const hipError_t q = hipEventQuery(e->snap_ev[j]);
+        if (q == hipErrorNotReady) return 0;
+        if (q != hipSuccess) return fail(ALSVINN_HIP_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+    }
+    for (int s = 0; s < e->nsamples; ++s) {
+        const SampleState& S = e->h_snap[j][s];
+        if (t) t[s] = S.t;
+        if (steps) steps[s] = S.steps;
+        if (active) active[s] = S.active;
+        if (status) status[s] = S.status;
+    }
+    *ready = 1;
+    return 0;
+}
+
+int alsvinn_hip_device_memory(long long* free_bytes, long long* total_bytes) {
+    size_t f = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&f, &tot));
+    if (free_bytes) *free_bytes = (long long)f;
+    if (total_bytes) *total_bytes = (long long)tot;
+    return 0;
 }
 
 int alsvinn_hip_ensemble_states(alsvinn_hip_ensemble* e, double* t, long long* steps, int* active,
@@ -811,10 +1003,10 @@ int alsvinn_hip_ensemble_apply_boundary(alsvinn_hip_ensemble* e) {
     const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->max_vstride + 1023) / 1024));
     if (e->esize == 8)
         hipLaunchKernelGGL(ghost_fill_kernel<double>, dim3(bx, e->nsamples), dim3(256), 0, e->stream,
-                           e->d_desc, (double*)e->U, e->nvar);
+                           e->d_desc, (double*)e->U, e->nvar, 0);
     else
         hipLaunchKernelGGL(ghost_fill_kernel<float>, dim3(bx, e->nsamples), dim3(256), 0, e->stream,
-                           e->d_desc, (float*)e->U, e->nvar);
+                           e->d_desc, (float*)e->U, e->nvar, 0);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1117,15 +1309,13 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
     HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
     const SampleDesc& D = e->desc[sample];
     const size_t n = (size_t)D.vstride * e->nvar;
-    const int blocks = (e->ntasks + 3) / 4;
     if (e->esize == 8) {
         StageArgs<double> a;
         std::memset(&a, 0, sizeof(a));
         a.desc = e->d_desc;
         a.state = e->d_state;
         a.ws = e->d_ws;
-        a.tasks = e->d_tasks;
-        a.ntasks = e->ntasks;
+        const int blocks = set_tasks(e, a);
         a.gamma = e->scheme.gamma;
         a.mode = MODE_RHS;
         a.in = (double*)e->U;
@@ -1141,8 +1331,7 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
         a.desc = e->d_desc;
         a.state = e->d_state;
         a.ws = e->d_ws;
-        a.tasks = e->d_tasks;
-        a.ntasks = e->ntasks;
+        const int blocks = set_tasks(e, a);
         a.gamma = e->scheme.gamma;
         a.mode = MODE_RHS;
         a.in = (float*)e->U;
@@ -1162,6 +1351,19 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
     if (wave_speeds)
         for (int k = 0; k < 3; ++k) std::memcpy(&wave_speeds[k], &w[k], 8);
     HIP_TRY(hipMemcpy(e->d_ws, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int alsvinn_hip_ensemble_interval_ms(alsvinn_hip_ensemble* e, alsvinn_hip_ensemble* ref,
+                                     double* start_ms, double* end_ms) {
+    if (!e || !ref || !start_ms || !end_ms) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    HIP_TRY(hipEventSynchronize(e->ev1));
+    HIP_TRY(hipEventSynchronize(ref->ev1));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, ref->ev0, e->ev0));
+    HIP_TRY(hipEventElapsedTime(&b, ref->ev0, e->ev1));
+    *start_ms = a;
+    *end_ms = b;
     return 0;
 }
 

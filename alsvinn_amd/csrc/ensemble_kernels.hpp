@@ -115,11 +115,11 @@ __device__ __forceinline__ int ghost_source(int p, int N, int G, int bc) {
 }
 
 // Boundary::applyBoundaryConditions for every sample: fills each ghost cell
-// from its interior source. blockIdx.y selects the sample.
+// from its interior source. blockIdx.y selects the sample (first + y).
 template <class T>
 __global__ __launch_bounds__(256) void ghost_fill_kernel(const SampleDesc* __restrict__ desc,
-                                                        T* U, int nvar) {
-    const int s = blockIdx.y;
+                                                        T* U, int nvar, int first) {
+    const int s = first + blockIdx.y;
     const SampleDesc D = desc[s];
     const long long n = D.vstride;
     T* u = U + D.base;

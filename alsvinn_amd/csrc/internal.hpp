@@ -257,6 +257,25 @@ struct alsvinn_hip_ensemble {
                       alsvinn_hip::kDefaultVariant};
     int band_rows = 0;  // rows per wave task; 0 = heuristic
     bool tuned = false;
+    // pools: the work list lives in a preallocated device buffer (an int
+    // count, then up to task_cap tasks) that asynchronous copies from pinned
+    // staging buffers update, and the stage kernels read the count at run
+    // time, so admitting or releasing a solve neither allocates, nor
+    // synchronises, nor invalidates the captured step graph
+    static constexpr int kRing = 3;
+    bool pool = false;
+    int max_n[3] = {1, 1, 1};        // the largest grid a slot may hold
+    long long task_cap = 0;
+    int pool_blocks = 0;             // fixed grid of the pool's stage launches
+    unsigned char* d_taskbuf = nullptr;
+    unsigned char* h_stage[kRing] = {};
+    hipEvent_t stage_ev[kRing] = {};
+    int stage_next = 0;
+    // state snapshots for non-blocking polls (pinned, one per ticket slot)
+    alsvinn_hip::SampleState* h_snap[kRing] = {};
+    hipEvent_t snap_ev[kRing] = {};
+    long long snap_ticket[kRing] = {-1, -1, -1};
+    long long next_ticket = 0;
     alsvinn_hip::KernelSet<double> k64;
     alsvinn_hip::KernelSet<float> k32;
 };

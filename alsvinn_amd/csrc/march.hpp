@@ -83,6 +83,10 @@ struct StageArgs {
     unsigned long long* ws;  // 3 per sample: bit patterns of non-negative doubles
     const WaveTask* tasks;
     int ntasks;
+    // when set, the number of tasks is read here at run time (pools: the
+    // work list changes as solves come and go, the launch and a captured
+    // graph stay the same; ntasks then is the grid size of the 3D kernel)
+    const int* ntasks_dev;
     int mode;   // StageMode; MODE_RK3_1 forms its stage input as in*c + in2
     int check;  // evaluate the constraint check on the result
     int no_ws;  // MODE_RHS without the wave-speed reduction (RK4 stages 2-4)
@@ -382,7 +386,8 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     Eq eq;
     eq.gamma = T(a.gamma);
 
-    for (int w = wave; w < a.ntasks; w += nwaves) {
+    const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    for (int w = wave; w < ntasks; w += nwaves) {
         const WaveTask& tk = a.tasks[w];
         // this lane's segment
         int seg = -1;
@@ -575,7 +580,8 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     Eq eq;
     eq.gamma = T(a.gamma);
 
-    for (int t = xcd_block(); t < a.ntasks; t += gridDim.x) {  // uniform over the workgroup
+    const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    for (int t = xcd_block(); t < ntasks; t += gridDim.x) {  // uniform over the workgroup
         const WaveTask& tk = a.tasks[t];
         int seg = -1;
         for (int k = 0; k < tk.nseg; ++k)
@@ -807,7 +813,8 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
     Eq eq;
     eq.gamma = T(a.gamma);
 
-    for (int w = wave; w < a.ntasks; w += nwaves) {
+    const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    for (int w = wave; w < ntasks; w += nwaves) {
         const WaveTask& tk = a.tasks[w];
         int seg = -1;
         for (int k = 0; k < tk.nseg; ++k)

@@ -193,6 +193,15 @@ class Ensemble:
         check(_lib.load().alsvinn_hip_ensemble_rhs(self.h, sample, _dp(out), _dp(ws)))
         return out, ws
 
+    def interval_ms(self, ref):
+        """(start, end) of this ensemble's last step/run call on the device,
+        in ms after the start of ref's last call."""
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        check(_lib.load().alsvinn_hip_ensemble_interval_ms(self.h, ref.h, ctypes.byref(a),
+                                                           ctypes.byref(b)))
+        return a.value, b.value
+
     def last_elapsed_ms(self):
         ms = ctypes.c_double()
         check(_lib.load().alsvinn_hip_ensemble_last_elapsed_ms(self.h, ctypes.byref(ms)))
@@ -265,16 +274,22 @@ class Pool(Ensemble):
 
     Solves are placed first-fit by footprint (elements per state buffer) and
     released when their result has been read; the other slots keep stepping.
+    Assigning, uploading into, releasing and downloading a slot do not wait
+    for the pool's stream; `tick` enqueues steps plus a state snapshot and
+    `poll` reads a snapshot without blocking, so the host admits and retires
+    solves while the device steps.
     """
 
-    def __init__(self, scheme, nslots, dim, arena_elements, stream=None):
+    def __init__(self, scheme, nslots, max_n, arena_elements, stream=None):
         L = _lib.load()
         self.scheme = scheme
         self.samples = [None] * nslots
         h = ctypes.c_void_p()
-        check(L.alsvinn_hip_pool_create(ctypes.byref(scheme), nslots, dim, arena_elements, stream,
+        mx = (ctypes.c_int * 3)(*max_n)
+        check(L.alsvinn_hip_pool_create(ctypes.byref(scheme), nslots, mx, arena_elements, stream,
                                         ctypes.byref(h)))
         self.h = h
+        self.max_n = tuple(max_n)
         self.nvar = 1 if scheme.equation == _lib.EQ_BURGERS else scheme.equation + 2
         self.ghosts = L.alsvinn_hip_ghost_cells(ctypes.byref(scheme))
         self.arena = arena_elements
@@ -284,6 +299,19 @@ class Pool(Ensemble):
         g = self.ghosts
         nx, ny, nz = n
         return self.nvar * (nx + 2 * g) * (ny + (2 * g if ny > 1 else 0)) * (nz + (2 * g if nz > 1 else 0))
+
+    def fits(self, sample):
+        """Whether a solve could ever be admitted: its grid within max_n and
+        of the pool's dimension, at least the ghost width, its footprint
+        within the arena."""
+        n = (sample.nx, sample.ny, sample.nz)
+        dim = 1 + (n[1] > 1) + (n[2] > 1)
+        pool_dim = 1 + (self.max_n[1] > 1) + (self.max_n[2] > 1)
+        if dim != pool_dim or any(a > b for a, b in zip(n, self.max_n)):
+            return False
+        if n[0] < self.ghosts or (n[1] > 1 and n[1] < self.ghosts) or (n[2] > 1 and n[2] < self.ghosts):
+            return False
+        return self.footprint(n) <= self.arena
 
     def place(self, size):
         return first_fit(self.allocs.values(), self.arena, size)
@@ -313,6 +341,41 @@ class Pool(Ensemble):
         self.allocs.pop(slot, None)
         self.samples[slot] = None
 
+    def upload(self, slot, u):
+        """Initial state of a just-assigned slot, then its ghost cells."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == self.volume_shape(slot), (u.shape, self.volume_shape(slot))
+        check(_lib.load().alsvinn_hip_pool_upload(self.h, slot, _dp(u)))
+        check(_lib.load().alsvinn_hip_pool_apply_boundary(self.h, slot))
+
+    def download(self, slot):
+        """Result of a finished slot (inactive in a snapshot)."""
+        out = np.empty(self.volume_shape(slot))
+        check(_lib.load().alsvinn_hip_pool_download(self.h, slot, _dp(out)))
+        return out
+
+    def tick(self, nsteps, steps_per_graph):
+        """Enqueues nsteps steps and a state snapshot; returns its ticket."""
+        self.run_steps(nsteps, steps_per_graph)
+        t = ctypes.c_longlong()
+        check(_lib.load().alsvinn_hip_pool_snapshot(self.h, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket, wait=False):
+        """(t, steps, active, status) of a snapshot, or None while its copy
+        has not run (wait=False)."""
+        n = len(self.samples)
+        t = np.zeros(n)
+        steps = np.zeros(n, dtype=np.int64)
+        act = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        ready = ctypes.c_int()
+        check(_lib.load().alsvinn_hip_pool_poll(
+            self.h, ticket, int(wait), _dp(t), steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+            act.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(ready)))
+        return (t, steps, act, st) if ready.value else None
+
     def states(self):
         n = len(self.samples)
         t = np.zeros(n)
@@ -324,3 +387,11 @@ class Pool(Ensemble):
             act.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return t, steps, act, st
+
+
+def device_memory():
+    """(free, total) bytes of device memory (hipMemGetInfo)."""
+    f = ctypes.c_longlong()
+    t = ctypes.c_longlong()
+    check(_lib.load().alsvinn_hip_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value

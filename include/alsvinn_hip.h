@@ -136,15 +136,43 @@ int alsvinn_hip_ensemble_create_bound(const alsvinn_hip_scheme* scheme, int nsam
 
 /* Pools for continuous batching (no counterpart in the reference, whose
  * Runner creates one Simulator per sample): nslots sample slots over an
- * arena of arena_elements state elements (per buffer). A slot is assigned a
+ * arena of arena_elements state elements (per buffer); max_n is the largest
+ * grid a slot may hold (its dimension is the pool's). A slot is assigned a
  * solve at an arena offset (the caller places solves, e.g. first-fit by
  * footprint nvar*(nx+2g)*(ny+2g)*(nz+2g)) and released when its result was
- * read; other slots keep stepping. Unassigned slots cost nothing. */
-int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, int dim,
+ * read; other slots keep stepping. Unassigned slots cost nothing.
+ *
+ * Assignment and release are asynchronous, in the pool's stream order: they
+ * allocate nothing (the work list lives in a device buffer sized at creation
+ * for nslots grids of max_n cells, updated by copies from pinned staging
+ * buffers), do not synchronise the stream, and keep the captured step graph
+ * valid, so a request can be admitted while the pool is stepping. */
+int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, const int max_n[3],
                             long long arena_elements, void* stream, alsvinn_hip_ensemble** out);
 int alsvinn_hip_pool_assign(alsvinn_hip_ensemble* e, int slot, const alsvinn_hip_sample* sample,
                             long long offset);
 int alsvinn_hip_pool_release(alsvinn_hip_ensemble* e, int slot);
+/* Initial state of a just-assigned slot (full padded volume, FP64 on the
+ * host): a synchronous copy that does not wait for the pool's stream (no
+ * step enqueued before the assignment touches the slot's memory). */
+int alsvinn_hip_pool_upload(alsvinn_hip_ensemble* e, int slot, const double* host);
+/* Boundary::applyBoundaryConditions on one slot, in stream order. */
+int alsvinn_hip_pool_apply_boundary(alsvinn_hip_ensemble* e, int slot);
+/* Result of a finished slot (a snapshot showed it inactive; no step writes
+ * an inactive slot), without waiting for the pool's stream. Synchronous. */
+int alsvinn_hip_pool_download(alsvinn_hip_ensemble* e, int slot, double* host);
+/* Non-blocking progress checks: pool_snapshot enqueues a copy of every
+ * slot's t, steps, active flag and status into pinned host memory and
+ * returns its ticket; pool_poll reports it (*ready = 1) once the copy has
+ * run, or returns at once with *ready = 0 (wait = 0), or waits (wait = 1).
+ * The last 3 tickets are kept. */
+int alsvinn_hip_pool_snapshot(alsvinn_hip_ensemble* e, long long* ticket);
+int alsvinn_hip_pool_poll(alsvinn_hip_ensemble* e, long long ticket, int wait, double* t,
+                          long long* steps, int* active, int* status, int* ready);
+
+/* Free and total device memory (hipMemGetInfo), for sizing pools against
+ * the device's memory budget. */
+int alsvinn_hip_device_memory(long long* free_bytes, long long* total_bytes);
 
 /* Per-sample t, steps, active flag and status of every sample/slot in one
  * copy. Synchronous. */
@@ -254,6 +282,12 @@ int alsvinn_hip_ensemble_rhs(alsvinn_hip_ensemble* e, int sample, double* host_o
 /* Time of the kernels enqueued by the last alsvinn_hip_ensemble_step/run
  * call, from HIP events on the ensemble's stream (milliseconds). */
 int alsvinn_hip_ensemble_last_elapsed_ms(alsvinn_hip_ensemble* e, double* ms);
+/* Start and end of e's last step/run call on the device, in milliseconds
+ * after the start of ref's last call (HIP events on the two streams; no
+ * counterpart in the reference): shows whether two ensembles or pools ran at
+ * the same time. Synchronous. */
+int alsvinn_hip_ensemble_interval_ms(alsvinn_hip_ensemble* e, alsvinn_hip_ensemble* ref,
+                                     double* start_ms, double* end_ms);
 
 /* ---- alsuq statistics on the device (accumulated in FP64) ----
  * Each call takes a group of sample indices whose grids are identical and

@@ -150,5 +150,84 @@ def test_empty_ensemble_and_pool_rejected_before_any_gpu_call():
         hip.Ensemble(s, [])
     L = hip._lib.load()
     out = ctypes.c_void_p()
-    assert L.alsvinn_hip_pool_create(ctypes.byref(s), 0, 2, 1000, None, ctypes.byref(out)) != 0
+    mx = (ctypes.c_int * 3)(64, 64, 1)
+    assert L.alsvinn_hip_pool_create(ctypes.byref(s), 0, mx, 1000, None, ctypes.byref(out)) != 0
     assert b"bad pool arguments" in L.alsvinn_hip_last_error()
+
+
+class _FakePool:
+    """A pool without a GPU for the serving loop: every solve finishes after
+    int(end_time * 1000) steps; snapshots are taken at tick time."""
+
+    def __init__(self, nslots, arena, max_n, ghosts=2):
+        self.nslots, self.arena, self.max_n, self.ghosts = nslots, arena, max_n, ghosts
+        self.allocs, self.need, self.done, self.snaps = {}, {}, {}, []
+
+    def footprint(self, n):
+        return (n[0] + 2 * self.ghosts) * ((n[1] + 2 * self.ghosts) if n[1] > 1 else 1)
+
+    def fits(self, s):
+        n = (s.nx, s.ny, s.nz)
+        return (all(a <= b for a, b in zip(n, self.max_n)) and n[0] >= self.ghosts
+                and self.footprint(n) <= self.arena)
+
+    def assign(self, s):
+        from alsvinn_amd.ensemble import first_fit
+        free = [k for k in range(self.nslots) if k not in self.allocs]
+        size = self.footprint((s.nx, s.ny, s.nz))
+        off = first_fit(self.allocs.values(), self.arena, size)
+        if not free or off is None:
+            return None
+        self.allocs[free[0]] = (off, size)
+        self.need[free[0]] = int(s.end_time * 1000)
+        self.done[free[0]] = 0
+        return free[0]
+
+    def upload(self, slot, u):
+        pass
+
+    def release(self, slot):
+        del self.allocs[slot], self.need[slot], self.done[slot]
+
+    def download(self, slot):
+        return np.zeros(1)
+
+    def tick(self, k, g):
+        n = self.nslots
+        steps = np.zeros(n, dtype=np.int64)
+        act = np.zeros(n, dtype=np.int32)
+        for s in self.allocs:
+            self.done[s] = min(self.need[s], self.done[s] + k)
+            steps[s], act[s] = self.done[s], int(self.done[s] < self.need[s])
+        self.snaps.append((np.zeros(n), steps, act, np.zeros(n, dtype=np.int32)))
+        return len(self.snaps) - 1
+
+    def poll(self, ticket, wait=False):
+        return self.snaps[ticket]
+
+    def close(self):
+        pass
+
+
+def test_serving_loop_terminates_and_rejects_unfittable_requests():
+    """serve() with pools that need no GPU: requests larger than a pool's
+    arena or grid, or smaller than the ghost width, are rejected at arrival
+    (status STATUS_REJECTED, no latency) instead of waiting forever; the
+    others all finish, with queueing when the arena is full."""
+    from alsvinn_amd import serving
+    cfg = serving.ServerConfig()
+    cfg.steps_per_tick = 7
+    pools = {"burgers": _FakePool(2, 2 * 104, (100, 1, 1)),
+             "euler2d": _FakePool(2, 2 * 24 * 24, (20, 20, 1))}
+    reqs = serving.make_requests(6, rate=0.0, seed=1, t_range=(0.02, 0.05),
+                                 burgers_n=(40, 100), euler_n=(8, 20))
+    reqs[0].kind, reqs[0].n = "burgers", 500      # larger than the pool's grid and arena
+    reqs[1].kind, reqs[1].n = "euler2d", 1        # smaller than the ghost width
+    for r in reqs[:2]:
+        r.states, r.hurst, r.seed = serving.ic.riemann_parameters(np.full(16, 0.5)), 0.5, 1
+    srv = serving.Server(cfg, pools=pools)
+    srv.serve(reqs)
+    assert [r.status for r in reqs[:2]] == [serving.STATUS_REJECTED] * 2
+    assert all(r.finished is not None and r.status == 0 for r in reqs[2:])
+    assert all(r.steps == int(r.end_time * 1000) for r in reqs[2:])
+    assert serving.latency_stats(reqs)["count"] == 4
