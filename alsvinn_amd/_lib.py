@@ -82,6 +82,7 @@ EXPORTS = [
     "alsvinn_hip_statistics_outputs",
     "alsvinn_hip_statistics_result",
     "alsvinn_hip_statistics_download",
+    "alsvinn_hip_numflux_compute_noalloc",
 ]
 
 STAT_MEANVAR, STAT_ONEPOINTMOMENT, STAT_STRUCTURE_CUBE, STAT_STRUCTURE_BASIC, STAT_BV = 0, 1, 2, 3, 4
@@ -223,6 +224,8 @@ def load():
     L.alsvinn_hip_statistics_outputs.argtypes = [vp, ip, llp]
     L.alsvinn_hip_statistics_result.argtypes = [vp, ctypes.c_int, ctypes.POINTER(dp)]
     L.alsvinn_hip_statistics_download.argtypes = [vp, ctypes.c_int, dp]
+    L.alsvinn_hip_numflux_compute_noalloc.argtypes = [ctypes.POINTER(Scheme), volp, volp, ip, ip,
+                                                      vp, vp]
     _lib = L
     return L
 

@@ -12,6 +12,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -219,6 +220,118 @@ void recon_launch(const alsvinn_hip_operator* op, const KernelSet<T>& k,
     k.op_recon(a, blocks, op->stream);
 }
 
+// ---- allocation-free computeFlux: every piece of device memory the call
+// needs beyond its input and output volumes (the volume's descriptor and
+// run state, three wave-speed slots and the region's work list) lives in
+// this file-scope array, one copy per GPU, held until the process ends.
+// Host code never uses the array's name as an address: it takes the
+// device's pointer from hipGetSymbolAddress (noalloc_scratch_pointer).
+constexpr size_t kScratchBytes = size_t(1) << 20;
+constexpr size_t kScratchDesc = 0, kScratchState = 256, kScratchWs = 512, kScratchTasks = 1024;
+constexpr int kScratchMaxTasks = int((kScratchBytes - kScratchTasks) / sizeof(WaveTask));
+static_assert(sizeof(SampleDesc) <= kScratchState - kScratchDesc, "descriptor slot");
+static_assert(sizeof(SampleState) <= kScratchWs - kScratchState, "state slot");
+static_assert(kScratchTasks % alignof(WaveTask) == 0, "work list alignment");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array, looked up once per device.
+int noalloc_scratch_pointer(unsigned char** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ALSVINN_HIP_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Band heights of region_tasks (rows per 2D task, planes per 3D task).
+int region_band(int G, int dim, const int lo[3], const int hi[3]) {
+    const int W = 64 - 2 * G;
+    const long long strips = (hi[0] - lo[0] + W - 1) / W;
+    const int rows = hi[1] - lo[1], planes = hi[2] - lo[2];
+    auto band = [&](int extent, long long per_band) {
+        int t = extent;
+        while (t > 8 && per_band * ((extent + t - 1) / t) < 4096) t = (t + 1) / 2;
+        return t;
+    };
+    if (dim == 3) return band(planes, strips * ((rows + kGroupRows - 1) / kGroupRows) * kGroupRows);
+    if (dim == 2) return band(rows, strips);
+    return 1;
+}
+
+struct RegionArgs {
+    int lo[3], hi[3];
+    int G, dim, band, ntasks;
+};
+
+// Writes the descriptor, the run state and cleared wave-speed slots, and the
+// region's work list (the tasks of region_tasks, in its order: planes or rows
+// outer, x-strips inner) into the scratch memory at `base`.
+__global__ __launch_bounds__(256) void noalloc_setup_kernel(unsigned char* base, SampleDesc D,
+                                                           RegionArgs r) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        *(SampleDesc*)(base + kScratchDesc) = D;
+        SampleState S;
+        S.t = 0;
+        S.last_dt = 0;
+        S.steps = 0;
+        S.saved[0] = S.saved[1] = 0;
+        S.active = 1;
+        S.status = 0;
+        S.hold[0] = S.hold[1] = LLONG_MAX;
+        *(SampleState*)(base + kScratchState) = S;
+        unsigned long long* ws = (unsigned long long*)(base + kScratchWs);
+        ws[0] = ws[1] = ws[2] = 0;
+    }
+    if (t >= r.ntasks) return;
+    const int W = 64 - 2 * r.G;
+    const int strips = (r.hi[0] - r.lo[0] + W - 1) / W;
+    const int xs = t % strips, outer = t / strips;
+    WaveTask k;
+    k.nseg = 1;
+    k.pad[0] = k.pad[1] = k.pad[2] = 0;
+    for (int j = 0; j < kMaxSeg; ++j) k.sample[j] = k.x0[j] = k.ncols[j] = k.lane0[j] = 0;
+    k.x0[0] = r.lo[0] + xs * W;
+    k.ncols[0] = min(W, r.hi[0] - k.x0[0]);
+    if (r.dim == 3) {
+        const int groups = (r.hi[1] - r.lo[1] + kGroupRows - 1) / kGroupRows;
+        const int yg = outer % groups, zb = outer / groups;
+        k.y0 = r.lo[1] + yg * kGroupRows;
+        k.y1 = min(r.hi[1], k.y0 + kGroupRows);
+        k.z0 = r.lo[2] + zb * r.band;
+        k.z1 = min(r.hi[2], k.z0 + r.band);
+    } else if (r.dim == 2) {
+        k.y0 = r.lo[1] + outer * r.band;
+        k.y1 = min(r.hi[1], k.y0 + r.band);
+        k.z0 = 0;
+        k.z1 = 1;
+    } else {
+        k.y0 = 0;
+        k.y1 = 1;
+        k.z0 = 0;
+        k.z1 = 1;
+    }
+    ((WaveTask*)(base + kScratchTasks))[t] = k;
+}
+
+// Wave speeds of the active directions from the scratch slots (bit patterns
+// of non-negative doubles) into the caller's device array.
+__global__ void noalloc_ws_kernel(const unsigned char* base, double* out, int dim) {
+    const int d = threadIdx.x;
+    if (d < dim) out[d] = __longlong_as_double((long long)((const unsigned long long*)(base + kScratchWs))[d]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -412,6 +525,122 @@ int alsvinn_hip_boundary_apply(int boundary, const alsvinn_hip_volume* volume, v
                            volume->nvar, D);
     }
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int alsvinn_hip_numflux_compute_noalloc(const alsvinn_hip_scheme* scheme,
+                                        const alsvinn_hip_volume* conserved,
+                                        const alsvinn_hip_volume* output, const int start[3],
+                                        const int end[3], double* device_wave_speeds,
+                                        void* stream) {
+    if (int rc = validate(scheme)) return rc;
+    if (!conserved) return fail(ALSVINN_HIP_E_INVALID, "input volume is NULL");
+    const int n[3] = {conserved->nx, conserved->ny, conserved->nz};
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || (n[2] > 1 && n[1] < 2))
+        return fail(ALSVINN_HIP_E_INVALID, "bad grid extents");
+    const int dim = 1 + (n[1] > 1) + (n[2] > 1);
+    const int nvar = scheme->equation == ALSVINN_HIP_EQ_BURGERS ? 1 : scheme->equation + 2;
+    const int G = ghost_cells(*scheme);
+    if (n[0] < G || (n[1] > 1 && n[1] < G) || (n[2] > 1 && n[2] < G))
+        return fail(ALSVINN_HIP_E_INVALID, "grid smaller than the number of ghost cells");
+    if (int rc = check_volume(conserved, nvar, scheme->precision, n, G, "input")) return rc;
+    if (int rc = check_volume(output, nvar, scheme->precision, n, G, "output")) return rc;
+    if (output->gx != conserved->gx || output->gy != conserved->gy || output->gz != conserved->gz)
+        return fail(ALSVINN_HIP_E_INVALID, "input and output volumes need the same ghost layers");
+    for (int v = 0; v < nvar; ++v)
+        for (int w = 0; w < nvar; ++w)
+            if (output->var[v] == conserved->var[w])
+                return fail(ALSVINN_HIP_E_INVALID, "output must not alias the input");
+    RegionArgs r;
+    std::memset(&r, 0, sizeof(r));
+    for (int a = 0; a < 3; ++a) {
+        const int s = start ? start[a] : 0, t = end ? end[a] : 0;
+        r.lo[a] = 0;
+        r.hi[a] = n[a];
+        if (a >= dim) {
+            if (s != 0 || t != 0) return fail(ALSVINN_HIP_E_INVALID, "start/end on an inactive axis");
+            continue;
+        }
+        if (s < 0 || t > 0) return fail(ALSVINN_HIP_E_INVALID, "start must be >= 0 and end <= 0");
+        r.lo[a] = s;
+        r.hi[a] = n[a] + t;
+    }
+    r.G = G;
+    r.dim = dim;
+    const bool empty = r.hi[0] <= r.lo[0] || r.hi[1] <= r.lo[1] || r.hi[2] <= r.lo[2];
+    if (!empty) {
+        r.band = region_band(G, dim, r.lo, r.hi);
+        const long long W = 64 - 2 * G;
+        const long long strips = (r.hi[0] - r.lo[0] + W - 1) / W;
+        long long outer = 1;
+        if (dim == 3)
+            outer = (long long)((r.hi[1] - r.lo[1] + kGroupRows - 1) / kGroupRows) *
+                    ((r.hi[2] - r.lo[2] + r.band - 1) / r.band);
+        else if (dim == 2)
+            outer = (r.hi[1] - r.lo[1] + r.band - 1) / r.band;
+        const long long ntasks = strips * outer;
+        if (ntasks > kScratchMaxTasks)
+            return fail(ALSVINN_HIP_E_INVALID,
+                        "the region needs " + std::to_string(ntasks) +
+                            " work items; the allocation-free computeFlux holds at most " +
+                            std::to_string(kScratchMaxTasks) +
+                            " (1 MiB of device scratch): use alsvinn_hip_numflux_compute");
+        r.ntasks = (int)ntasks;
+    }
+    KernelSet<double> k64;
+    KernelSet<float> k32;
+    const int rc = scheme->precision == ALSVINN_HIP_FP64 ? select_kernels<double>(*scheme, dim, k64)
+                                                         : select_kernels<float>(*scheme, dim, k32);
+    if (rc) return rc;
+    unsigned char* base = nullptr;
+    if (int e = noalloc_scratch_pointer(&base)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    const SampleDesc D = volume_desc(*conserved, scheme->boundary);
+    hipLaunchKernelGGL(noalloc_setup_kernel, dim3(std::max(1, (r.ntasks + 255) / 256)), dim3(256), 0, s,
+                       base, D, r);
+    HIP_TRY(hipGetLastError());
+    if (!empty) {
+        if (scheme->precision == ALSVINN_HIP_FP64) {
+            StageArgs<double> a;
+            std::memset(&a, 0, sizeof(a));
+            a.desc = (const SampleDesc*)(base + kScratchDesc);
+            a.state = (SampleState*)(base + kScratchState);
+            a.ws = (unsigned long long*)(base + kScratchWs);
+            a.tasks = (const WaveTask*)(base + kScratchTasks);
+            a.ntasks = r.ntasks;
+            a.mode = MODE_RHS;
+            a.gamma = scheme->gamma;
+            for (int v = 0; v < nvar; ++v) {
+                a.vin[v] = (const double*)conserved->var[v];
+                a.vout[v] = (double*)output->var[v];
+            }
+            a.in = a.in2 = a.vin[0];
+            a.out = a.vout[0];
+            k64.op_flux(a, (a.ntasks + 3) / 4, s);
+        } else {
+            StageArgs<float> a;
+            std::memset(&a, 0, sizeof(a));
+            a.desc = (const SampleDesc*)(base + kScratchDesc);
+            a.state = (SampleState*)(base + kScratchState);
+            a.ws = (unsigned long long*)(base + kScratchWs);
+            a.tasks = (const WaveTask*)(base + kScratchTasks);
+            a.ntasks = r.ntasks;
+            a.mode = MODE_RHS;
+            a.gamma = scheme->gamma;
+            for (int v = 0; v < nvar; ++v) {
+                a.vin[v] = (const float*)conserved->var[v];
+                a.vout[v] = (float*)output->var[v];
+            }
+            a.in = a.in2 = a.vin[0];
+            a.out = a.vout[0];
+            k32.op_flux(a, (a.ntasks + 3) / 4, s);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (device_wave_speeds) {
+        hipLaunchKernelGGL(noalloc_ws_kernel, dim3(1), dim3(64), 0, s, base, device_wave_speeds, dim);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
 

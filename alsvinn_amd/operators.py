@@ -100,6 +100,17 @@ class NumericalFlux:
             pass
 
 
+def compute_flux_noalloc(scheme, conserved, output, start=(0, 0, 0), end=(0, 0, 0),
+                         device_wave_speeds=None, stream=None):
+    """computeFlux without an operator object and without any allocation
+    (alsvinn_hip_numflux_compute_noalloc): asynchronous on `stream`; the wave
+    speeds go to `device_wave_speeds` (device address of 3 doubles) if given."""
+    check(_lib.load().alsvinn_hip_numflux_compute_noalloc(
+        ctypes.byref(scheme), ctypes.byref(conserved), ctypes.byref(output), _ivec(start),
+        _ivec(end), ctypes.c_void_p(device_wave_speeds) if device_wave_speeds else None,
+        _stream(stream)))
+
+
 def apply_boundary_conditions(boundary, volume, stream=None):
     """Fills every ghost layer of the volume (BC_NEUMANN or BC_PERIODIC)."""
     check(_lib.load().alsvinn_hip_boundary_apply(boundary, ctypes.byref(volume), _stream(stream)))

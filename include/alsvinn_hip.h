@@ -443,6 +443,27 @@ int alsvinn_hip_statistics_result(alsvinn_hip_statistics* st, int index, double*
 /* Host copy of output `index`. Synchronous. */
 int alsvinn_hip_statistics_download(alsvinn_hip_statistics* st, int index, double* host);
 
+/* ---- Allocation-free computeFlux.
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same computation as
+ * alsvinn_hip_numflux_compute for a scheme and one region, without an
+ * operator object: the volume descriptor, the wave-speed slots and the
+ * region's work list live in a library-owned 1 MiB device array per GPU
+ * (held until the process ends), written by a setup kernel on `stream`; the
+ * call makes no allocation, no host copy and no synchronisation. A region
+ * that needs more than about 10900 work items (for example a 1D grid of more
+ * than 650k cells, or 2D/3D grids whose x-strips times row bands exceed it)
+ * is rejected with ALSVINN_HIP_E_INVALID. device_wave_speeds (may be NULL):
+ * device memory of 3 doubles that receives the max wave speed of each active
+ * direction. Calls on one GPU share its array: calls that overlap in time on
+ * that GPU (different streams, or different host threads) race. */
+int alsvinn_hip_numflux_compute_noalloc(const alsvinn_hip_scheme* scheme,
+                                        const alsvinn_hip_volume* conserved,
+                                        const alsvinn_hip_volume* output, const int start[3],
+                                        const int end[3], double* device_wave_speeds,
+                                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif
