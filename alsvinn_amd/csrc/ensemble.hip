@@ -28,32 +28,6 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-// End of a timestep (Simulator::incrementSolution tail, Simulator::atEnd,
-// FixedIntervalWriter::write): t += dt, save counters, stop flag, wave-speed
-// reset. One thread per sample.
-__global__ void bookkeeping_kernel(const SampleDesc* __restrict__ desc, SampleState* state,
-                                   unsigned long long* ws, double* dt_hist, long long dt_cap,
-                                   int nsamples) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsamples) return;
-    const SampleDesc D = desc[s];
-    SampleState S = state[s];
-    if (!runnable(S)) return;
-    const double dt = sample_dt(D, S, ws + 3 * s);
-    if (dt_hist && S.steps < dt_cap) dt_hist[(long long)s * dt_cap + S.steps] = dt;
-    S.t = S.t + dt;
-    S.steps += 1;
-    S.last_dt = dt;
-    for (int k = 0; k < D.nadj; ++k)
-        if (S.t >= double(S.saved[k]) * D.interval[k] + 0.0) S.saved[k] += 1;
-    ws[3 * s + 0] = 0;
-    ws[3 * s + 1] = 0;
-    ws[3 * s + 2] = 0;
-    if (S.status != 0 || S.t >= D.end_time || (D.max_steps >= 0 && S.steps >= D.max_steps))
-        S.active = 0;
-    state[s] = S;
-}
-
 // Pool slot (re)assignment in stream order: descriptor, fresh state and
 // cleared wave-speed slots, passed by value.
 __global__ void slot_update_kernel(SampleDesc* desc, SampleState* state, unsigned long long* ws,
@@ -309,6 +283,7 @@ struct KernelTimer {
 template <class T>
 int set_tasks(const alsvinn_hip_ensemble* e, StageArgs<T>& a) {
     a.tasks = e->d_tasks;
+    a.ws_filter = e->ws_filter;
     if (e->pool) {
         a.ntasks_dev = (const int*)e->d_taskbuf;
         a.ntasks = e->pool_blocks;
@@ -335,6 +310,17 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     T* Dv = (T*)e->D;
     const int bx = (int)std::max<long long>(1, std::min<long long>(64, (e->max_vstride + 1023) / 1024));
     const dim3 pgrid(bx, e->nsamples);
+    // the step's last kernel ends the step (t += dt, counters, stop flags,
+    // wave-speed reset) once all its workgroups are done
+    StepEnd end{};
+    end.counter = e->d_end_counter;
+    end.desc = e->d_desc;
+    end.state = e->d_state;
+    end.ws = e->d_ws;
+    end.dt_hist = e->d_dt_hist;
+    end.dt_cap = e->dt_cap;
+    end.nsamples = e->nsamples;
+    const StepEnd none{};
 
     // stage 1: A = L(U), wave speeds
     a.mode = MODE_RHS;
@@ -347,49 +333,52 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
         k.stage[e->variant[stage_kind(a.mode)]](a, blocks, e->stream);
         if (tm) tm->end(0);
     };
-    auto axpy = [&](const T* X, const T* Y, T* O, int check) {
+    auto axpy = [&](const T* X, const T* Y, T* O, int check, const StepEnd& en) {
         if (tm) tm->begin();
-        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, X, Y, O, check, e->scheme.gamma);
+        k.axpy(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, X, Y, O, check, e->scheme.gamma,
+               en);
         if (tm) tm->end(1);
     };
     stage();
     if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_FORWARD_EULER) {
-        axpy(A, U, U, 1);
+        axpy(A, U, U, 1, end);
     } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK2) {
-        axpy(A, U, A, 0);
+        axpy(A, U, A, 0, none);
         a.mode = MODE_RK2_1;
         a.in = A;
         a.in2 = U;
         a.out = U;
         a.check = 1;
+        a.end = end;
         stage();
     } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK4) {
         // RungeKutta4.cpp:50-96: U1 = ((L0*c)*0.5) + U0, U2 = ((L1*c)*0.5) + U0,
         // U3 = L2*c + U0, U' = (c/6)L3 + U1/3 + 2U2/3 + U3/3 - U0/3; only the
         // first evaluation feeds the CFL condition
-        auto rk4 = [&](int kind, const T* X, const T* Z, const T* W, const T* V, T* O, int check) {
+        auto rk4 = [&](int kind, const T* X, const T* Z, const T* W, const T* V, T* O, int check,
+                       const StepEnd& en) {
             if (tm) tm->begin();
             k.rk4(pgrid, e->stream, e->d_desc, e->d_state, e->d_ws, kind, X, U, Z, W, V, O, check,
-                  e->scheme.gamma);
+                  e->scheme.gamma, en);
             if (tm) tm->end(1);
         };
-        rk4(0, A, nullptr, nullptr, nullptr, B, 0);  // U1
+        rk4(0, A, nullptr, nullptr, nullptr, B, 0, none);  // U1
         a.no_ws = 1;
         a.in = B;
         a.in2 = B;
         a.out = C;
         stage();                                      // L1
-        rk4(0, C, nullptr, nullptr, nullptr, C, 0);  // U2
+        rk4(0, C, nullptr, nullptr, nullptr, C, 0, none);  // U2
         a.in = C;
         a.in2 = C;
         a.out = Dv;
         stage();                                      // L2
-        axpy(Dv, U, Dv, 0);                           // U3
+        axpy(Dv, U, Dv, 0, none);                           // U3
         a.in = Dv;
         a.in2 = Dv;
         a.out = A;
         stage();                                      // L3
-        rk4(1, A, B, C, Dv, U, 1);                    // U^{n+1}
+        rk4(1, A, B, C, Dv, U, 1, end);                    // U^{n+1}
     } else {
         a.mode = MODE_RK3_1;
         a.in = A;
@@ -401,13 +390,9 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
         a.in2 = U;
         a.out = U;
         a.check = 1;
+        a.end = end;
         stage();
     }
-    if (tm) tm->begin();
-    hipLaunchKernelGGL(bookkeeping_kernel, dim3((e->nsamples + 255) / 256), dim3(256), 0,
-                       e->stream, e->d_desc, e->d_state, e->d_ws, e->d_dt_hist, e->dt_cap,
-                       e->nsamples);
-    if (tm) tm->end(2);
     return 0;
 }
 
@@ -491,6 +476,12 @@ int upload_tasks(alsvinn_hip_ensemble* e) {
     hipFree(e->d_tasks);
     e->d_tasks = nullptr;
     e->ntasks = (int)tasks.size();
+    // wavefronts per sample: many on one sample filter their wave-speed atomics
+    std::vector<int> per(e->nsamples, 0);
+    for (const auto& t : tasks)
+        for (int k = 0; k < t.nseg; ++k) per[t.sample[k]] += 1;
+    const int most = per.empty() ? 0 : *std::max_element(per.begin(), per.end());
+    e->ws_filter = most * (e->dim == 3 ? kGroupRows : 1) > kWsFilterWaves ? 1 : 0;
     if (tasks.empty()) return 0;
     HIP_TRY(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
     HIP_TRY(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
@@ -512,7 +503,11 @@ void time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps, doubl
     StageArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.desc = e->d_desc;
-    a.state = e->d_state;
+    // the samples' states are copied for the dry run: the last stage's
+    // constraint check may flag a sample on the scratch volumes' values
+    (void)hipMemcpyAsync(e->d_state_dry, e->d_state, sizeof(SampleState) * e->nsamples,
+                         hipMemcpyDeviceToDevice, e->stream);
+    a.state = e->d_state_dry;
     a.ws = e->d_ws;
     a.gamma = e->scheme.gamma;
     const int blocks = set_tasks(e, a);
@@ -602,8 +597,11 @@ static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* sche
     if (e->D) HIP_TRY_C(hipMemset(e->D, 0, bytes));
     HIP_TRY_C(hipMalloc(&e->d_desc, sizeof(SampleDesc) * nsamples));
     HIP_TRY_C(hipMalloc(&e->d_state, sizeof(SampleState) * nsamples));
+    HIP_TRY_C(hipMalloc(&e->d_state_dry, sizeof(SampleState) * nsamples));
     HIP_TRY_C(hipMalloc(&e->d_ws, sizeof(unsigned long long) * 3 * nsamples));
     HIP_TRY_C(hipMalloc(&e->d_count, sizeof(int)));
+    HIP_TRY_C(hipMalloc(&e->d_end_counter, sizeof(unsigned int)));
+    HIP_TRY_C(hipMemset(e->d_end_counter, 0, sizeof(unsigned int)));
     HIP_TRY_C(hipHostMalloc(&e->h_count, sizeof(int)));
     HIP_TRY_C(hipMemcpy(e->d_desc, e->desc.data(), sizeof(SampleDesc) * nsamples, hipMemcpyHostToDevice));
     if (e->pool) {
@@ -725,6 +723,7 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     hipFree(e->D);
     hipFree(e->d_desc);
     hipFree(e->d_state);
+    hipFree(e->d_state_dry);
     hipFree(e->d_ws);
     if (e->pool) {
         (void)hipFree(e->d_taskbuf);
@@ -738,6 +737,7 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
         hipFree(e->d_tasks);
     }
     hipFree(e->d_count);
+    hipFree(e->d_end_counter);
     hipFree(e->d_dt_hist);
     if (e->h_count) hipHostFree(e->h_count);
     if (e->ev0) hipEventDestroy(e->ev0);
@@ -773,6 +773,7 @@ int alsvinn_hip_pool_create(const alsvinn_hip_scheme* scheme, int nslots, const 
         per_slot *= (long long)((max_n[1] + kGroupRows - 1) / kGroupRows) *
                     ((max_n[2] + e->band_rows - 1) / e->band_rows);
     e->task_cap = per_slot * nslots;
+    e->ws_filter = per_slot * (e->dim == 3 ? kGroupRows : 1) > kWsFilterWaves ? 1 : 0;
     // the elementwise kernels' grid is sized once, for a sample the size of
     // the arena's share of one variable
     e->max_vstride = std::max<long long>(1, arena_elements / e->nvar);
@@ -1083,7 +1084,7 @@ int alsvinn_hip_ensemble_autotune(alsvinn_hip_ensemble* e, int* rows, int* varia
         for (int w = 1; w <= 3; ++w)
             for (int rounds = 1; rounds <= 2; ++rounds) {
                 const long long slots = simds * w * rounds;
-                if (waves_of(maxny) > slots) {
+                if (waves_of(maxny) <= slots) {  // some band height fits
                     int lo = 1, hi = maxny;  // waves_of decreases with h: smallest h that fits
                     while (lo < hi) {
                         const int mid = (lo + hi) / 2;

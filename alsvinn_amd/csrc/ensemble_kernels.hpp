@@ -11,17 +11,11 @@ namespace alsvinn_hip {
 // blockIdx.y selects the sample. With check set, interior cells are tested
 // against the equation's constraints (Simulator::checkConstraints).
 template <class T, class Eq>
-__global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict__ desc,
-                                                  SampleState* state,
-                                                  const unsigned long long* __restrict__ ws,
-                                                  const T* X, const T* Y, T* out, int check,
-                                                  double gamma) {
-    const int s = blockIdx.y;
-    const SampleDesc D = desc[s];
-    SampleState* Sp = state + s;
-    if (!runnable(*Sp)) return;
+__device__ __forceinline__ bool axpy_sample(const SampleDesc& D, SampleState* Sp,
+                                            const unsigned long long* __restrict__ wss, const T* X,
+                                            const T* Y, T* out, int check, double gamma) {
     __shared__ double cs;
-    if (threadIdx.x == 0) cs = sample_dt(D, *Sp, ws + 3 * s) / D.dx[0];
+    if (threadIdx.x == 0) cs = sample_dt(D, *Sp, wss) / D.dx[0];
     __syncthreads();
     const T c = T(cs);
     const long long n = D.vstride;
@@ -49,6 +43,23 @@ __global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict_
         }
     }
     if (bad) atomicOr(&Sp->status, 2);
+    return bad;
+}
+
+// The kernel: blockIdx.y selects the sample, blockIdx.x of gridDim.x shares
+// its cells; `end` ends the step after the work when this is the step's
+// last kernel.
+template <class T, class Eq>
+__global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict__ desc,
+                                                  SampleState* state,
+                                                  const unsigned long long* __restrict__ ws,
+                                                  const T* X, const T* Y, T* out, int check,
+                                                  double gamma, StepEnd end) {
+    const int s = blockIdx.y;
+    const SampleDesc D = desc[s];
+    SampleState* Sp = state + s;
+    const bool bad = runnable(*Sp) && axpy_sample<T, Eq>(D, Sp, ws + 3 * s, X, Y, out, check, gamma);
+    end_step(end, bad);
 }
 
 // Stage combinations of the classical RK4 integrator (RungeKutta4.cpp:50-96)
@@ -59,18 +70,12 @@ __global__ __launch_bounds__(256) void axpy_kernel(const SampleDesc* __restrict_
 //           (HostMemory::addLinearCombination, HostMemory.cpp:266-279),
 //           followed by the constraint check when `check` is set.
 template <class T, class Eq>
-__global__ __launch_bounds__(256) void rk4_kernel(const SampleDesc* __restrict__ desc,
-                                                 SampleState* state,
-                                                 const unsigned long long* __restrict__ ws,
-                                                 int kind, const T* X, const T* Y, const T* Z,
-                                                 const T* W, const T* V, T* out, int check,
-                                                 double gamma) {
-    const int s = blockIdx.y;
-    const SampleDesc D = desc[s];
-    SampleState* Sp = state + s;
-    if (!runnable(*Sp)) return;
+__device__ __forceinline__ bool rk4_sample(const SampleDesc& D, SampleState* Sp,
+                                           const unsigned long long* __restrict__ wss, int kind,
+                                           const T* X, const T* Y, const T* Z, const T* W,
+                                           const T* V, T* out, int check, double gamma) {
     __shared__ double cs;
-    if (threadIdx.x == 0) cs = sample_dt(D, *Sp, ws + 3 * s) / D.dx[0];
+    if (threadIdx.x == 0) cs = sample_dt(D, *Sp, wss) / D.dx[0];
     __syncthreads();
     const T c = T(cs);
     const T a1 = T((1.0 / 6.0) * cs);
@@ -104,6 +109,23 @@ __global__ __launch_bounds__(256) void rk4_kernel(const SampleDesc* __restrict__
         }
     }
     if (bad) atomicOr(&Sp->status, 2);
+    return bad;
+}
+
+// The kernel, laid out as axpy_kernel.
+template <class T, class Eq>
+__global__ __launch_bounds__(256) void rk4_kernel(const SampleDesc* __restrict__ desc,
+                                                 SampleState* state,
+                                                 const unsigned long long* __restrict__ ws,
+                                                 int kind, const T* X, const T* Y, const T* Z,
+                                                 const T* W, const T* V, T* out, int check,
+                                                 double gamma, StepEnd end) {
+    const int s = blockIdx.y;
+    const SampleDesc D = desc[s];
+    SampleState* Sp = state + s;
+    const bool bad =
+        runnable(*Sp) && rk4_sample<T, Eq>(D, Sp, ws + 3 * s, kind, X, Y, Z, W, V, out, check, gamma);
+    end_step(end, bad);
 }
 
 // Source cell of padded position p on one axis (BoundaryCPU.cpp:53-104 with

@@ -23,6 +23,9 @@ int ghost_cells(const alsvinn_hip_scheme& s);
 // dimension, precision) needs.
 constexpr int kStageVariants = 3;  // register budgets (KernelSet::waves gives each one's waves per SIMD)
 constexpr int kDefaultVariant = 1;
+// A sample spanning more wavefronts than this reads its wave-speed slots
+// before raising them (StageArgs::ws_filter).
+constexpr int kWsFilterWaves = 128;
 // Stage kind of a stage mode, for the per-kind register budget.
 inline int stage_kind(int mode) { return mode == MODE_RK3_1 ? 1 : (mode == MODE_RK3_2 ? 2 : 0); }
 
@@ -32,10 +35,11 @@ struct KernelSet {
     int waves[kStageVariants] = {1, 2, 3};  // waves per SIMD of each variant's register budget
     void (*axpy)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                  const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
-                 double gamma) = nullptr;
+                 double gamma, const StepEnd& end) = nullptr;
     void (*rk4)(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                 const unsigned long long* ws, int kind, const T* X, const T* Y, const T* Z,
-                const T* W, const T* V, T* out, int check, double gamma) = nullptr;
+                const T* W, const T* V, T* out, int check, double gamma,
+                const StepEnd& end) = nullptr;
     // volume operators: NumericalFlux::computeFlux on a reference Volume
     // (the first-stage kernel on per-variable memories) and
     // Reconstruction::performReconstruction
@@ -157,17 +161,17 @@ void set_stage(KernelSet<T>& k) {
 template <class T, class Eq>
 void launch_axpy(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                  const unsigned long long* ws, const T* X, const T* Y, T* out, int check,
-                 double gamma) {
+                 double gamma, const StepEnd& end) {
     hipLaunchKernelGGL((axpy_kernel<T, Eq>), grid, dim3(256), 0, s, desc, state, ws, X, Y, out,
-                       check, gamma);
+                       check, gamma, end);
 }
 
 template <class T, class Eq>
 void launch_rk4(dim3 grid, hipStream_t s, const SampleDesc* desc, SampleState* state,
                 const unsigned long long* ws, int kind, const T* X, const T* Y, const T* Z,
-                const T* W, const T* V, T* out, int check, double gamma) {
+                const T* W, const T* V, T* out, int check, double gamma, const StepEnd& end) {
     hipLaunchKernelGGL((rk4_kernel<T, Eq>), grid, dim3(256), 0, s, desc, state, ws, kind, X, Y,
-                       Z, W, V, out, check, gamma);
+                       Z, W, V, out, check, gamma, end);
 }
 
 template <class T, class Eq, class Flux, int DIM>
@@ -237,10 +241,13 @@ struct alsvinn_hip_ensemble {
     void* D = nullptr;  // RK4 U3
     alsvinn_hip::SampleDesc* d_desc = nullptr;
     alsvinn_hip::SampleState* d_state = nullptr;
+    alsvinn_hip::SampleState* d_state_dry = nullptr;  // the autotuner's dry runs
     unsigned long long* d_ws = nullptr;
     alsvinn_hip::WaveTask* d_tasks = nullptr;
     int ntasks = 0;
     int* d_count = nullptr;
+    unsigned int* d_end_counter = nullptr;
+    int ws_filter = 0;  // StageArgs::ws_filter for this work list  // workgroups done in the step's last kernel (0 between launches)
     int* h_count = nullptr;
     double* d_dt_hist = nullptr;
     long long dt_cap = 0;

@@ -76,6 +76,17 @@ struct WaveTask {
     int lane0[kMaxSeg];
 };
 
+// Where the step's last kernel ends the step (end_step).
+struct StepEnd {
+    unsigned int* counter;  // null: no step end in this launch
+    const SampleDesc* desc;
+    SampleState* state;
+    unsigned long long* ws;
+    double* dt_hist;
+    long long dt_cap;
+    int nsamples;
+};
+
 template <class T>
 struct StageArgs {
     const SampleDesc* desc;
@@ -88,8 +99,12 @@ struct StageArgs {
     // graph stay the same; ntasks then is the grid size of the 3D kernel)
     const int* ntasks_dev;
     int mode;   // StageMode; MODE_RK3_1 forms its stage input as in*c + in2
-    int check;  // evaluate the constraint check on the result
+    int check;  // unused by the stage kernels: the step's last stage checks
     int no_ws;  // MODE_RHS without the wave-speed reduction (RK4 stages 2-4)
+    // wave speeds through raise_wave_speed (read, then atomic only if
+    // larger): set when a sample spans many wavefronts, whose atomics on the
+    // sample's slots would otherwise serialise; with few wavefronts per
+    // sample the plain atomic (no wait for the read) is cheaper
     const T* in;
     const T* in2;  // U0
     T* out;
@@ -99,6 +114,7 @@ struct StageArgs {
     // per-variable memories), used instead of in / out
     const T* vin[5];
     T* vout[5];
+    StepEnd end;  // the step's last stage: ends the step after its work
 };
 
 // Memory of one volume: variable v of padded cell i. The ensemble's buffers
@@ -163,6 +179,66 @@ __device__ __forceinline__ double sample_dt(const SampleDesc& D, const SampleSta
     return dt;
 }
 
+// End of a timestep of sample s (Simulator::incrementSolution tail,
+// Simulator::atEnd, FixedIntervalWriter::write): t += dt, save counters,
+// stop flag, wave-speed reset; the dt is recorded when asked.
+__device__ __forceinline__ void finish_sample(const SampleDesc* __restrict__ desc, SampleState* state,
+                                              unsigned long long* ws, double* dt_hist,
+                                              long long dt_cap, int s) {
+    const SampleDesc D = desc[s];
+    SampleState S = state[s];
+    if (!runnable(S)) return;
+    const double dt = sample_dt(D, S, ws + 3 * s);
+    if (dt_hist && S.steps < dt_cap) dt_hist[(long long)s * dt_cap + S.steps] = dt;
+    S.t = S.t + dt;
+    S.steps += 1;
+    S.last_dt = dt;
+    // constant indices: the state stays in registers
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (k < D.nadj && S.t >= double(S.saved[k]) * D.interval[k] + 0.0) S.saved[k] += 1;
+    ws[3 * s + 0] = 0;
+    ws[3 * s + 1] = 0;
+    ws[3 * s + 2] = 0;
+    if (S.status != 0 || S.t >= D.end_time || (D.max_steps >= 0 && S.steps >= D.max_steps))
+        S.active = 0;
+    state[s] = S;
+}
+
+// The end of a step fused into the step's last kernel: every workgroup
+// counts itself out once its work is done, and the last one to finish (its
+// count is the grid size) ends the step of every sample and clears the
+// counter for the next launch. This saves the step a launch; the counter is
+// the only state, and it is zero again after every launch.
+// The only values the last workgroup reads that this launch writes are the
+// status flags of failed constraint checks (`flagged`: this thread set one).
+// So only a workgroup that set one publishes its writes before counting
+// itself out: the threads that set a flag wait for their atomics and write
+// back the L2 (an agent-scope release fence) before the barrier, and the
+// counting thread releases again after it. The others count out with a
+// relaxed atomic, and the last workgroup acquires (its fence synchronises
+// with every released workgroup through the counter's read-modify-write
+// chain). A release per workgroup would write back the L2 once per
+// workgroup, thousands of times per launch.
+__device__ __forceinline__ void end_step(const StepEnd& E, bool flagged) {
+    if (!E.counter) return;
+    __shared__ int s_last;
+    if (flagged) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const bool publish = __syncthreads_or(flagged);
+    if (threadIdx.x == 0) {
+        if (publish) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+        s_last = __hip_atomic_fetch_add(E.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 nb - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the published status flags
+    for (int s = threadIdx.x; s < E.nsamples; s += blockDim.x)
+        finish_sample(E.desc, E.state, E.ws, E.dt_hist, E.dt_cap, s);
+    if (threadIdx.x == 0) __hip_atomic_store(E.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Lane i receives the value of lane i-1 (lane_from_lower) or lane i+1
 // (lane_from_upper); the lane without a source (lane 0, resp. 63) receives
 // 0. DPP wave_shr:1 / wave_shl:1 moves, two 32-bit halves for a double. Lanes
@@ -206,6 +282,18 @@ __device__ __forceinline__ T segment_max(T v, int seg) {
 
 // Stage input at flat index i (variable stride vs): plain or lazy
 // U1 = L0*c + U0.
+// Raises a sample's wave-speed slot to v (the bit pattern of a non-negative
+// double, whose unsigned order is the order of the values). With `filter`
+// the slot is read first and the atomic skipped when it already holds v or
+// more: the result is the same maximum, and a single solve's thousands of
+// wavefronts then issue a few dozen atomics on the slot instead of one
+// each, which would serialise at the slot.
+__device__ __forceinline__ void raise_wave_speed(unsigned long long* slot, double v, int filter) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (!filter || b > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(slot, b);
+}
+
 template <class T, int NV>
 __device__ __forceinline__ void load_strided(const T* __restrict__ in, const T* __restrict__ in2,
                                              bool lazy, T c, long long i, long long vs, T* q) {
@@ -321,40 +409,60 @@ __device__ __forceinline__ void axis_images(int c, int N, int G, int bc, int& a,
     }
 }
 
-// Writes one result cell to its padded position and, with IMAGES, to every
-// ghost cell that mirrors it (at most one image per axis side). The volume
-// operators (computeFlux) write the cell only, as the reference does.
-template <class T, int NV, int DIM, bool IMAGES = true, class Acc>
-__device__ __forceinline__ void store_cell(const Acc& out, const SampleDesc& D, int col, int row,
-                                           int plane, const T* res) {
-    if constexpr (!IMAGES) {
-        const long long o = ((long long)(plane + D.gz) * D.ty + (row + D.gy)) * D.tx + (col + D.gx);
+// The padded column of a lane's cell and of the (at most two) ghost columns
+// that mirror it: computed once per task, since a lane keeps its column
+// while it marches.
+struct ColumnImages {
+    int px, xb, xd;  // xb / xd: -1 when there is none
+};
+__device__ __forceinline__ ColumnImages column_images(const SampleDesc& D, int col) {
+    ColumnImages c;
+    axis_images(col, D.nx, D.gx, D.bc, c.px, c.xb, c.xd);
+    return c;
+}
+
+// Writes one result cell of row `row` (and plane `plane`) to its padded
+// position and, with IMAGES, to every ghost cell that mirrors it. The row
+// and plane are the same for every lane of a wave: their images are
+// computed on the scalar unit and skipped by a uniform branch on the rows
+// that have none (all but the G rows next to each boundary).
+template <class T, int NV, int DIM, bool IMAGES, class Acc>
+__device__ __forceinline__ void store_result(const Acc& out, const SampleDesc& D,
+                                             const ColumnImages& C, int row, int plane,
+                                             const T* res) {
+    row = __builtin_amdgcn_readfirstlane(row);
+    plane = __builtin_amdgcn_readfirstlane(plane);
+    const long long tx = D.tx;
+    auto put_row = [&](int py, int pz) {
+        const long long base = ((long long)pz * D.ty + py) * tx;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) out(v, o) = res[v];
-        return;
-    }
-    int x[3], y[3], z[3];
-    axis_images(col, D.nx, D.gx, D.bc, x[0], x[1], x[2]);
-    y[0] = 0;
-    y[1] = y[2] = -1;
-    z[0] = 0;
-    z[1] = z[2] = -1;
-    if (DIM >= 2) axis_images(row, D.ny, D.gy, D.bc, y[0], y[1], y[2]);
-    if (DIM >= 3) axis_images(plane, D.nz, D.gz, D.bc, z[0], z[1], z[2]);
+        for (int v = 0; v < NV; ++v) out(v, base + C.px) = res[v];
+        if constexpr (IMAGES) {
+            if (C.xb >= 0) {
 #pragma unroll
-    for (int iz = 0; iz < (DIM >= 3 ? 3 : 1); ++iz) {
-        if (z[iz] < 0) continue;
+                for (int v = 0; v < NV; ++v) out(v, base + C.xb) = res[v];
+            }
+            if (C.xd >= 0) {
 #pragma unroll
-        for (int iy = 0; iy < (DIM >= 2 ? 3 : 1); ++iy) {
-            if (y[iy] < 0) continue;
-#pragma unroll
-            for (int ix = 0; ix < 3; ++ix) {
-                if (x[ix] < 0) continue;
-                const long long o = ((long long)z[iz] * D.ty + y[iy]) * D.tx + x[ix];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) out(v, o) = res[v];
+                for (int v = 0; v < NV; ++v) out(v, base + C.xd) = res[v];
             }
         }
+    };
+    int ya = DIM >= 2 ? row + D.gy : 0, yb = -1, yd = -1;
+    int za = DIM >= 3 ? plane + D.gz : 0, zb = -1, zd = -1;
+    if constexpr (IMAGES && DIM >= 2) axis_images(row, D.ny, D.gy, D.bc, ya, yb, yd);
+    if constexpr (IMAGES && DIM >= 3) axis_images(plane, D.nz, D.gz, D.bc, za, zb, zd);
+    auto put_plane = [&](int pz) {
+        put_row(ya, pz);
+        if constexpr (IMAGES && DIM >= 2) {
+            if (yb >= 0) put_row(yb, pz);
+            if (yd >= 0) put_row(yd, pz);
+        }
+    };
+    put_plane(za);
+    if constexpr (IMAGES && DIM >= 3) {
+        if (zb >= 0) put_plane(zb);
+        if (zd >= 0) put_plane(zd);
     }
 }
 
@@ -380,13 +488,17 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
+    constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     const int lane = threadIdx.x & 63;
-    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6;
+    // wave-uniform: the task's fields, the row loop and the row's ghost
+    // images are then scalar
+    const int wave = __builtin_amdgcn_readfirstlane((xcd_block() * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
 
     const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    bool flagged = false;  // this thread set a sample's failure status
     for (int w = wave; w < ntasks; w += nwaves) {
         const WaveTask& tk = a.tasks[w];
         // this lane's segment
@@ -410,6 +522,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         const bool interior = active && li >= G && li < G + ncols;
         const bool xface = active && li >= G - 1 && li < G + ncols;  // face col+1/2
         const int px = in_strip ? col + D.gx : D.gx;                 // padded column
+        const ColumnImages cimg = column_images(D, col);
         const auto in = stage_in<OP>(a, D, false);
         const auto in2 = stage_in<OP>(a, D, true);
         const auto out = stage_out<OP>(a, D);
@@ -525,8 +638,11 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             if (interior) {
                 T res[NV];
                 combine<T, NV>(MODE, c, out_v, yc, u0, res);
-                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, DIM, OP == 0>(out, D, col, r, 0, res);
+                if (CHECK && !eq.obeys(res)) {
+                    atomicOr(&Sp->status, 2);
+                    flagged = true;
+                }
+                store_result<T, NV, DIM, OP == 0>(out, D, cimg, r, 0, res);
             }
             if constexpr (DIM >= 2) {
 #pragma unroll
@@ -545,13 +661,13 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
             const T mx = segment_max(wsx, seg);
             const T my = segment_max(wsy, seg);
             if (in_strip && li == 0 && runnable(*Sp)) {
-                atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
+                raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
                 if (DIM >= 2)
-                    atomicMax(a.ws + 3 * sample + 1,
-                              (unsigned long long)__double_as_longlong(double(my)));
+                    raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
             }
         }
     }
+    if constexpr (CHECK) end_step(a.end, flagged);
 }
 
 // 3D stage kernel. A workgroup of kGroupRows wavefronts owns the x-strips of
@@ -569,6 +685,12 @@ constexpr int kGroupRows = 4;
 template <class T, class Eq, class Recon, class Flux, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
+    constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
+    // FP32 lazy stage: U^n of the two planes ahead of the combination stays
+    // in registers from the lazy load that formed the stage input there,
+    // instead of being loaded again for the combination (FP64 keeps the
+    // reload: twice the registers)
+    constexpr bool KEEP_U0 = LAZY && std::is_same_v<T, float>;
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     constexpr int NW = kGroupRows;
@@ -581,6 +703,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     eq.gamma = T(a.gamma);
 
     const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    bool flagged = false;  // this thread set a sample's failure status
     for (int t = xcd_block(); t < ntasks; t += gridDim.x) {  // uniform over the workgroup
         const WaveTask& tk = a.tasks[t];
         int seg = -1;
@@ -608,6 +731,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         const bool xface = active && row_in && li >= G - 1 && li < G + ncols;
         const int px = in_strip ? col + D.gx : D.gx;
         const int py = row + D.gy;
+        const ColumnImages cimg = column_images(D, col);
         const long long plane_stride = (long long)D.tx * D.ty;
         const auto in = stage_in<OP>(a, D, false);
         const auto in2 = stage_in<OP>(a, D, true);
@@ -625,13 +749,27 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         const int h0 = lo_halo ? tk.y0 - 2 + D.gy : tk.y0 + NW + D.gy;
 
         T zc[NV], zp[NV], Rz[NV], Hm[NV];
+        T u_zc[NV], u_zp[NV];  // KEEP_U0: U^n at planes zc, zp
         {
             const int pz = tk.z0 + D.gz;
             T qa[NV], zm[NV];
             load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 2), qa);
             load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 1), zm);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), zc);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), zp);
+            if constexpr (KEEP_U0) {
+                PendingCell<T, NV> p0, p1;
+                p0.issue(in, in2, LAZY, idx(py, pz));
+                p1.issue(in, in2, LAZY, idx(py, pz + 1));
+                p0.finish(LAZY, c, zc);
+                p1.finish(LAZY, c, zp);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    u_zc[v] = p0.b[v];
+                    u_zp[v] = p1.b[v];
+                }
+            } else {
+                load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), zc);
+                load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), zp);
+            }
             T Lt[NV], Rt[NV], L0[NV], R0[NV];
             Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt, eq.gamma);
             Recon::template apply<T, NV>(zm, zc, zp, L0, R0, eq.gamma);
@@ -652,7 +790,13 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             }
             const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
             T u0[NV];
-            if (MODE != MODE_RHS && interior) {
+            if constexpr (KEEP_U0) {
+                // loaded at idx(py, pz) two planes ago: the same cell (no
+                // clamping reaches the interior planes, which lie G or more
+                // planes inside the padded volume)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) u0[v] = u_zc[v];
+            } else if (MODE != MODE_RHS && interior) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
             }
@@ -765,13 +909,20 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             if (interior) {
                 T res[NV];
                 combine<T, NV>(MODE, c, out_v, zc, u0, res);
-                if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                store_cell<T, NV, 3, OP == 0>(out, D, col, row, zz, res);
+                if (CHECK && !eq.obeys(res)) {
+                    atomicOr(&Sp->status, 2);
+                    flagged = true;
+                }
+                store_result<T, NV, 3, OP == 0>(out, D, cimg, row, zz, res);
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 zc[v] = zp[v];
                 zp[v] = qn[v];
+                if constexpr (KEEP_U0) {
+                    u_zc[v] = u_zp[v];
+                    u_zp[v] = pq.b[v];
+                }
             }
         }
 
@@ -780,13 +931,14 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
             const T my = segment_max(wsy, seg);
             const T mz = segment_max(wsz, seg);
             if (in_strip && li == 0 && runnable(*Sp) && row_in) {
-                atomicMax(a.ws + 3 * sample + 0, (unsigned long long)__double_as_longlong(double(mx)));
-                atomicMax(a.ws + 3 * sample + 1, (unsigned long long)__double_as_longlong(double(my)));
-                atomicMax(a.ws + 3 * sample + 2, (unsigned long long)__double_as_longlong(double(mz)));
+                raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
+                raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
+                raise_wave_speed(a.ws + 3 * sample + 2, double(mz), a.ws_filter);
             }
         }
         __syncthreads();  // the next task reuses the LDS rows
     }
+    if constexpr (CHECK) end_step(a.end, flagged);
 }
 
 // Stage kernel for reconstructions wider than three cells (ENO3, ENO4,
@@ -803,17 +955,19 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
+    constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     constexpr int RAD = recon_radius<Recon>::value;
     constexpr int MD = DIM == 3 ? 2 : 1;  // marched direction (DIM >= 2)
     const int lane = threadIdx.x & 63;
-    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane((xcd_block() * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
 
     const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
+    bool flagged = false;  // this thread set a sample's failure status
     for (int w = wave; w < ntasks; w += nwaves) {
         const WaveTask& tk = a.tasks[w];
         int seg = -1;
@@ -835,6 +989,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
         const bool interior = active && li >= G && li < G + ncols;
         const bool xlane = active && li >= G - 1 && li <= G + ncols;  // reconstructs in x
         const bool xface = active && li >= G - 1 && li < G + ncols;   // face col+1/2
+        const ColumnImages cimg = column_images(D, col);
         const auto in = stage_in<OP>(a, D, false);
         const auto in2 = stage_in<OP>(a, D, true);
         const auto out = stage_out<OP>(a, D);
@@ -987,9 +1142,12 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                         for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
                     }
                     combine<T, NV>(MODE, c, out_v, uin, u0, res);
-                    if (a.check && !eq.obeys(res)) atomicOr(&Sp->status, 2);
-                    store_cell<T, NV, DIM, OP == 0>(out, D, col, DIM == 3 ? o : m,
-                                                    DIM == 3 ? m : 0, res);
+                    if (CHECK && !eq.obeys(res)) {
+                        atomicOr(&Sp->status, 2);
+                        flagged = true;
+                    }
+                    store_result<T, NV, DIM, OP == 0>(out, D, cimg, DIM == 3 ? o : m,
+                                                      DIM == 3 ? m : 0, res);
                 }
                 if constexpr (DIM == 2) {  // advance the window by one row
 #pragma unroll
@@ -1006,10 +1164,11 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
             for (int d = 0; d < DIM; ++d) {
                 const T m = segment_max(ws[d], seg);
                 if (in_strip && li == 0 && runnable(*Sp))
-                    atomicMax(a.ws + 3 * sample + d, (unsigned long long)__double_as_longlong(double(m)));
+                    raise_wave_speed(a.ws + 3 * sample + d, double(m), a.ws_filter);
             }
         }
     }
+    if constexpr (CHECK) end_step(a.end, flagged);
 }
 
 // ---- Reconstruction::performReconstruction on a reference Volume

@@ -179,6 +179,7 @@ int numflux_compute(alsvinn_hip_operator* op, const KernelSet<T>& k, const alsvi
     a.tasks = it->second.first;
     a.ntasks = it->second.second;
     a.mode = MODE_RHS;
+    a.ws_filter = 1;  // one sample, any number of wavefronts
     a.gamma = op->scheme.gamma;
     for (int v = 0; v < op->nvar; ++v) {
         a.vin[v] = (const T*)U.var[v];
@@ -609,6 +610,7 @@ int alsvinn_hip_numflux_compute_noalloc(const alsvinn_hip_scheme* scheme,
             a.tasks = (const WaveTask*)(base + kScratchTasks);
             a.ntasks = r.ntasks;
             a.mode = MODE_RHS;
+            a.ws_filter = 1;  // one sample, any number of wavefronts
             a.gamma = scheme->gamma;
             for (int v = 0; v < nvar; ++v) {
                 a.vin[v] = (const double*)conserved->var[v];
@@ -626,6 +628,7 @@ int alsvinn_hip_numflux_compute_noalloc(const alsvinn_hip_scheme* scheme,
             a.tasks = (const WaveTask*)(base + kScratchTasks);
             a.ntasks = r.ntasks;
             a.mode = MODE_RHS;
+            a.ws_filter = 1;  // one sample, any number of wavefronts
             a.gamma = scheme->gamma;
             for (int v = 0; v < nvar; ++v) {
                 a.vin[v] = (const float*)conserved->var[v];

@@ -252,6 +252,19 @@ class Ensemble:
                                                             direction, _dp(out)))
         return out
 
+    def states(self):
+        """(t, steps, active, status) of every sample (synchronous)."""
+        n = len(self.samples)
+        t = np.zeros(n)
+        steps = np.zeros(n, dtype=np.int64)
+        act = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        check(_lib.load().alsvinn_hip_ensemble_states(
+            self.h, _dp(t), steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+            act.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return t, steps, act, st
+
     def totals(self):
         out = np.zeros((len(self.samples), self.nvar))
         check(_lib.load().alsvinn_hip_ensemble_totals(self.h, _dp(out)))
@@ -376,17 +389,6 @@ class Pool(Ensemble):
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(ready)))
         return (t, steps, act, st) if ready.value else None
 
-    def states(self):
-        n = len(self.samples)
-        t = np.zeros(n)
-        steps = np.zeros(n, dtype=np.int64)
-        act = np.zeros(n, dtype=np.int32)
-        st = np.zeros(n, dtype=np.int32)
-        check(_lib.load().alsvinn_hip_ensemble_states(
-            self.h, _dp(t), steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
-            act.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-        return t, steps, act, st
 
 
 def device_memory():

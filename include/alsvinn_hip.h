@@ -253,7 +253,8 @@ int alsvinn_hip_ensemble_run_steps(alsvinn_hip_ensemble* e, int nsteps, int step
 /* Measurement mode: enqueues nsteps timesteps with HIP events around every
  * kernel (synchronising after each) and returns the summed kernel times in
  * milliseconds and launch counts for [stage kernels, elementwise kernels,
- * bookkeeping]. Synchronous. */
+ * separate step-end kernels]; the last is 0: the step's last kernel ends the
+ * step (time, counters, stop flags). Synchronous. */
 int alsvinn_hip_ensemble_step_timed(alsvinn_hip_ensemble* e, int nsteps, double kernel_ms[3],
                                     long long launches[3]);
 

@@ -161,6 +161,31 @@ def compute_flux(setup, u, start=(0, 0, 0), end=(0, 0, 0), out=None):
     return out, ws
 
 
+def ghosts3(setup):
+    """Ghost layers per axis (none on inactive axes)."""
+    g = ghosts(setup)
+    return (g, g if setup.ny > 1 else 0, g if setup.nz > 1 else 0)
+
+
+def conserved_system_regions(setup):
+    """The computeFlux regions (start, end) of ConservedSystem.cpp:31-103:
+    the inner region (ghost layers cut off each active axis), then for each
+    active direction d the low side (end[d] = -size[d] + 3g) and the high
+    side (start[d] = size[d] - 3g), size the padded extent."""
+    dim = 1 + (setup.ny > 1) + (setup.nz > 1)
+    gv = ghosts3(setup)
+    size = (setup.nx + 2 * gv[0], setup.ny + 2 * gv[1], setup.nz + 2 * gv[2])
+    regs = [(gv, tuple(-x for x in gv))]
+    for d in range(dim):
+        eb = [0, 0, 0]
+        eb[d] = -size[d] + 3 * gv[d]
+        st = [0, 0, 0]
+        st[d] = size[d] - 3 * gv[d]
+        regs.append(((0, 0, 0), tuple(eb)))
+        regs.append((tuple(st), (0, 0, 0)))
+    return regs
+
+
 def reconstruct(setup, u, direction, start=(0, 0, 0), end=(0, 0, 0), left=None, right=None):
     """Reconstruction::performReconstruction of the setup's reconstruction;
     left/right hold the prior contents (zeros by default)."""

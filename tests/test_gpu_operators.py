@@ -63,7 +63,7 @@ def _bits_equal(ref, got, what):
 
 
 def _ghosts3(setup, g):
-    return (g, g if setup.ny > 1 else 0, g if setup.nz > 1 else 0)
+    return O.ghosts3(setup)
 
 
 def _valid_state(setup, rng):
@@ -97,20 +97,9 @@ def _valid_state(setup, rng):
 
 
 def _regions(setup, g):
-    """The computeFlux regions of ConservedSystem.cpp:31-103 (start, end) in
-    cells, plus the whole interior."""
-    dim = 1 + (setup.ny > 1) + (setup.nz > 1)
-    gv = _ghosts3(setup, g)
-    size = (setup.nx + 2 * gv[0], setup.ny + 2 * gv[1], setup.nz + 2 * gv[2])
-    regs = [((0, 0, 0), (0, 0, 0)), (gv, tuple(-x for x in gv))]
-    for d in range(dim):
-        eb = [0, 0, 0]
-        eb[d] = -size[d] + 3 * gv[d]
-        st = [0, 0, 0]
-        st[d] = size[d] - 3 * gv[d]
-        regs.append(((0, 0, 0), tuple(eb)))
-        regs.append((tuple(st), (0, 0, 0)))
-    return regs
+    """The whole interior, then the computeFlux regions of
+    ConservedSystem.cpp:31-103."""
+    return [((0, 0, 0), (0, 0, 0))] + O.conserved_system_regions(setup)
 
 
 FLUX_CASES = [

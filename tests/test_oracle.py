@@ -355,6 +355,94 @@ def test_eno2_known_answer():
     assert left[0, 4] == -0.5
 
 
+def _smooth_state(setup, seed):
+    """An admissible smooth state (Euler: rho, p > 0) with the boundary's ghosts."""
+    shape = O.volume_shape(setup)
+    rng = np.random.default_rng(seed)
+    grids = np.meshgrid(*[np.linspace(0, 1, m) for m in shape[1:]], indexing="ij")
+
+    def field(lo, hi):
+        k = rng.integers(1, 3, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        f = np.prod([np.sin(2 * np.pi * k[a] * grids[a] + ph[a]) for a in range(3)], axis=0)
+        return lo + (hi - lo) * 0.5 * (f + 1)
+
+    u = np.zeros(shape)
+    if setup.equation == O.EQ_BURGERS:
+        u[0] = field(-1, 1)
+    else:
+        nsd = setup.equation
+        rho = field(0.7, 1.8)
+        vel = np.stack([field(-0.5, 0.5) for _ in range(nsd)])
+        p = field(0.6, 2.0)
+        u[0] = rho
+        u[1:1 + nsd] = rho * vel
+        u[-1] = p / (GAMMA - 1) + 0.5 * rho * (vel ** 2).sum(axis=0)
+    return O.apply_boundary(setup, u)
+
+
+REGION_SETUPS = [
+    (O.EQ_EULER1, O.FLUX_HLL, O.RECON_NONE, O.BC_NEUMANN, (23, 1, 1)),
+    (O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.BC_PERIODIC, (17, 13, 1)),
+    (O.EQ_EULER3, O.FLUX_HLL3, O.RECON_ENO3, O.BC_PERIODIC, (11, 9, 10)),
+    (O.EQ_BURGERS, O.FLUX_GODUNOV, O.RECON_ENO2, O.BC_PERIODIC, (31, 1, 1)),
+]
+
+
+@pytest.mark.parametrize("setup_args", REGION_SETUPS, ids=["euler1", "euler2", "euler3", "burgers"])
+def test_compute_flux_regions_compose_the_system_rhs(setup_args):
+    """The oracle's computeFlux on a region (NumericalFluxCPU.cpp:170-221):
+    the inner region and the 2*d side regions of ConservedSystem.cpp:31-103,
+    applied one after the other to one output, give the whole system RHS bit
+    for bit (each cell exactly once: a cell written twice would be summed
+    twice, a missed one would keep the prior value), the wave speeds' max
+    over the regions is the system's, and a region alone leaves every cell
+    outside it at its prior value."""
+    eq, flux, recon, bc, n = setup_args
+    setup = O.make_setup(eq, flux, recon, O.INT_RK3, bc, n, (0, 0, 0), (1, 1, 1))
+    u = _smooth_state(setup, 3)
+    ref, ws_ref = O.system_rhs(setup, u)
+    out = np.zeros_like(u)
+    ws = np.zeros(3)
+    regs = O.conserved_system_regions(setup)
+    for start, end in regs:
+        out, w = O.compute_flux(setup, u, start, end, out=out)
+        ws = np.maximum(ws, w)
+    assert np.array_equal(ref.view(np.uint64), out.view(np.uint64))
+    assert np.array_equal(ws_ref, ws)
+    whole, w_whole = O.compute_flux(setup, u)
+    assert np.array_equal(ref.view(np.uint64), whole.view(np.uint64))
+    # one side region alone: the cells it covers hold the RHS, the rest the prior
+    prior = np.random.default_rng(4).standard_normal(u.shape)
+    start, end = regs[-1]
+    side, _ = O.compute_flux(setup, u, start, end, out=prior)
+    changed = side != prior
+    assert changed.any() and (~changed).any()
+    assert np.array_equal(side[changed], ref[changed])
+
+
+@pytest.mark.parametrize("recon", [O.RECON_WENO2, O.RECON_ENO2, O.RECON_NONE, O.RECON_ENO3])
+def test_volume_reconstruction_equals_line_reconstruction(recon):
+    """The oracle's performReconstruction on a 2D volume (each reconstruction's
+    own index ranges) gives, on every interior row and interior cell, the
+    values of the 1D line reconstruction that the known-answer tests above
+    pin, in both directions."""
+    setup = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, recon, O.INT_RK3, O.BC_PERIODIC, (14, 11, 1),
+                         (0, 0, 0), (1, 1, 0))
+    g = O.ghosts(setup)
+    u = _smooth_state(setup, 8)
+    for d in (0, 1):
+        left, right = O.reconstruct(setup, u, d)
+        lines = np.moveaxis(u, 3 - d, -1)[:, 0]  # (nvar, rows, cells) along d
+        ll = np.moveaxis(left, 3 - d, -1)[:, 0]
+        rr = np.moveaxis(right, 3 - d, -1)[:, 0]
+        for r in range(g, lines.shape[1] - g):
+            l1, r1 = O.reconstruct_1d(recon, np.ascontiguousarray(lines[:, r]))
+            sl = slice(g, lines.shape[2] - g)
+            assert np.array_equal(ll[:, r, sl], l1[:, sl]), (d, r)
+            assert np.array_equal(rr[:, r, sl], r1[:, sl]), (d, r)
+
+
 @pytest.mark.parametrize("recon", [O.RECON_NONE, O.RECON_WENO2, O.RECON_ENO2])
 def test_reconstruction_of_constant(recon):
     """A constant state is reproduced (to a few ulp: the WENO2 weights are
