@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--serving-tmin", type=float, default=0.1)
     ap.add_argument("--serving-tmax", type=float, default=1.0)
     ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving", "full-uq", "kh3d",
-                                       "stats"],
+                                       "stats", "single"],
                     default="all")
     ap.add_argument("--samples3d", type=int, default=64)
     ap.add_argument("--n3d", type=int, default=256)
@@ -94,9 +94,16 @@ def timed_exact(e, k):
     return time.perf_counter() - t0
 
 
+_HOST_CPU = []
+
+
 def host_cpu():
     """CPU model name and physical core count of this host (/proc/cpuinfo),
-    and the cores this process may run on."""
+    and the cores this process may run on, read on the first call (main()
+    calls it first: once OpenMP with OMP_PROC_BIND has run, the calling
+    thread is bound to one core and its affinity mask says so)."""
+    if _HOST_CPU:
+        return _HOST_CPU[0]
     model, cores = "unknown", set()
     phys = core = None
     try:
@@ -120,7 +127,8 @@ def host_cpu():
         allowed = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
         allowed = os.cpu_count() or 1
-    return model, len(cores) or (os.cpu_count() or 1), allowed
+    _HOST_CPU.append((model, len(cores) or (os.cpu_count() or 1), allowed))
+    return _HOST_CPU[0]
 
 
 def kh_scheme(hip, precision, eq=None):
@@ -170,11 +178,11 @@ def cpu_baseline(n, X0, steps):
     s = O.make_setup(O.EQ_EULER2, O.FLUX_HLL3, O.RECON_WENO2, O.INT_RK3, O.BC_PERIODIC,
                      (n, n, 1), (0, 0, 0), (1, 1, 0), cfl=0.475, end_time=2.0, max_steps=steps)
     u0 = ic.kelvin_helmholtz_2d(n, X0)
+    model, physical, allowed = host_cpu()
     t0 = time.perf_counter()
     rc, out, t, st, _ = O.run(s, u0)
     dt = time.perf_counter() - t0
     assert rc == 0 and st == steps
-    model, physical, allowed = host_cpu()
     threads = int(os.environ.get("OMP_NUM_THREADS", allowed))
     return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "port", "cpu_model": model, "host_physical_cores": physical,
@@ -220,10 +228,12 @@ def fp32_error(hip, M, n, X, steps):
     return out
 
 
-def _pair_error(hip, scheme_kw, samples, u0s, steps=None):
+def _pair_error(hip, scheme_kw, samples, u0s, steps=None, mixed=False):
     """Runs the same solves with FP64 and FP32 state; relative L1 of rho (var 0)
     of the FP32 result against the FP64 one (which is bit-identical to the
-    oracle), and of the ensemble mean/variance when there are several samples."""
+    oracle), and of the ensemble mean/variance when there are several samples
+    of one size (mixed: per grid size, the mean and largest per-sample error
+    instead)."""
     out = {}
     res = {}
     for prec in ("fp64", "fp32"):
@@ -237,11 +247,17 @@ def _pair_error(hip, scheme_kw, samples, u0s, steps=None):
         else:
             e.run(32)
         res[prec] = ([e.download(i) for i in range(len(samples))],
-                     e.meanvar() if len(samples) > 1 else None, e.info(0))
+                     e.meanvar() if len(samples) > 1 and not mixed else None, e.info(0))
         e.close()
     a, b = res["fp64"][0], res["fp32"][0]
-    out["rho_rel_l1"] = float(np.mean([np.abs(x[0] - y[0]).sum() / np.abs(x[0]).sum()
-                                       for x, y in zip(a, b)]))
+    errs = [np.abs(x[0] - y[0]).sum() / np.abs(x[0]).sum() for x, y in zip(a, b)]
+    out["rho_rel_l1"] = float(np.mean(errs))
+    if mixed:
+        by = {}
+        for x, err in zip(a, errs):
+            by.setdefault("%dx%d" % x.shape[-2:], []).append(err)
+        out["by_grid_with_ghosts"] = {k: {"samples": len(v), "mean": float(np.mean(v)),
+                                          "max": float(np.max(v))} for k, v in by.items()}
     if res["fp64"][1] is not None:
         (m64, v64), (m32, v32) = res["fp64"][1], res["fp32"][1]
         out["mean_rho_rel_l1"] = float(np.abs(m64[0] - m32[0]).sum() / np.abs(m64[0]).sum())
@@ -279,6 +295,11 @@ def fp32_errors(hip):
                integrator=hip.INT_RK2, boundary=hip.BC_PERIODIC, cfl=0.475)
     out["burgers_16384_T0.1"] = _pair_error(hip, kwb, [hip.make_sample((16384, 1, 1), (0, 0, 0), (1, 0, 0), 0.1)],
                                             [ic.burgers_fbm(16384, 0.5, 3)])
+    sizes = [128] * 171 + [256] * 171 + [512] * 170
+    Xm = sampling.stl_mersenne_uniform(len(sizes), 40)
+    out["mixed_171x128_171x256_170x512_40steps"] = _pair_error(
+        hip, kw2, [hip.make_sample((m, m, 1), (0, 0, 0), (1, 1, 0), 2.0) for m in sizes],
+        [ic.kelvin_helmholtz_2d(m, Xm[i]) for i, m in enumerate(sizes)], steps=40, mixed=True)
     kwr = dict(kw2, boundary=hip.BC_NEUMANN)
     out["riemann4_256_T0.2"] = _pair_error(hip, kwr, [hip.make_sample((256, 256, 1), (0, 0, 0), (1, 1, 0), 0.2)],
                                            [ic.riemann_four_quadrant(256, ic.riemann_parameters(X[1][:16]))])
@@ -316,6 +337,7 @@ def serving_bench(args):
 
 def main():
     args = parse()
+    host_cpu()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sample sharding across GPUs is a design note, not code). Run with --gpus 1.",
@@ -382,6 +404,21 @@ def main():
             log("  %s: %s ms" % (name, ", ".join("%.2f" % t for t in ts)))
         e.close()
         print(json.dumps(out), flush=True)
+        return
+
+    if args.only == "single":
+        # one KH solve of 1024^2 (the plain GPU path's large configuration)
+        # on its own, with the stage kernel's launch time from HIP events
+        X1 = sampling.stl_mersenne_uniform(1, 40)
+        es = make_batch(hip, [1024], args.precision, X1)
+        ssch = es.autotune()
+        sv, sms, sk = throughput(es, 1024 * 1024, args.steps, warm=args.warmup)
+        kms, launches = es.step_timed(10)
+        es.close()
+        print(json.dumps({"workload": "KH 2D 1024^2 single solve, %s" % args.precision,
+                          "value": sv, "unit": "cell-updates/s", "ms_per_step": sms, "steps": sk,
+                          "schedule": ssch, "stage_kernel_avg_ms": kms[0] / launches[0]}),
+              flush=True)
         return
 
     if args.only == "kh3d":
