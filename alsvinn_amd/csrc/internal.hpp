@@ -246,8 +246,8 @@ struct alsvinn_hip_ensemble {
     alsvinn_hip::WaveTask* d_tasks = nullptr;
     int ntasks = 0;
     int* d_count = nullptr;
-    unsigned int* d_end_counter = nullptr;
-    int ws_filter = 0;  // StageArgs::ws_filter for this work list  // workgroups done in the step's last kernel (0 between launches)
+    unsigned int* d_end_counter = nullptr;  // workgroups done in the step's last kernel (0 between launches)
+    int ws_filter = 0;                      // StageArgs::ws_filter for this work list
     int* h_count = nullptr;
     double* d_dt_hist = nullptr;
     long long dt_cap = 0;

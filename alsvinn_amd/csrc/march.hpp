@@ -105,6 +105,7 @@ struct StageArgs {
     // larger): set when a sample spans many wavefronts, whose atomics on the
     // sample's slots would otherwise serialise; with few wavefronts per
     // sample the plain atomic (no wait for the read) is cheaper
+    int ws_filter;
     const T* in;
     const T* in2;  // U0
     T* out;
@@ -131,6 +132,34 @@ struct PerVarVol {
     T* p[5];
     __device__ __forceinline__ T& operator()(int v, long long i) const { return p[v][i]; }
 };
+
+// A cell of a wavefront whose lanes all belong to one sample: its row's
+// byte offset, the same for every lane (scalar registers), and the lane's
+// column in bytes. With the variables' base addresses also wave-uniform, a
+// load or store is a scalar base plus a 32-bit lane offset, which the
+// hardware adds itself (no 64-bit address arithmetic per access).
+struct UniIdx {
+    unsigned row, lane;
+};
+template <class T>
+struct UniVol {
+    T* p[5];  // each variable's memory of the sample (wave-uniform)
+    __device__ __forceinline__ T& operator()(int v, UniIdx i) const {
+        using C = std::conditional_t<std::is_const_v<T>, const char, char>;
+        return *(T*)(((C*)p[v] + i.row) + i.lane);
+    }
+};
+// A struct read from a wave-uniform address into scalar registers.
+template <class S>
+__device__ __forceinline__ S uniform_load(const S* p) {
+    static_assert(sizeof(S) % 4 == 0, "whole words");
+    S r;
+    const int* src = (const int*)p;
+    int* dst = (int*)&r;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(S) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+    return r;
+}
 
 // Input, second input and output accessors of a stage kernel for sample D:
 // OP = 0 the ensemble's buffers, OP = 1 the volume operators' pointers.
@@ -306,9 +335,9 @@ __device__ __forceinline__ void load_strided(const T* __restrict__ in, const T* 
     }
 }
 
-template <class T, int NV, class Acc>
-__device__ __forceinline__ void load_cell(const Acc& in, const Acc& in2, bool lazy, T c,
-                                          long long i, T* q) {
+template <class T, int NV, class Acc, class I>
+__device__ __forceinline__ void load_cell(const Acc& in, const Acc& in2, bool lazy, T c, I i,
+                                          T* q) {
     if constexpr (std::is_same_v<Acc, StridedVol<const T>>) {
         load_strided<T, NV>(in.p, in2.p, lazy, c, i, in.vs, q);
     } else if (lazy) {
@@ -337,8 +366,8 @@ struct PendingCell {
             for (int v = 0; v < NV; ++v) b[v] = in2[i + v * vs];
         }
     }
-    template <class Acc>
-    __device__ __forceinline__ void issue(const Acc& in, const Acc& in2, bool lazy, long long i) {
+    template <class Acc, class I>
+    __device__ __forceinline__ void issue(const Acc& in, const Acc& in2, bool lazy, I i) {
         if constexpr (std::is_same_v<Acc, StridedVol<const T>>) {
             issue_strided(in.p, in2.p, lazy, i, in.vs);
         } else {
