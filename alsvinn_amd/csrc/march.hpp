@@ -134,19 +134,52 @@ struct PerVarVol {
 };
 
 // A cell of a wavefront whose lanes all belong to one sample: its row's
-// byte offset, the same for every lane (scalar registers), and the lane's
-// column in bytes. With the variables' base addresses also wave-uniform, a
-// load or store is a scalar base plus a 32-bit lane offset, which the
-// hardware adds itself (no 64-bit address arithmetic per access).
+// byte offset inside the variable's memory, the same for every lane (a
+// scalar register), and the lane's column in bytes, fixed for the task.
 struct UniIdx {
-    unsigned row, lane;
+    unsigned row;   // wave-uniform
+    unsigned lane;  // per lane
 };
+// The volumes of such a wave are read and written through buffer resources
+// (scalar base, size and format): an access is the resource, the scalar
+// byte offset of the variable and row, and the lane's column offset, which
+// the hardware adds itself, so loads and stores need no vector address
+// arithmetic. Every offset used stays below the resource's size.
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned lane, unsigned off) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, lane, off, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, lane, off, 0));
+}
+template <class T>
+__device__ __forceinline__ void buf_store(T x, __amdgpu_buffer_rsrc_t r, unsigned lane,
+                                          unsigned off) {
+    if constexpr (sizeof(T) == 8) {
+        using W = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(W, x), r, lane, off, 0);
+    } else {
+        using W = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0u, 0u, 0));
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(W, x), r, lane, off, 0);
+    }
+}
+template <class T>
+struct BufRef {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned lane, off;
+    __device__ __forceinline__ operator T() const { return buf_load<T>(r, lane, off); }
+    __device__ __forceinline__ void operator=(T x) const { buf_store<T>(x, r, lane, off); }
+};
+// Variable v of a volume: resource r[v] at byte offset vo[v] (one resource
+// for a whole sample of the ensemble, variables vstride elements apart; one
+// per memory for a reference Volume).
 template <class T>
 struct UniVol {
-    T* p[5];  // each variable's memory of the sample (wave-uniform)
-    __device__ __forceinline__ T& operator()(int v, UniIdx i) const {
-        using C = std::conditional_t<std::is_const_v<T>, const char, char>;
-        return *(T*)(((C*)p[v] + i.row) + i.lane);
+    __amdgpu_buffer_rsrc_t r[5];
+    unsigned vo[5];
+    __device__ __forceinline__ BufRef<std::remove_const_t<T>> operator()(int v, UniIdx i) const {
+        return {r[v], i.lane, vo[v] + i.row};
     }
 };
 // A struct read from a wave-uniform address into scalar registers.
@@ -507,20 +540,381 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < r ? x : r) + k;
 }
 
+// Reconstructions whose weights come from one quantity per face, shared by
+// the face's two cells (ReconWENO2::face_weight).
+template <class R, class = void>
+struct recon_face_weights : std::false_type {};
+template <class R>
+struct recon_face_weights<R, std::void_t<decltype(R::face_weights)>> : std::true_type {};
+
+// max(acc, s) for the running wave speed of a lane: s is a NaN (skipped, as
+// std::max(acc, s) skips it) or non-negative, acc starts at +0 and stays
+// finite, so the hardware maximum (IEEE maxNum: a quiet NaN operand yields
+// the other) is std::max here. One instruction; the compiler's fmax would
+// first canonicalise both operands.
+__device__ __forceinline__ double ws_max(double acc, double s) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(acc), "v"(s));
+    return r;
+}
+__device__ __forceinline__ float ws_max(float acc, float s) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(acc), "v"(s));
+    return r;
+}
+
+// A struct read through the scalar data cache (one s_load per few words,
+// the values land in scalar registers). Only for data that no wave of the
+// running kernel writes (work lists and sample descriptors are written by
+// the host or by earlier launches).
+template <class S>
+__device__ __forceinline__ S scalar_load(const S* p) {
+    static_assert(sizeof(S) % 4 == 0, "whole words");
+    using CW = const __attribute__((address_space(4))) int;
+    CW* src = (CW*)p;
+    S r;
+    int* dst = (int*)&r;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(S) / 4); ++k) dst[k] = src[k];
+    return r;
+}
+
+// Accessors of a wave whose lanes all belong to one sample (buffer
+// resources: OP = 0 one per volume, covering the sample's NV variables;
+// OP = 1 one per reference memory). The single-sample march runs only where
+// these fit 32-bit offsets (uni_fits).
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const T* p, unsigned long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(unsigned)bytes, kBufferWord3);
+}
+template <int OP, int NV, class Vol, class T>
+__device__ __forceinline__ Vol uni_vol(const T* base, const T* const* per_var, const SampleDesc& D) {
+    Vol r;
+    const unsigned long long vbytes = (unsigned long long)D.vstride * sizeof(T);
+    if constexpr (OP == 0) {
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(base + D.base, NV * vbytes);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            r.r[v] = rs;
+            r.vo[v] = unsigned(v * vbytes);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            r.r[v] = make_rsrc(per_var[v], vbytes);
+            r.vo[v] = 0;
+        }
+    }
+    return r;
+}
+template <int OP, int NV, class T>
+__device__ __forceinline__ UniVol<const T> uni_in(const StageArgs<T>& a, const SampleDesc& D,
+                                                  bool second) {
+    return uni_vol<OP, NV, UniVol<const T>>(second ? a.in2 : a.in, a.vin, D);
+}
+template <int OP, int NV, class T>
+__device__ __forceinline__ UniVol<T> uni_out(const StageArgs<T>& a, const SampleDesc& D) {
+    return uni_vol<OP, NV, UniVol<T>>((const T*)a.out, a.vout, D);
+}
+// A sample's volume fits the buffer resources' 32-bit offsets.
+template <class T, int NV>
+__device__ __forceinline__ bool uni_fits(const StageArgs<T>& a, int sample) {
+    const long long vs = scalar_load(&a.desc[sample].vstride);
+    return (unsigned long long)vs * sizeof(T) * NV < (1ull << 32);
+}
+
+// store_result for a single-sample wave: the row's byte offset and its ghost
+// images are scalar, the lane's column offset and its ghost columns' offsets
+// (negative when none) are fixed for the task.
+template <class T, int NV, int DIM, bool IMAGES>
+__device__ __forceinline__ void store_result_uni(const UniVol<T>& out, const SampleDesc& D,
+                                                 unsigned lo, int xbo, int xdo, int row,
+                                                 const T* res) {
+    auto put_row = [&](int py) {
+        const unsigned rb = unsigned(py) * unsigned(D.tx) * unsigned(sizeof(T));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out(v, UniIdx{rb, lo}) = res[v];
+        if constexpr (IMAGES) {
+            if (xbo >= 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out(v, UniIdx{rb, unsigned(xbo)}) = res[v];
+            }
+            if (xdo >= 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out(v, UniIdx{rb, unsigned(xdo)}) = res[v];
+            }
+        }
+    };
+    int ya = DIM >= 2 ? row + D.gy : 0, yb = -1, yd = -1;
+    if constexpr (IMAGES && DIM >= 2) axis_images(row, D.ny, D.gy, D.bc, ya, yb, yd);
+    put_row(ya);
+    if constexpr (IMAGES && DIM >= 2) {
+        if (yb >= 0) put_row(yb);
+        if (yd >= 0) put_row(yd);
+    }
+}
+
+// One wave task of the 1D/2D stage kernel: the strip [x0, x0 + ncols) of
+// `sample` occupying lanes [lane0, lane0 + ncols + 2G), marched over rows
+// [y0, y1). UNI: every lane of the wave belongs to this segment's sample,
+// so sample, x0, ncols and lane0 are wave-uniform and the volumes are
+// addressed from scalar bases; otherwise each lane passes its own segment.
+template <class T, class Eq, class Recon, class Flux, int DIM, int MODE, int OP, bool UNI>
+__device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, int sample, int x0,
+                                         int ncols, int lane0, bool in_strip, int seg, int y0,
+                                         int y1, bool& flagged) {
+    constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
+    constexpr int NV = Eq::nvar;
+    constexpr int G = Recon::ghosts;
+    constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
+    constexpr bool PRIM = recon_prim<Recon>::value;
+    constexpr bool FW = recon_face_weights<Recon>::value && !PRIM;
+    const int lane = threadIdx.x & 63;
+    const SampleDesc D = [&] {
+        if constexpr (UNI)
+            return scalar_load(a.desc + sample);
+        else
+            return a.desc[sample];
+    }();
+    SampleState* Sp = a.state + sample;
+    const bool active = in_strip && runnable(*Sp);
+    if (!__any(active)) return;  // wave-uniform
+    const bool need_dt = LAZY || MODE != MODE_RHS;
+    T c = T(0);
+    if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
+
+    const int li = lane - lane0;
+    const int col = x0 - G + li;  // interior x coordinate of this lane
+    const bool interior = active && li >= G && li < G + ncols;
+    const bool xface = active && li >= G - 1 && li < G + ncols;  // face col+1/2
+    const int px = in_strip ? col + D.gx : D.gx;                 // padded column
+    const ColumnImages cimg = column_images(D, col);
+    // byte offsets of the lane's column and of its ghost images (UNI)
+    const unsigned lo = unsigned(px) * unsigned(sizeof(T));
+    // XLOAD: the x-neighbours of a row are loaded (UNI, stage input read
+    // as stored) instead of moved across lanes: two loads per variable, no
+    // VALU, issued one row ahead; the lanes at the ends of the padded row
+    // read their own cell (they are halo lanes, whose neighbours are unused)
+    constexpr bool XLOAD = UNI && !LAZY && !PRIM && G >= 2;
+    const unsigned lom = unsigned(px > 0 ? px - 1 : px) * unsigned(sizeof(T));
+    const unsigned lop = unsigned(px < D.tx - 1 ? px + 1 : px) * unsigned(sizeof(T));
+    const int xbo = cimg.xb >= 0 ? cimg.xb * int(sizeof(T)) : -1;
+    const int xdo = cimg.xd >= 0 ? cimg.xd * int(sizeof(T)) : -1;
+    auto make_in = [&](bool second) {
+        if constexpr (UNI)
+            return uni_in<OP, NV>(a, D, second);
+        else
+            return stage_in<OP>(a, D, second);
+    };
+    const auto in = make_in(false);
+    const auto in2 = make_in(true);
+    const auto out = [&] {
+        if constexpr (UNI)
+            return uni_out<OP, NV>(a, D);
+        else
+            return stage_out<OP>(a, D);
+    }();
+
+    T wsx = T(0), wsy = T(0);
+
+    // Rows are addressed in padded coordinates; loads outside the volume
+    // are clamped (they only occur for G == 1, whose neighbours are unused).
+    auto row_at = [&](int py) {
+        if constexpr (UNI)
+            return UniIdx{unsigned(py) * unsigned(D.tx) * unsigned(sizeof(T)), lo};
+        else
+            return (long long)py * D.tx + px;
+    };
+    auto row_index = [&](int py) {
+        py = py < 0 ? 0 : (py >= D.ty ? D.ty - 1 : py);
+        return row_at(py);
+    };
+
+    T yc[NV], yp[NV];  // rows r, r+1 of the stage input
+    T xmn[NV], xpn[NV];  // XLOAD: x-neighbours of the next row to sweep
+    auto load_xn = [&](int py) {
+        if constexpr (XLOAD) {
+            const UniIdx im{unsigned(py) * unsigned(D.tx) * unsigned(sizeof(T)), lom};
+            const UniIdx ip{im.row, lop};
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xmn[v] = in(v, im);
+                xpn[v] = in(v, ip);
+            }
+        }
+    };
+    T Ry[NV];          // right (r+1/2) reconstruction of row r in y
+    T Gm[NV];          // y face flux at r-1/2
+    T wy = T(0);       // FW: the weight of face r+1/2 in y
+    // reconstructions from primitive variables: those of rows r, r+1,
+    // computed once per row and shared by the x and y stencils
+    T Pc[NV], Pp[NV];
+    const int gy = D.gy;
+    if constexpr (DIM >= 2) {
+        const int pr = y0 + gy;
+        T qa[NV], ym[NV];
+        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 2), qa);
+        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), ym);
+        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), yc);
+        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), yp);
+        T Lt[NV], Rt[NV], L0[NV], R0[NV];
+        if constexpr (PRIM) {
+            T Pa[NV], Pm[NV];
+            Recon::template prim<T, NV>(qa, Pa, eq.gamma);
+            Recon::template prim<T, NV>(ym, Pm, eq.gamma);
+            Recon::template prim<T, NV>(yc, Pc, eq.gamma);
+            Recon::template prim<T, NV>(yp, Pp, eq.gamma);
+            Recon::template apply_prim<T, NV>(Pa, Pm, Pc, Lt, Rt, eq.gamma);  // row r-1
+            Recon::template apply_prim<T, NV>(Pm, Pc, Pp, L0, R0, eq.gamma);  // row r
+        } else if constexpr (FW) {
+            const T wa = Recon::face_weight(qa[0], ym[0]);  // face r-3/2
+            const T wm = Recon::face_weight(ym[0], yc[0]);  // face r-1/2
+            wy = Recon::face_weight(yc[0], yp[0]);          // face r+1/2
+            Recon::template apply_w<T, NV>(qa, ym, yc, wm, wa, Lt, Rt);  // row r-1: keep R
+            Recon::template apply_w<T, NV>(ym, yc, yp, wy, wm, L0, R0);  // row r
+        } else {
+            Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt, eq.gamma);  // row r-1: keep R
+            Recon::template apply<T, NV>(ym, yc, yp, L0, R0, eq.gamma);  // row r
+        }
+        wsy = ws_max(wsy, face_flux<T, Eq, Flux, 1>(eq, Rt, L0, Gm));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Ry[v] = R0[v];
+    } else {
+        load_cell<T, NV>(in, in2, LAZY, c, row_index(0), yc);
+        if constexpr (PRIM) Recon::template prim<T, NV>(yc, Pc, eq.gamma);
+    }
+
+    load_xn(y0 + gy);
+    for (int r = y0; r < y1; ++r) {
+        const int pr = r + gy;
+        T qn[NV], Pn[NV];
+        // next row first: its latency overlaps the x sweep
+        PendingCell<T, NV> pn;
+        if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2));
+        // U^n of this row for the stage combination, also issued before
+        // the sweep (loaded after it, its latency would stall the wave);
+        // every lane's column lies inside the padded row
+        T u0[NV];
+        if constexpr (MODE != MODE_RHS) {
+            const auto i0 = row_at(pr);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
+        }
+        T out_v[NV];
+        // ---- x direction on row r (first: out = 0 + x divergence)
+        {
+            T xm[NV], xp[NV];
+            const T* xc = PRIM ? Pc : yc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xm[v] = XLOAD ? xmn[v] : lane_from_lower(xc[v]);
+                xp[v] = XLOAD ? xpn[v] : lane_from_upper(xc[v]);
+            }
+            T Lx[NV], Rx[NV];
+            if constexpr (PRIM) {
+                Recon::template apply_prim<T, NV>(xm, Pc, xp, Lx, Rx, eq.gamma);
+            } else if constexpr (FW) {
+                // face col+1/2 of this lane; face col-1/2 is the lower lane's
+                const T wf = Recon::face_weight(yc[0], xp[0]);
+                Recon::template apply_w<T, NV>(xm, yc, xp, wf, lane_from_lower(wf), Lx, Rx);
+            } else {
+                Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx, eq.gamma);
+            }
+            // the next row's x-neighbours, in flight during the x flux and
+            // the y sweep (after the last row this reads row y1 + gy <= ty - 1,
+            // a ghost row). Issued only once this row's are consumed: the
+            // loads then write the registers that held them, no copies.
+            if constexpr (XLOAD && DIM >= 2) {
+                __builtin_amdgcn_sched_barrier(0);
+                load_xn(pr + 1);
+            }
+            T Ln[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
+            T Fp[NV];
+            wsx = ws_max(wsx, face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const T Fm = lane_from_lower(Fp[v]);
+                const T t0 = (T(-1.0) * Fp[v]) + Fm;
+                out_v[v] = T(0) + t0;
+            }
+        }
+        // ---- y direction: out += (-G(r+1/2)) + G(r-1/2)
+        T wn = T(0);
+        if constexpr (DIM >= 2) {
+            pn.finish(LAZY, c, qn);
+            T Ln[NV], Rn[NV];
+            if constexpr (PRIM) {
+                Recon::template prim<T, NV>(qn, Pn, eq.gamma);
+                Recon::template apply_prim<T, NV>(Pc, Pp, Pn, Ln, Rn, eq.gamma);  // row r+1
+            } else if constexpr (FW) {
+                wn = Recon::face_weight(yp[0], qn[0]);  // face r+3/2
+                Recon::template apply_w<T, NV>(yc, yp, qn, wn, wy, Ln, Rn);  // row r+1
+            } else {
+                Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn, eq.gamma);  // row r+1
+            }
+            T Gp[NV];
+            wsy = ws_max(wsy, face_flux<T, Eq, Flux, 1>(eq, Ry, Ln, Gp));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+                Gm[v] = Gp[v];
+                Ry[v] = Rn[v];
+            }
+        }
+        if (interior) {
+            T res[NV];
+            combine<T, NV>(MODE, c, out_v, yc, u0, res);
+            if (CHECK && !eq.obeys(res)) {
+                atomicOr(&Sp->status, 2);
+                flagged = true;
+            }
+            if constexpr (UNI)
+                store_result_uni<T, NV, DIM, OP == 0>(out, D, lo, xbo, xdo, r, res);
+            else
+                store_result<T, NV, DIM, OP == 0>(out, D, cimg, r, 0, res);
+        }
+        if constexpr (DIM >= 2) {
+            wy = wn;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                yc[v] = yp[v];
+                yp[v] = qn[v];
+                if constexpr (PRIM) {
+                    Pc[v] = Pp[v];
+                    Pp[v] = Pn[v];
+                }
+            }
+        }
+    }
+
+    if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
+        // the lanes' running maxima cover every row of their faces: only
+        // the faces of the strip count (x: col+1/2 for the lanes from G-1,
+        // y: the interior columns)
+        const T mx = segment_max(xface ? wsx : T(0), seg);
+        const T my = segment_max(interior ? wsy : T(0), seg);
+        if (in_strip && li == 0 && runnable(*Sp)) {
+            raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
+            if (DIM >= 2) raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
+        }
+    }
+}
+
 // 1D and 2D stage kernel (DIM = active grid dimensions). WPS is the
 // register budget expressed as waves per SIMD (2: no spills for FP64 Euler
 // 2D; 3 and 4 trade spills for occupancy); the autotuner picks one. OP = 1
 // is the volume operator (computeFlux on a reference Volume's per-variable
 // memories over a region; the result cells only, no ghost images).
+// A task whose wave holds one segment (every full-width strip) runs the
+// single-sample march with scalar addressing; packed narrow strips of
+// several samples run the per-lane one.
 template <class T, class Eq, class Recon, class Flux, int DIM, int WPS, int MODE, int OP = 0>
 __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
-    constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
-    constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     const int lane = threadIdx.x & 63;
-    // wave-uniform: the task's fields, the row loop and the row's ghost
-    // images are then scalar
     const int wave = __builtin_amdgcn_readfirstlane((xcd_block() * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     Eq eq;
@@ -529,171 +923,30 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
     const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
     bool flagged = false;  // this thread set a sample's failure status
     for (int w = wave; w < ntasks; w += nwaves) {
-        const WaveTask& tk = a.tasks[w];
-        // this lane's segment
-        int seg = -1;
-        for (int k = 0; k < tk.nseg; ++k)
-            if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
-        const bool in_strip = seg >= 0;
-        const int sk = in_strip ? seg : 0;
-        const int sample = tk.sample[sk];
-        const int li = lane - tk.lane0[sk];
-        const int ncols = tk.ncols[sk];
-        const SampleDesc D = a.desc[sample];
-        SampleState* Sp = a.state + sample;
-        const bool active = in_strip && runnable(*Sp);
-        if (!__any(active)) continue;  // wave-uniform
-        const bool need_dt = LAZY || MODE != MODE_RHS;
-        T c = T(0);
-        if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
-
-        const int col = tk.x0[sk] - G + li;  // interior x coordinate of this lane
-        const bool interior = active && li >= G && li < G + ncols;
-        const bool xface = active && li >= G - 1 && li < G + ncols;  // face col+1/2
-        const int px = in_strip ? col + D.gx : D.gx;                 // padded column
-        const ColumnImages cimg = column_images(D, col);
-        const auto in = stage_in<OP>(a, D, false);
-        const auto in2 = stage_in<OP>(a, D, true);
-        const auto out = stage_out<OP>(a, D);
-
-        T wsx = T(0), wsy = T(0);
-
-        // Rows are addressed in padded coordinates; loads outside the volume
-        // are clamped (they only occur for G == 1, whose neighbours are unused).
-        auto row_index = [&](int py) -> long long {
-            py = py < 0 ? 0 : (py >= D.ty ? D.ty - 1 : py);
-            return (long long)py * D.tx + px;
-        };
-
-        T yc[NV], yp[NV];  // rows r, r+1 of the stage input
-        T Ry[NV];          // right (r+1/2) reconstruction of row r in y
-        T Gm[NV];          // y face flux at r-1/2
-        // reconstructions from primitive variables: those of rows r, r+1,
-        // computed once per row and shared by the x and y stencils
-        constexpr bool PRIM = recon_prim<Recon>::value;
-        T Pc[NV], Pp[NV];
-        const int gy = D.gy;
-        if constexpr (DIM >= 2) {
-            const int pr = tk.y0 + gy;
-            T qa[NV], ym[NV];
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 2), qa);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), ym);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), yc);
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), yp);
-            T Lt[NV], Rt[NV], L0[NV], R0[NV];
-            if constexpr (PRIM) {
-                T Pa[NV], Pm[NV];
-                Recon::template prim<T, NV>(qa, Pa, eq.gamma);
-                Recon::template prim<T, NV>(ym, Pm, eq.gamma);
-                Recon::template prim<T, NV>(yc, Pc, eq.gamma);
-                Recon::template prim<T, NV>(yp, Pp, eq.gamma);
-                Recon::template apply_prim<T, NV>(Pa, Pm, Pc, Lt, Rt, eq.gamma);  // row r-1
-                Recon::template apply_prim<T, NV>(Pm, Pc, Pp, L0, R0, eq.gamma);  // row r
-            } else {
-                Recon::template apply<T, NV>(qa, ym, yc, Lt, Rt, eq.gamma);  // row r-1: keep R
-                Recon::template apply<T, NV>(ym, yc, yp, L0, R0, eq.gamma);  // row r
-            }
-            const T s = face_flux<T, Eq, Flux, 1>(eq, Rt, L0, Gm);
-            if (interior) wsy = std_max(wsy, s);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) Ry[v] = R0[v];
+        const WaveTask tk = scalar_load(a.tasks + w);
+        if (tk.nseg == 1 && uni_fits<T, Eq::nvar>(a, tk.sample[0])) {
+            const bool in_strip = lane >= tk.lane0[0] && lane < tk.lane0[0] + tk.ncols[0] + 2 * G;
+            march_12<T, Eq, Recon, Flux, DIM, MODE, OP, true>(
+                a, eq, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
+                in_strip ? 0 : -1, tk.y0, tk.y1, flagged);
         } else {
-            load_cell<T, NV>(in, in2, LAZY, c, row_index(0), yc);
-            if constexpr (PRIM) Recon::template prim<T, NV>(yc, Pc, eq.gamma);
-        }
-
-        for (int r = tk.y0; r < tk.y1; ++r) {
-            const int pr = r + gy;
-            T qn[NV], Pn[NV];
-            // next row first: its latency overlaps the x sweep
-            PendingCell<T, NV> pn;
-            if constexpr (DIM >= 2) pn.issue(in, in2, LAZY, row_index(pr + 2));
-            // U^n of this row for the stage combination, also issued before
-            // the sweep (loaded after it, its latency would stall the wave)
-            const long long i0 = (long long)pr * D.tx + px;
-            T u0[NV];
-            if (MODE != MODE_RHS && interior) {
+            // this lane's segment
+            int seg = -1, sample = tk.sample[0], x0 = tk.x0[0], ncols = tk.ncols[0],
+                lane0 = tk.lane0[0];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
-            }
-            T out_v[NV];
-            // ---- x direction on row r (first: out = 0 + x divergence)
-            {
-                T xm[NV], xp[NV];
-                const T* xc = PRIM ? Pc : yc;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    xm[v] = lane_from_lower(xc[v]);
-                    xp[v] = lane_from_upper(xc[v]);
-                }
-                T Lx[NV], Rx[NV];
-                if constexpr (PRIM)
-                    Recon::template apply_prim<T, NV>(xm, Pc, xp, Lx, Rx, eq.gamma);
-                else
-                    Recon::template apply<T, NV>(xm, yc, xp, Lx, Rx, eq.gamma);
-                T Ln[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
-                T Fp[NV];
-                const T s = face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp);
-                if (xface) wsx = std_max(wsx, s);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    const T Fm = lane_from_lower(Fp[v]);
-                    const T t0 = (T(-1.0) * Fp[v]) + Fm;
-                    out_v[v] = T(0) + t0;
+            for (int k = 0; k < kMaxSeg; ++k) {
+                if (k < tk.nseg && lane >= tk.lane0[k] &&
+                    lane < tk.lane0[k] + tk.ncols[k] + 2 * G) {
+                    seg = k;
+                    sample = tk.sample[k];
+                    x0 = tk.x0[k];
+                    ncols = tk.ncols[k];
+                    lane0 = tk.lane0[k];
                 }
             }
-            // ---- y direction: out += (-G(r+1/2)) + G(r-1/2)
-            if constexpr (DIM >= 2) {
-                pn.finish(LAZY, c, qn);
-                T Ln[NV], Rn[NV];
-                if constexpr (PRIM) {
-                    Recon::template prim<T, NV>(qn, Pn, eq.gamma);
-                    Recon::template apply_prim<T, NV>(Pc, Pp, Pn, Ln, Rn, eq.gamma);  // row r+1
-                } else {
-                    Recon::template apply<T, NV>(yc, yp, qn, Ln, Rn, eq.gamma);  // row r+1
-                }
-                T Gp[NV];
-                const T s = face_flux<T, Eq, Flux, 1>(eq, Ry, Ln, Gp);
-                if (interior) wsy = std_max(wsy, s);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
-                    Gm[v] = Gp[v];
-                    Ry[v] = Rn[v];
-                }
-            }
-            if (interior) {
-                T res[NV];
-                combine<T, NV>(MODE, c, out_v, yc, u0, res);
-                if (CHECK && !eq.obeys(res)) {
-                    atomicOr(&Sp->status, 2);
-                    flagged = true;
-                }
-                store_result<T, NV, DIM, OP == 0>(out, D, cimg, r, 0, res);
-            }
-            if constexpr (DIM >= 2) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    yc[v] = yp[v];
-                    yp[v] = qn[v];
-                    if constexpr (PRIM) {
-                        Pc[v] = Pp[v];
-                        Pp[v] = Pn[v];
-                    }
-                }
-            }
-        }
-
-        if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
-            const T mx = segment_max(wsx, seg);
-            const T my = segment_max(wsy, seg);
-            if (in_strip && li == 0 && runnable(*Sp)) {
-                raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
-                if (DIM >= 2)
-                    raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
-            }
+            march_12<T, Eq, Recon, Flux, DIM, MODE, OP, false>(a, eq, sample, x0, ncols, lane0,
+                                                               seg >= 0, seg, tk.y0, tk.y1,
+                                                               flagged);
         }
     }
     if constexpr (CHECK) end_step(a.end, flagged);

@@ -412,25 +412,30 @@ struct ReconNone {
 
 struct ReconWENO2 {
     static constexpr int ghosts = 2;
-    template <class T, int NV>
-    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
-                                                 T* R, T /*gamma*/) {
-        // eps = 1e-8 (WENOCoefficients.hpp:21); the FP64 path evaluates the
-        // reference's expressions exactly.
+    static constexpr bool face_weights = true;
+    // 1/D of the face between two neighbouring cells whose first variables
+    // are lo (lower cell) and hi (upper cell): D = 3(eps + (hi - lo)^2)^2,
+    // eps = 1e-8 (WENOCoefficients.hpp:21). It is a0L of the lower cell
+    // (its d0 = q[i+1] - q[i]) and a1R of the upper cell (its
+    // d1 = q[i+1] - q[i]): the same expression of the same two values, so a
+    // marching kernel computes it once per face and hands it to both cells.
+    template <class T>
+    __device__ __forceinline__ static T face_weight(T lo, T hi) {
         const T eps = T(1e-8);
-        const T i0 = q0[0];
-        const T d0 = qp[0] - i0;
-        const T d1 = i0 - qm[0];
-        const T b0 = d0 * d0;
-        const T b1 = d1 * d1;
-        const T e0 = eps + b0;
-        const T e1 = eps + b1;
-        const T D0 = T(3) * e0 * e0;
-        const T D1 = T(3) * e1 * e1;
+        const T d = hi - lo;
+        const T b = d * d;
+        const T e = eps + b;
+        const T D = T(3) * e * e;
+        return dvp(T(1), D);
+    }
+    // The reconstruction from the two face quantities of the cell:
+    // a0L = face_weight(q0, qp) (face i+1/2), a1R = face_weight(qm, q0)
+    // (face i-1/2).
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply_w(const T* qm, const T* q0, const T* qp, T a0L,
+                                                   T a1R, T* L, T* R) {
         // 2/D == 2*(1/D) exactly while 1/D is a normal number (D < 2^1021,
         // i.e. |w_{i+1}-w_i| < 2^510), so two of the four divisions are shared.
-        const T a0L = dvp(T(1), D0);
-        const T a1R = dvp(T(1), D1);
         const T a1L = T(2) * a1R;
         const T a0R = T(2) * a0L;
         const T w0L = dvp(a0L, a0L + a1L);
@@ -444,6 +449,11 @@ struct ReconWENO2 {
             L[v] = T(0.5) * (w1L * qm[v] + cL * q0[v] - w0L * qp[v]);
             R[v] = T(0.5) * (w0R * qp[v] + cR * q0[v] - w1R * qm[v]);
         }
+    }
+    template <class T, int NV>
+    __device__ __forceinline__ static void apply(const T* qm, const T* q0, const T* qp, T* L,
+                                                 T* R, T /*gamma*/) {
+        apply_w<T, NV>(qm, q0, qp, face_weight(q0[0], qp[0]), face_weight(qm[0], q0[0]), L, R);
     }
 };
 
