@@ -8,7 +8,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libalsvinn_hip.so")
+# the in-tree build; ALSVINN_HIP_LIBRARY names another build of the same sources
+LIB_PATH = os.environ.get("ALSVINN_HIP_LIBRARY") or os.path.join(HERE, "libalsvinn_hip.so")
 
 EQ_BURGERS, EQ_EULER1, EQ_EULER2, EQ_EULER3 = 0, 1, 2, 3
 FLUX_HLL, FLUX_HLL3, FLUX_GODUNOV, FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE = 0, 1, 2, 3, 4, 5
