@@ -623,15 +623,16 @@ __device__ __forceinline__ bool uni_fits(const StageArgs<T>& a, int sample) {
     return (unsigned long long)vs * sizeof(T) * NV < (1ull << 32);
 }
 
-// store_result for a single-sample wave: the row's byte offset and its ghost
-// images are scalar, the lane's column offset and its ghost columns' offsets
-// (negative when none) are fixed for the task.
+// store_result for a single-sample wave: the row's (and plane's) byte
+// offset and their ghost images are scalar, the lane's column offset and its
+// ghost columns' offsets (negative when none) are fixed for the task.
 template <class T, int NV, int DIM, bool IMAGES>
 __device__ __forceinline__ void store_result_uni(const UniVol<T>& out, const SampleDesc& D,
-                                                 unsigned lo, int xbo, int xdo, int row,
+                                                 unsigned lo, int xbo, int xdo, int row, int plane,
                                                  const T* res) {
-    auto put_row = [&](int py) {
-        const unsigned rb = unsigned(py) * unsigned(D.tx) * unsigned(sizeof(T));
+    auto put_row = [&](int py, int pz) {
+        const unsigned rb =
+            (unsigned(pz) * unsigned(D.ty) + unsigned(py)) * unsigned(D.tx) * unsigned(sizeof(T));
 #pragma unroll
         for (int v = 0; v < NV; ++v) out(v, UniIdx{rb, lo}) = res[v];
         if constexpr (IMAGES) {
@@ -646,11 +647,20 @@ __device__ __forceinline__ void store_result_uni(const UniVol<T>& out, const Sam
         }
     };
     int ya = DIM >= 2 ? row + D.gy : 0, yb = -1, yd = -1;
+    int za = DIM >= 3 ? plane + D.gz : 0, zb = -1, zd = -1;
     if constexpr (IMAGES && DIM >= 2) axis_images(row, D.ny, D.gy, D.bc, ya, yb, yd);
-    put_row(ya);
-    if constexpr (IMAGES && DIM >= 2) {
-        if (yb >= 0) put_row(yb);
-        if (yd >= 0) put_row(yd);
+    if constexpr (IMAGES && DIM >= 3) axis_images(plane, D.nz, D.gz, D.bc, za, zb, zd);
+    auto put_plane = [&](int pz) {
+        put_row(ya, pz);
+        if constexpr (IMAGES && DIM >= 2) {
+            if (yb >= 0) put_row(yb, pz);
+            if (yd >= 0) put_row(yd, pz);
+        }
+    };
+    put_plane(za);
+    if constexpr (IMAGES && DIM >= 3) {
+        if (zb >= 0) put_plane(zb);
+        if (zd >= 0) put_plane(zd);
     }
 }
 
@@ -871,7 +881,7 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
                 flagged = true;
             }
             if constexpr (UNI)
-                store_result_uni<T, NV, DIM, OP == 0>(out, D, lo, xbo, xdo, r, res);
+                store_result_uni<T, NV, DIM, OP == 0>(out, D, lo, xbo, xdo, r, 0, res);
             else
                 store_result<T, NV, DIM, OP == 0>(out, D, cimg, r, 0, res);
         }
@@ -964,8 +974,16 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
 // instead of per row.
 constexpr int kGroupRows = 4;
 
-template <class T, class Eq, class Recon, class Flux, int WPS, int MODE, int OP = 0>
-__global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs<T> a) {
+// One task of the 3D stage kernel, run by the whole workgroup: wave w
+// marches row y0 + w of the strip [x0, x0 + ncols) of `sample` (lanes
+// [lane0, lane0 + ncols + 2G)) over planes [z0, z1). UNI as in march_12:
+// the workgroup's lanes all belong to one sample, addressed from scalar
+// bases through buffer resources. s_rows, s_L, s_G: the group's LDS rows.
+template <class T, class Eq, class Recon, class Flux, int MODE, int OP, bool UNI, int NV = Eq::nvar>
+__device__ __forceinline__ void march_3(const StageArgs<T>& a, const Eq& eq, int sample, int x0,
+                                        int ncols, int lane0, bool in_strip, int seg, int y0,
+                                        int y1, int z0, int z1, T (*s_rows)[NV][64],
+                                        T (*s_L)[NV][64], T (*s_G)[NV][64], bool& flagged) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
     constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     // FP32 lazy stage: U^n of the two planes ahead of the combination stays
@@ -973,6 +991,334 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     // instead of being loaded again for the combination (FP64 keeps the
     // reload: twice the registers)
     constexpr bool KEEP_U0 = LAZY && std::is_same_v<T, float>;
+    constexpr int G = Recon::ghosts;
+    constexpr int NW = kGroupRows;
+    constexpr bool FW = recon_face_weights<Recon>::value;
+    // the x-neighbours of the next plane loaded (single-sample groups, stage
+    // input read as stored), one plane ahead, instead of moved across lanes
+    constexpr bool XLOAD = UNI && !LAZY && G >= 2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const SampleDesc D = [&] {
+        if constexpr (UNI)
+            return scalar_load(a.desc + sample);
+        else
+            return a.desc[sample];
+    }();
+    SampleState* Sp = a.state + sample;
+    const bool active = in_strip && runnable(*Sp);
+    // the samples of a task are the same for every wavefront of the group
+    if (!__syncthreads_or(active)) return;
+    const bool need_dt = LAZY || MODE != MODE_RHS;
+    T c = T(0);
+    if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
+
+    const int li = lane - lane0;
+    const int row = y0 + w;            // this wavefront's row
+    const bool row_in = row < y1;      // rows past the group end only feed neighbours
+    const int col = x0 - G + li;
+    const bool lanes_in = active && li >= G && li < G + ncols;
+    const bool interior = lanes_in && row_in;
+    const bool xface = active && row_in && li >= G - 1 && li < G + ncols;
+    const int px = in_strip ? col + D.gx : D.gx;
+    const int py = row + D.gy;
+    const ColumnImages cimg = column_images(D, col);
+    const unsigned lo = unsigned(px) * unsigned(sizeof(T));
+    const unsigned lom = unsigned(px > 0 ? px - 1 : px) * unsigned(sizeof(T));
+    const unsigned lop = unsigned(px < D.tx - 1 ? px + 1 : px) * unsigned(sizeof(T));
+    const int xbo = cimg.xb >= 0 ? cimg.xb * int(sizeof(T)) : -1;
+    const int xdo = cimg.xd >= 0 ? cimg.xd * int(sizeof(T)) : -1;
+    const long long plane_stride = (long long)D.tx * D.ty;
+    auto make_in = [&](bool second) {
+        if constexpr (UNI)
+            return uni_in<OP, NV>(a, D, second);
+        else
+            return stage_in<OP>(a, D, second);
+    };
+    const auto in = make_in(false);
+    const auto in2 = make_in(true);
+    const auto out = [&] {
+        if constexpr (UNI)
+            return uni_out<OP, NV>(a, D);
+        else
+            return stage_out<OP>(a, D);
+    }();
+
+    T wsx = T(0), wsy = T(0), wsz = T(0);
+    // cell (padded row pyy, padded plane pz) of the lane's column, clamped
+    // to the padded volume (rows past the group end and the planes of the
+    // prologue's far halo)
+    auto idx = [&](int pyy, int pz) {
+        pz = pz < 0 ? 0 : (pz >= D.tz ? D.tz - 1 : pz);
+        pyy = pyy < 0 ? 0 : (pyy >= D.ty ? D.ty - 1 : pyy);
+        if constexpr (UNI)
+            return UniIdx{(unsigned(pz) * unsigned(D.ty) + unsigned(pyy)) * unsigned(D.tx) *
+                              unsigned(sizeof(T)),
+                          lo};
+        else
+            return (long long)pz * plane_stride + (long long)pyy * D.tx + px;
+    };
+    // halo rows: the first wavefront loads y0-2, y0-1, the last y0+NW, y0+NW+1
+    const bool lo_halo = w == 0;
+    const bool hi_halo = w == NW - 1;
+    const int h0 = lo_halo ? y0 - 2 + D.gy : y0 + NW + D.gy;
+
+    T zc[NV], zp[NV], Rz[NV], Hm[NV];
+    T u_zc[NV], u_zp[NV];  // KEEP_U0: U^n at planes zc, zp
+    T xmn[NV], xpn[NV];    // XLOAD: x-neighbours of the next plane's row
+    T wz = T(0);           // FW: the weight of the z face above plane zc
+    auto load_xn = [&](int pz) {
+        if constexpr (XLOAD) {
+            const UniIdx c0 = idx(py, pz);
+            const UniIdx im{c0.row, lom}, ip{c0.row, lop};
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xmn[v] = in(v, im);
+                xpn[v] = in(v, ip);
+            }
+        }
+    };
+    {
+        const int pz = z0 + D.gz;
+        T qa[NV], zm[NV];
+        load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 2), qa);
+        load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 1), zm);
+        if constexpr (KEEP_U0) {
+            PendingCell<T, NV> p0, p1;
+            p0.issue(in, in2, LAZY, idx(py, pz));
+            p1.issue(in, in2, LAZY, idx(py, pz + 1));
+            p0.finish(LAZY, c, zc);
+            p1.finish(LAZY, c, zp);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                u_zc[v] = p0.b[v];
+                u_zp[v] = p1.b[v];
+            }
+        } else {
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), zc);
+            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), zp);
+        }
+        load_xn(pz);
+        T Lt[NV], Rt[NV], L0[NV], R0[NV];
+        if constexpr (FW) {
+            const T wa = Recon::face_weight(qa[0], zm[0]);  // face pz-3/2
+            const T wm = Recon::face_weight(zm[0], zc[0]);  // face pz-1/2
+            wz = Recon::face_weight(zc[0], zp[0]);          // face pz+1/2
+            Recon::template apply_w<T, NV>(qa, zm, zc, wm, wa, Lt, Rt);
+            Recon::template apply_w<T, NV>(zm, zc, zp, wz, wm, L0, R0);
+        } else {
+            Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt, eq.gamma);
+            Recon::template apply<T, NV>(zm, zc, zp, L0, R0, eq.gamma);
+        }
+        wsz = ws_max(wsz, face_flux<T, Eq, Flux, 2>(eq, Rt, L0, Hm));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Rz[v] = R0[v];
+    }
+
+    for (int zz = z0; zz < z1; ++zz) {
+        const int pz = zz + D.gz;
+        T qn[NV];
+        PendingCell<T, NV> pq, ph0, ph1;
+        pq.issue(in, in2, LAZY, idx(py, pz + 2));
+        if (lo_halo || hi_halo) {
+            ph0.issue(in, in2, LAZY, idx(h0, pz));
+            ph1.issue(in, in2, LAZY, idx(h0 + 1, pz));
+        }
+        T u0[NV];
+        if constexpr (KEEP_U0) {
+            // loaded at idx(py, pz) two planes ago: the same cell (no
+            // clamping reaches the interior planes, which lie G or more
+            // planes inside the padded volume)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u0[v] = u_zc[v];
+        } else if constexpr (MODE != MODE_RHS && UNI) {
+            // every lane's cell (clamped rows past the group end) lies in
+            // the volume: loaded unconditionally, before the sweep
+            const auto i0 = idx(py, pz);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
+        } else if constexpr (MODE != MODE_RHS) {
+            if (interior) {
+                const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
+            }
+        }
+        T out_v[NV];
+        {  // x
+            T xm[NV], xp[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xm[v] = XLOAD ? xmn[v] : lane_from_lower(zc[v]);
+                xp[v] = XLOAD ? xpn[v] : lane_from_upper(zc[v]);
+            }
+            T Lx[NV], Rx[NV];
+            if constexpr (FW) {
+                const T wf = Recon::face_weight(zc[0], xp[0]);  // face col+1/2
+                Recon::template apply_w<T, NV>(xm, zc, xp, wf, lane_from_lower(wf), Lx, Rx);
+            } else {
+                Recon::template apply<T, NV>(xm, zc, xp, Lx, Rx, eq.gamma);
+            }
+            // the next plane's x-neighbours, in flight during the rest of
+            // the plane (after the last plane this reads plane z1 + gz <=
+            // tz - 1, a ghost plane); issued once this plane's are consumed
+            if constexpr (XLOAD) {
+                __builtin_amdgcn_sched_barrier(0);
+                load_xn(pz + 1);
+            }
+            T Ln[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
+            T Fp[NV];
+            wsx = ws_max(wsx, face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const T Fm = lane_from_lower(Fp[v]);
+                out_v[v] = T(0) + ((T(-1.0) * Fp[v]) + Fm);
+            }
+        }
+        // ---- y, phase 1: publish the plane's rows
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_rows[w + 2][v][lane] = zc[v];
+        if (lo_halo || hi_halo) {
+            T q0[NV], q1[NV];
+            ph0.finish(LAZY, c, q0);
+            ph1.finish(LAZY, c, q1);
+            const int k0 = lo_halo ? 0 : NW + 2;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                s_rows[k0][v][lane] = q0[v];
+                s_rows[k0 + 1][v][lane] = q1[v];
+            }
+        }
+        __syncthreads();
+        // ---- y, phase 2: reconstruct the own row (and the halo row on the edge)
+        T L0[NV], R0[NV], Rb[NV];
+        {
+            T ym1[NV], yp1[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                ym1[v] = s_rows[w + 1][v][lane];
+                yp1[v] = s_rows[w + 3][v][lane];
+            }
+            T wy1 = T(0), wy2 = T(0);  // FW: faces below and above the own row
+            if constexpr (FW) {
+                wy1 = Recon::face_weight(ym1[0], zc[0]);
+                wy2 = Recon::face_weight(zc[0], yp1[0]);
+                Recon::template apply_w<T, NV>(ym1, zc, yp1, wy2, wy1, L0, R0);
+            } else {
+                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0, eq.gamma);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_L[w][v][lane] = L0[v];
+            if (lo_halo) {  // right value of row y0-1
+                T q0[NV], Lb[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) q0[v] = s_rows[0][v][lane];
+                if constexpr (FW)
+                    Recon::template apply_w<T, NV>(q0, ym1, zc, wy1,
+                                                   Recon::face_weight(q0[0], ym1[0]), Lb, Rb);
+                else
+                    Recon::template apply<T, NV>(q0, ym1, zc, Lb, Rb, eq.gamma);
+            }
+            if (hi_halo) {  // left value of row y0+NW
+                T q1[NV], La[NV], Ra[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) q1[v] = s_rows[NW + 3][v][lane];
+                if constexpr (FW)
+                    Recon::template apply_w<T, NV>(zc, yp1, q1, Recon::face_weight(yp1[0], q1[0]),
+                                                   wy2, La, Ra);
+                else
+                    Recon::template apply<T, NV>(zc, yp1, q1, La, Ra, eq.gamma);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s_L[NW][v][lane] = La[v];
+            }
+        }
+        // ---- z (own row only; placed here to overlap the exchange)
+        T zpart[NV];
+        T wn = T(0);
+        {
+            pq.finish(LAZY, c, qn);
+            T Ln[NV], Rn[NV];
+            if constexpr (FW) {
+                wn = Recon::face_weight(zp[0], qn[0]);  // face pz+3/2
+                Recon::template apply_w<T, NV>(zc, zp, qn, wn, wz, Ln, Rn);
+            } else {
+                Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn, eq.gamma);
+            }
+            T Hp[NV];
+            wsz = ws_max(wsz, face_flux<T, Eq, Flux, 2>(eq, Rz, Ln, Hp));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                zpart[v] = (T(-1.0) * Hp[v]) + Hm[v];
+                Hm[v] = Hp[v];
+                Rz[v] = Rn[v];
+            }
+        }
+        __syncthreads();
+        // ---- y, phase 3: the flux of the face above the own row
+        T Gp[NV], Gm[NV];
+        {
+            T La[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) La[v] = s_L[w + 1][v][lane];
+            wsy = ws_max(wsy, face_flux<T, Eq, Flux, 1>(eq, R0, La, Gp));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_G[w + 1][v][lane] = Gp[v];
+            if (lo_halo)  // the face below row y0
+                wsy = ws_max(wsy, face_flux<T, Eq, Flux, 1>(eq, Rb, L0, Gm));
+        }
+        __syncthreads();
+        if (!lo_halo) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Gm[v] = s_G[w][v][lane];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
+            out_v[v] = out_v[v] + zpart[v];
+        }
+        if (interior) {
+            T res[NV];
+            combine<T, NV>(MODE, c, out_v, zc, u0, res);
+            if (CHECK && !eq.obeys(res)) {
+                atomicOr(&Sp->status, 2);
+                flagged = true;
+            }
+            if constexpr (UNI)
+                store_result_uni<T, NV, 3, OP == 0>(out, D, lo, xbo, xdo, row, zz, res);
+            else
+                store_result<T, NV, 3, OP == 0>(out, D, cimg, row, zz, res);
+        }
+        wz = wn;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            zc[v] = zp[v];
+            zp[v] = qn[v];
+            if constexpr (KEEP_U0) {
+                u_zc[v] = u_zp[v];
+                u_zp[v] = pq.b[v];
+            }
+        }
+    }
+
+    if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
+        // the lanes' running maxima over the planes: only the strip's
+        // faces count (x: col+1/2 from lane G-1; y, z: interior cells)
+        const T mx = segment_max(xface ? wsx : T(0), seg);
+        const T my = segment_max(interior ? wsy : T(0), seg);
+        const T mz = segment_max(interior ? wsz : T(0), seg);
+        if (in_strip && li == 0 && runnable(*Sp) && row_in) {
+            raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
+            raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
+            raise_wave_speed(a.ws + 3 * sample + 2, double(mz), a.ws_filter);
+        }
+    }
+}
+
+template <class T, class Eq, class Recon, class Flux, int WPS, int MODE, int OP = 0>
+__global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs<T> a) {
+    constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
     constexpr int NW = kGroupRows;
@@ -980,243 +1326,35 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
     __shared__ T s_L[NW + 1][NV][64];     // y-left face values of rows y0 .. y0+NW
     __shared__ T s_G[NW + 1][NV][64];     // y fluxes of the faces above rows y0-1 .. y0+NW-1
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
     Eq eq;
     eq.gamma = T(a.gamma);
 
     const int ntasks = a.ntasks_dev ? *a.ntasks_dev : a.ntasks;
     bool flagged = false;  // this thread set a sample's failure status
     for (int t = xcd_block(); t < ntasks; t += gridDim.x) {  // uniform over the workgroup
-        const WaveTask& tk = a.tasks[t];
-        int seg = -1;
-        for (int k = 0; k < tk.nseg; ++k)
-            if (lane >= tk.lane0[k] && lane < tk.lane0[k] + tk.ncols[k] + 2 * G) seg = k;
-        const bool in_strip = seg >= 0;
-        const int sk = in_strip ? seg : 0;
-        const int sample = tk.sample[sk];
-        const int li = lane - tk.lane0[sk];
-        const int ncols = tk.ncols[sk];
-        const SampleDesc D = a.desc[sample];
-        SampleState* Sp = a.state + sample;
-        const bool active = in_strip && runnable(*Sp);
-        // the samples of a task are the same for every wavefront of the group
-        if (!__syncthreads_or(active)) continue;
-        const bool need_dt = LAZY || MODE != MODE_RHS;
-        T c = T(0);
-        if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
-
-        const int row = tk.y0 + w;            // this wavefront's row
-        const bool row_in = row < tk.y1;      // rows past the group end only feed neighbours
-        const int col = tk.x0[sk] - G + li;
-        const bool lanes_in = active && li >= G && li < G + ncols;
-        const bool interior = lanes_in && row_in;
-        const bool xface = active && row_in && li >= G - 1 && li < G + ncols;
-        const int px = in_strip ? col + D.gx : D.gx;
-        const int py = row + D.gy;
-        const ColumnImages cimg = column_images(D, col);
-        const long long plane_stride = (long long)D.tx * D.ty;
-        const auto in = stage_in<OP>(a, D, false);
-        const auto in2 = stage_in<OP>(a, D, true);
-        const auto out = stage_out<OP>(a, D);
-
-        T wsx = T(0), wsy = T(0), wsz = T(0);
-        auto idx = [&](int pyy, int pz) -> long long {
-            pz = pz < 0 ? 0 : (pz >= D.tz ? D.tz - 1 : pz);
-            pyy = pyy < 0 ? 0 : (pyy >= D.ty ? D.ty - 1 : pyy);
-            return (long long)pz * plane_stride + (long long)pyy * D.tx + px;
-        };
-        // halo rows: the first wavefront loads y0-2, y0-1, the last y0+NW, y0+NW+1
-        const bool lo_halo = w == 0;
-        const bool hi_halo = w == NW - 1;
-        const int h0 = lo_halo ? tk.y0 - 2 + D.gy : tk.y0 + NW + D.gy;
-
-        T zc[NV], zp[NV], Rz[NV], Hm[NV];
-        T u_zc[NV], u_zp[NV];  // KEEP_U0: U^n at planes zc, zp
-        {
-            const int pz = tk.z0 + D.gz;
-            T qa[NV], zm[NV];
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 2), qa);
-            load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz - 1), zm);
-            if constexpr (KEEP_U0) {
-                PendingCell<T, NV> p0, p1;
-                p0.issue(in, in2, LAZY, idx(py, pz));
-                p1.issue(in, in2, LAZY, idx(py, pz + 1));
-                p0.finish(LAZY, c, zc);
-                p1.finish(LAZY, c, zp);
+        const WaveTask tk = scalar_load(a.tasks + t);
+        if (tk.nseg == 1 && uni_fits<T, NV>(a, tk.sample[0])) {
+            const bool in_strip = lane >= tk.lane0[0] && lane < tk.lane0[0] + tk.ncols[0] + 2 * G;
+            march_3<T, Eq, Recon, Flux, MODE, OP, true>(
+                a, eq, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
+                in_strip ? 0 : -1, tk.y0, tk.y1, tk.z0, tk.z1, s_rows, s_L, s_G, flagged);
+        } else {
+            int seg = -1, sample = tk.sample[0], x0 = tk.x0[0], ncols = tk.ncols[0],
+                lane0 = tk.lane0[0];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    u_zc[v] = p0.b[v];
-                    u_zp[v] = p1.b[v];
-                }
-            } else {
-                load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz), zc);
-                load_cell<T, NV>(in, in2, LAZY, c, idx(py, pz + 1), zp);
-            }
-            T Lt[NV], Rt[NV], L0[NV], R0[NV];
-            Recon::template apply<T, NV>(qa, zm, zc, Lt, Rt, eq.gamma);
-            Recon::template apply<T, NV>(zm, zc, zp, L0, R0, eq.gamma);
-            const T s = face_flux<T, Eq, Flux, 2>(eq, Rt, L0, Hm);
-            if (interior) wsz = std_max(wsz, s);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) Rz[v] = R0[v];
-        }
-
-        for (int zz = tk.z0; zz < tk.z1; ++zz) {
-            const int pz = zz + D.gz;
-            T qn[NV];
-            PendingCell<T, NV> pq, ph0, ph1;
-            pq.issue(in, in2, LAZY, idx(py, pz + 2));
-            if (lo_halo || hi_halo) {
-                ph0.issue(in, in2, LAZY, idx(h0, pz));
-                ph1.issue(in, in2, LAZY, idx(h0 + 1, pz));
-            }
-            const long long i0 = (long long)pz * plane_stride + (long long)py * D.tx + px;
-            T u0[NV];
-            if constexpr (KEEP_U0) {
-                // loaded at idx(py, pz) two planes ago: the same cell (no
-                // clamping reaches the interior planes, which lie G or more
-                // planes inside the padded volume)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = u_zc[v];
-            } else if (MODE != MODE_RHS && interior) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
-            }
-            T out_v[NV];
-            {  // x
-                T xm[NV], xp[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    xm[v] = lane_from_lower(zc[v]);
-                    xp[v] = lane_from_upper(zc[v]);
-                }
-                T Lx[NV], Rx[NV];
-                Recon::template apply<T, NV>(xm, zc, xp, Lx, Rx, eq.gamma);
-                T Ln[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Ln[v] = lane_from_upper(Lx[v]);
-                T Fp[NV];
-                const T s = face_flux<T, Eq, Flux, 0>(eq, Rx, Ln, Fp);
-                if (xface) wsx = std_max(wsx, s);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    const T Fm = lane_from_lower(Fp[v]);
-                    out_v[v] = T(0) + ((T(-1.0) * Fp[v]) + Fm);
+            for (int k = 0; k < kMaxSeg; ++k) {
+                if (k < tk.nseg && lane >= tk.lane0[k] &&
+                    lane < tk.lane0[k] + tk.ncols[k] + 2 * G) {
+                    seg = k;
+                    sample = tk.sample[k];
+                    x0 = tk.x0[k];
+                    ncols = tk.ncols[k];
+                    lane0 = tk.lane0[k];
                 }
             }
-            // ---- y, phase 1: publish the plane's rows
-#pragma unroll
-            for (int v = 0; v < NV; ++v) s_rows[w + 2][v][lane] = zc[v];
-            if (lo_halo || hi_halo) {
-                T q0[NV], q1[NV];
-                ph0.finish(LAZY, c, q0);
-                ph1.finish(LAZY, c, q1);
-                const int k0 = lo_halo ? 0 : NW + 2;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    s_rows[k0][v][lane] = q0[v];
-                    s_rows[k0 + 1][v][lane] = q1[v];
-                }
-            }
-            __syncthreads();
-            // ---- y, phase 2: reconstruct the own row (and the halo row on the edge)
-            T L0[NV], R0[NV], Rb[NV];
-            {
-                T ym1[NV], yp1[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    ym1[v] = s_rows[w + 1][v][lane];
-                    yp1[v] = s_rows[w + 3][v][lane];
-                }
-                Recon::template apply<T, NV>(ym1, zc, yp1, L0, R0, eq.gamma);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) s_L[w][v][lane] = L0[v];
-                if (lo_halo) {  // right value of row y0-1
-                    T q0[NV], Lb[NV];
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) q0[v] = s_rows[0][v][lane];
-                    Recon::template apply<T, NV>(q0, ym1, zc, Lb, Rb, eq.gamma);
-                }
-                if (hi_halo) {  // left value of row y0+NW
-                    T q1[NV], La[NV], Ra[NV];
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) q1[v] = s_rows[NW + 3][v][lane];
-                    Recon::template apply<T, NV>(zc, yp1, q1, La, Ra, eq.gamma);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) s_L[NW][v][lane] = La[v];
-                }
-            }
-            // ---- z (own row only; placed here to overlap the exchange)
-            T zpart[NV];
-            {
-                pq.finish(LAZY, c, qn);
-                T Ln[NV], Rn[NV];
-                Recon::template apply<T, NV>(zc, zp, qn, Ln, Rn, eq.gamma);
-                T Hp[NV];
-                const T s = face_flux<T, Eq, Flux, 2>(eq, Rz, Ln, Hp);
-                if (interior) wsz = std_max(wsz, s);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    zpart[v] = (T(-1.0) * Hp[v]) + Hm[v];
-                    Hm[v] = Hp[v];
-                    Rz[v] = Rn[v];
-                }
-            }
-            __syncthreads();
-            // ---- y, phase 3: the flux of the face above the own row
-            T Gp[NV], Gm[NV];
-            {
-                T La[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) La[v] = s_L[w + 1][v][lane];
-                const T s1 = face_flux<T, Eq, Flux, 1>(eq, R0, La, Gp);
-                if (interior) wsy = std_max(wsy, s1);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) s_G[w + 1][v][lane] = Gp[v];
-                if (lo_halo) {  // the face below row y0
-                    const T s0 = face_flux<T, Eq, Flux, 1>(eq, Rb, L0, Gm);
-                    if (interior) wsy = std_max(wsy, s0);
-                }
-            }
-            __syncthreads();
-            if (!lo_halo) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Gm[v] = s_G[w][v][lane];
-            }
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                out_v[v] = out_v[v] + ((T(-1.0) * Gp[v]) + Gm[v]);
-                out_v[v] = out_v[v] + zpart[v];
-            }
-            if (interior) {
-                T res[NV];
-                combine<T, NV>(MODE, c, out_v, zc, u0, res);
-                if (CHECK && !eq.obeys(res)) {
-                    atomicOr(&Sp->status, 2);
-                    flagged = true;
-                }
-                store_result<T, NV, 3, OP == 0>(out, D, cimg, row, zz, res);
-            }
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                zc[v] = zp[v];
-                zp[v] = qn[v];
-                if constexpr (KEEP_U0) {
-                    u_zc[v] = u_zp[v];
-                    u_zp[v] = pq.b[v];
-                }
-            }
-        }
-
-        if (MODE == MODE_RHS && !LAZY && !a.no_ws) {
-            const T mx = segment_max(wsx, seg);
-            const T my = segment_max(wsy, seg);
-            const T mz = segment_max(wsz, seg);
-            if (in_strip && li == 0 && runnable(*Sp) && row_in) {
-                raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
-                raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
-                raise_wave_speed(a.ws + 3 * sample + 2, double(mz), a.ws_filter);
-            }
+            march_3<T, Eq, Recon, Flux, MODE, OP, false>(a, eq, sample, x0, ncols, lane0, seg >= 0,
+                                                         seg, tk.y0, tk.y1, tk.z0, tk.z1, s_rows,
+                                                         s_L, s_G, flagged);
         }
         __syncthreads();  // the next task reuses the LDS rows
     }
