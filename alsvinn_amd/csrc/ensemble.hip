@@ -40,6 +40,44 @@ __global__ void slot_update_kernel(SampleDesc* desc, SampleState* state, unsigne
     ws[3 * slot + 2] = 0;
 }
 
+// Each listed sample's maximum wave speeds from the partial maxima its
+// tasks stored in the first stage (StageArgs::ws_part; bit patterns of
+// non-negative doubles, whose unsigned maximum is the maximum of the
+// values): workgroup i reduces the entries idx[off[i] .. off[i+1]) of sample
+// samples[i] and stores its three slots (the same maximum the atomics
+// would have formed, in any order).
+__global__ __launch_bounds__(256) void ws_reduce_kernel(const unsigned long long* __restrict__ part,
+                                                        const int* __restrict__ off,
+                                                        const int* __restrict__ idx,
+                                                        const int* __restrict__ samples,
+                                                        unsigned long long* ws) {
+    __shared__ unsigned long long red[3][4];
+    const int i = blockIdx.x;
+    unsigned long long m[3] = {0, 0, 0};
+    for (int j = off[i] + (int)threadIdx.x; j < off[i + 1]; j += (int)blockDim.x) {
+        const long long e = idx[j];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m[d] = part[e * 3 + d] > m[d] ? part[e * 3 + d] : m[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long x = __shfl_xor(m[d], o, 64);
+            m[d] = x > m[d] ? x : m[d];
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; ++d) red[d][wv] = m[d];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long r = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r = red[threadIdx.x][k] > r ? red[threadIdx.x][k] : r;
+        ws[3 * (long long)samples[i] + threadIdx.x] = r;
+    }
+}
+
 __global__ void set_hold_kernel(SampleState* state, int nsamples, int k, long long limit) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsamples) state[s].hold[k] = limit;
@@ -294,6 +332,8 @@ int set_tasks(const alsvinn_hip_ensemble* e, StageArgs<T>& a) {
     return (e->ntasks + 3) / 4;
 }
 
+void launch_ws_reduce(const alsvinn_hip_ensemble* e);
+
 template <class T>
 int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm = nullptr) {
     StageArgs<T> a;
@@ -328,6 +368,7 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     a.in = U;
     a.in2 = U;
     a.out = A;
+    a.ws_part = e->d_ws_part;
     auto stage = [&]() {
         if (tm) tm->begin();
         k.stage[e->variant[stage_kind(a.mode)]](a, blocks, e->stream);
@@ -340,6 +381,7 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
         if (tm) tm->end(1);
     };
     stage();
+    launch_ws_reduce(e);
     if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_FORWARD_EULER) {
         axpy(A, U, U, 1, end);
     } else if (e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK2) {
@@ -450,6 +492,53 @@ int alloc_pool_tasks(alsvinn_hip_ensemble* e, long long cap) {
     return 0;
 }
 
+void free_ws_part(alsvinn_hip_ensemble* e) {
+    (void)hipFree(e->d_ws_part);
+    (void)hipFree(e->d_ws_off);
+    (void)hipFree(e->d_ws_idx);
+    (void)hipFree(e->d_ws_samples);
+    e->d_ws_part = nullptr;
+    e->d_ws_off = e->d_ws_idx = e->d_ws_samples = nullptr;
+    e->ws_nlist = 0;
+}
+
+// Ensembles with large samples: the partial-maxima buffer of the first
+// stage and each sample's list of its entries (task, row in 3D, segment).
+int build_ws_part(alsvinn_hip_ensemble* e, const std::vector<WaveTask>& tasks) {
+    const int R = e->dim == 3 ? kGroupRows : 1;
+    std::vector<std::vector<int>> lists(e->nsamples);
+    for (size_t t = 0; t < tasks.size(); ++t)
+        for (int r = 0; r < R; ++r)
+            for (int k = 0; k < tasks[t].nseg; ++k)
+                lists[tasks[t].sample[k]].push_back(int((t * R + r) * kMaxSeg + k));
+    std::vector<int> off(1, 0), idx, smp;
+    for (int s = 0; s < e->nsamples; ++s) {
+        if (lists[s].empty()) continue;
+        smp.push_back(s);
+        idx.insert(idx.end(), lists[s].begin(), lists[s].end());
+        off.push_back((int)idx.size());
+    }
+    if (smp.empty()) return 0;
+    const size_t entries = tasks.size() * R * kMaxSeg * 3;
+    HIP_TRY(hipMalloc(&e->d_ws_part, sizeof(unsigned long long) * entries));
+    HIP_TRY(hipMemset(e->d_ws_part, 0, sizeof(unsigned long long) * entries));
+    HIP_TRY(hipMalloc(&e->d_ws_off, sizeof(int) * off.size()));
+    HIP_TRY(hipMalloc(&e->d_ws_idx, sizeof(int) * idx.size()));
+    HIP_TRY(hipMalloc(&e->d_ws_samples, sizeof(int) * smp.size()));
+    HIP_TRY(hipMemcpy(e->d_ws_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_ws_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_ws_samples, smp.data(), sizeof(int) * smp.size(), hipMemcpyHostToDevice));
+    e->ws_nlist = (int)smp.size();
+    return 0;
+}
+
+// After the first stage of a step with partial maxima: each sample's slots.
+void launch_ws_reduce(const alsvinn_hip_ensemble* e) {
+    if (!e->d_ws_part || e->ws_nlist == 0) return;
+    hipLaunchKernelGGL(ws_reduce_kernel, dim3(e->ws_nlist), dim3(256), 0, e->stream, e->d_ws_part,
+                       e->d_ws_off, e->d_ws_idx, e->d_ws_samples, e->d_ws);
+}
+
 int upload_tasks(alsvinn_hip_ensemble* e) {
     std::vector<WaveTask> tasks;
     build_tasks(e, tasks);
@@ -482,6 +571,11 @@ int upload_tasks(alsvinn_hip_ensemble* e) {
         for (int k = 0; k < t.nseg; ++k) per[t.sample[k]] += 1;
     const int most = per.empty() ? 0 : *std::max_element(per.begin(), per.end());
     e->ws_filter = most * (e->dim == 3 ? kGroupRows : 1) > kWsFilterWaves ? 1 : 0;
+    // large samples reduce their wave speeds through partial maxima and a
+    // reduction kernel instead of thousands of same-address atomics
+    free_ws_part(e);
+    if (e->ws_filter)
+        if (int rc = build_ws_part(e, tasks)) return rc;
     if (tasks.empty()) return 0;
     HIP_TRY(hipMalloc(&e->d_tasks, sizeof(WaveTask) * tasks.size()));
     HIP_TRY(hipMemcpy(e->d_tasks, tasks.data(), sizeof(WaveTask) * tasks.size(), hipMemcpyHostToDevice));
@@ -523,7 +617,9 @@ void time_stages(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int reps, doubl
         a.in = U;
         a.in2 = U;
         a.out = A;
+        a.ws_part = e->d_ws_part;
         k.stage[e->variant[0]](a, blocks, e->stream);
+        launch_ws_reduce(e);
         if (timed) hipEventRecord(ev[1], e->stream);
         if (!rk3) return;
         a.mode = MODE_RK3_1;
@@ -725,6 +821,7 @@ int alsvinn_hip_ensemble_destroy(alsvinn_hip_ensemble* e) {
     hipFree(e->d_state);
     hipFree(e->d_state_dry);
     hipFree(e->d_ws);
+    free_ws_part(e);
     if (e->pool) {
         (void)hipFree(e->d_taskbuf);
         for (int j = 0; j < alsvinn_hip_ensemble::kRing; ++j) {

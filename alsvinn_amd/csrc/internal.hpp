@@ -248,6 +248,14 @@ struct alsvinn_hip_ensemble {
     int* d_count = nullptr;
     unsigned int* d_end_counter = nullptr;  // workgroups done in the step's last kernel (0 between launches)
     int ws_filter = 0;                      // StageArgs::ws_filter for this work list
+    // ensembles with large samples: the first stage's partial maxima
+    // (StageArgs::ws_part) and, per sample that has tasks, its entries
+    // (ws_idx[ws_off[i] .. ws_off[i+1]) for sample ws_samples[i])
+    unsigned long long* d_ws_part = nullptr;
+    int* d_ws_off = nullptr;
+    int* d_ws_idx = nullptr;
+    int* d_ws_samples = nullptr;
+    int ws_nlist = 0;
     int* h_count = nullptr;
     double* d_dt_hist = nullptr;
     long long dt_cap = 0;

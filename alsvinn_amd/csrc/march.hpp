@@ -106,6 +106,12 @@ struct StageArgs {
     // sample's slots would otherwise serialise; with few wavefronts per
     // sample the plain atomic (no wait for the read) is cheaper
     int ws_filter;
+    // when set (ensembles with large samples), the first stage stores each
+    // task's segment maxima here instead: entry ((slot * kMaxSeg + seg) * 3 +
+    // d), slot the task's index (3D: times kGroupRows, plus the row), and a
+    // reduction kernel forms each sample's maximum after the launch
+    // (thousands of same-address atomics at the end of a launch serialise)
+    unsigned long long* ws_part;
     const T* in;
     const T* in2;  // U0
     T* out;
@@ -354,6 +360,22 @@ __device__ __forceinline__ void raise_wave_speed(unsigned long long* slot, doubl
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
     if (!filter || b > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicMax(slot, b);
+}
+
+// A segment's maximum wave speed m in direction d, reported by its first
+// lane (`leader`) for a sample that is (`run`) or is not stepping: into the
+// task's partial-maxima entry (a.ws_part; 0 for a sample not stepping, so
+// the entry is written every launch) or onto the sample's slot.
+template <class T>
+__device__ __forceinline__ void report_wave_speed(const StageArgs<T>& a, long long slot, int seg,
+                                                  int sample, int d, double m, bool leader,
+                                                  bool run) {
+    if (!leader) return;
+    if (a.ws_part)
+        a.ws_part[(slot * kMaxSeg + seg) * 3 + d] =
+            run ? (unsigned long long)__double_as_longlong(m) : 0ull;
+    else if (run)
+        raise_wave_speed(a.ws + 3 * sample + d, m, a.ws_filter);
 }
 
 template <class T, int NV>
@@ -670,9 +692,9 @@ __device__ __forceinline__ void store_result_uni(const UniVol<T>& out, const Sam
 // so sample, x0, ncols and lane0 are wave-uniform and the volumes are
 // addressed from scalar bases; otherwise each lane passes its own segment.
 template <class T, class Eq, class Recon, class Flux, int DIM, int MODE, int OP, bool UNI>
-__device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, int sample, int x0,
-                                         int ncols, int lane0, bool in_strip, int seg, int y0,
-                                         int y1, bool& flagged) {
+__device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, int task, int sample,
+                                         int x0, int ncols, int lane0, bool in_strip, int seg,
+                                         int y0, int y1, bool& flagged) {
     constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
@@ -905,10 +927,9 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
         // y: the interior columns)
         const T mx = segment_max(xface ? wsx : T(0), seg);
         const T my = segment_max(interior ? wsy : T(0), seg);
-        if (in_strip && li == 0 && runnable(*Sp)) {
-            raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
-            if (DIM >= 2) raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
-        }
+        const bool leader = in_strip && li == 0, run = leader && runnable(*Sp);
+        report_wave_speed(a, task, seg, sample, 0, double(mx), leader, run);
+        if (DIM >= 2) report_wave_speed(a, task, seg, sample, 1, double(my), leader, run);
     }
 }
 
@@ -937,7 +958,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
         if (tk.nseg == 1 && uni_fits<T, Eq::nvar>(a, tk.sample[0])) {
             const bool in_strip = lane >= tk.lane0[0] && lane < tk.lane0[0] + tk.ncols[0] + 2 * G;
             march_12<T, Eq, Recon, Flux, DIM, MODE, OP, true>(
-                a, eq, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
+                a, eq, w, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
                 in_strip ? 0 : -1, tk.y0, tk.y1, flagged);
         } else {
             // this lane's segment
@@ -954,7 +975,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_12(StageArgs<T> a) {
                     lane0 = tk.lane0[k];
                 }
             }
-            march_12<T, Eq, Recon, Flux, DIM, MODE, OP, false>(a, eq, sample, x0, ncols, lane0,
+            march_12<T, Eq, Recon, Flux, DIM, MODE, OP, false>(a, eq, w, sample, x0, ncols, lane0,
                                                                seg >= 0, seg, tk.y0, tk.y1,
                                                                flagged);
         }
@@ -980,8 +1001,8 @@ constexpr int kGroupRows = 4;
 // the workgroup's lanes all belong to one sample, addressed from scalar
 // bases through buffer resources. s_rows, s_L, s_G: the group's LDS rows.
 template <class T, class Eq, class Recon, class Flux, int MODE, int OP, bool UNI, int NV = Eq::nvar>
-__device__ __forceinline__ void march_3(const StageArgs<T>& a, const Eq& eq, int sample, int x0,
-                                        int ncols, int lane0, bool in_strip, int seg, int y0,
+__device__ __forceinline__ void march_3(const StageArgs<T>& a, const Eq& eq, int task, int sample,
+                                        int x0, int ncols, int lane0, bool in_strip, int seg, int y0,
                                         int y1, int z0, int z1, T (*s_rows)[NV][64],
                                         T (*s_L)[NV][64], T (*s_G)[NV][64], bool& flagged) {
     constexpr bool LAZY = MODE == MODE_RK3_1;
@@ -1308,11 +1329,11 @@ __device__ __forceinline__ void march_3(const StageArgs<T>& a, const Eq& eq, int
         const T mx = segment_max(xface ? wsx : T(0), seg);
         const T my = segment_max(interior ? wsy : T(0), seg);
         const T mz = segment_max(interior ? wsz : T(0), seg);
-        if (in_strip && li == 0 && runnable(*Sp) && row_in) {
-            raise_wave_speed(a.ws + 3 * sample + 0, double(mx), a.ws_filter);
-            raise_wave_speed(a.ws + 3 * sample + 1, double(my), a.ws_filter);
-            raise_wave_speed(a.ws + 3 * sample + 2, double(mz), a.ws_filter);
-        }
+        const bool leader = in_strip && li == 0, run = leader && runnable(*Sp) && row_in;
+        const long long slot = (long long)task * NW + w;
+        report_wave_speed(a, slot, seg, sample, 0, double(mx), leader, run);
+        report_wave_speed(a, slot, seg, sample, 1, double(my), leader, run);
+        report_wave_speed(a, slot, seg, sample, 2, double(mz), leader, run);
     }
 }
 
@@ -1336,7 +1357,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
         if (tk.nseg == 1 && uni_fits<T, NV>(a, tk.sample[0])) {
             const bool in_strip = lane >= tk.lane0[0] && lane < tk.lane0[0] + tk.ncols[0] + 2 * G;
             march_3<T, Eq, Recon, Flux, MODE, OP, true>(
-                a, eq, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
+                a, eq, t, tk.sample[0], tk.x0[0], tk.ncols[0], tk.lane0[0], in_strip,
                 in_strip ? 0 : -1, tk.y0, tk.y1, tk.z0, tk.z1, s_rows, s_L, s_G, flagged);
         } else {
             int seg = -1, sample = tk.sample[0], x0 = tk.x0[0], ncols = tk.ncols[0],
@@ -1352,7 +1373,7 @@ __global__ __launch_bounds__(64 * kGroupRows, WPS) void stage_kernel_3(StageArgs
                     lane0 = tk.lane0[k];
                 }
             }
-            march_3<T, Eq, Recon, Flux, MODE, OP, false>(a, eq, sample, x0, ncols, lane0, seg >= 0,
+            march_3<T, Eq, Recon, Flux, MODE, OP, false>(a, eq, t, sample, x0, ncols, lane0, seg >= 0,
                                                          seg, tk.y0, tk.y1, tk.z0, tk.z1, s_rows,
                                                          s_L, s_G, flagged);
         }
@@ -1583,8 +1604,8 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 const T m = segment_max(ws[d], seg);
-                if (in_strip && li == 0 && runnable(*Sp))
-                    raise_wave_speed(a.ws + 3 * sample + d, double(m), a.ws_filter);
+                const bool leader = in_strip && li == 0;
+                report_wave_speed(a, w, seg, sample, d, double(m), leader, leader && runnable(*Sp));
             }
         }
     }
