@@ -46,12 +46,12 @@ __global__ void slot_update_kernel(SampleDesc* desc, SampleState* state, unsigne
 // values): workgroup i reduces the entries idx[off[i] .. off[i+1]) of sample
 // samples[i] and stores its three slots (the same maximum the atomics
 // would have formed, in any order).
-__global__ __launch_bounds__(256) void ws_reduce_kernel(const unsigned long long* __restrict__ part,
-                                                        const int* __restrict__ off,
-                                                        const int* __restrict__ idx,
-                                                        const int* __restrict__ samples,
-                                                        unsigned long long* ws) {
-    __shared__ unsigned long long red[3][4];
+__global__ __launch_bounds__(1024) void ws_reduce_kernel(const unsigned long long* __restrict__ part,
+                                                         const int* __restrict__ off,
+                                                         const int* __restrict__ idx,
+                                                         const int* __restrict__ samples,
+                                                         unsigned long long* ws) {
+    __shared__ unsigned long long red[3][16];
     const int i = blockIdx.x;
     unsigned long long m[3] = {0, 0, 0};
     for (int j = off[i] + (int)threadIdx.x; j < off[i + 1]; j += (int)blockDim.x) {
@@ -535,7 +535,7 @@ int build_ws_part(alsvinn_hip_ensemble* e, const std::vector<WaveTask>& tasks) {
 // After the first stage of a step with partial maxima: each sample's slots.
 void launch_ws_reduce(const alsvinn_hip_ensemble* e) {
     if (!e->d_ws_part || e->ws_nlist == 0) return;
-    hipLaunchKernelGGL(ws_reduce_kernel, dim3(e->ws_nlist), dim3(256), 0, e->stream, e->d_ws_part,
+    hipLaunchKernelGGL(ws_reduce_kernel, dim3(e->ws_nlist), dim3(1024), 0, e->stream, e->d_ws_part,
                        e->d_ws_off, e->d_ws_idx, e->d_ws_samples, e->d_ws);
 }
 
