@@ -650,7 +650,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_stage,
-                     "kernel": "stage_kernel_12<double, EulerEq<double,2>, ReconWENO2, FluxHLL3, 2>",
+                     "kernel": "stage_kernel_12<{t}, EulerEq<{t},2>, ReconWENO2, FluxHLL3, 2>".format(
+                         t="double" if args.precision == "fp64" else "float"),
                      "avg_launch_ms": stage_ms, "launches": launches[0]},
         "cpu_baseline": cpu,
         "extra": extra,
