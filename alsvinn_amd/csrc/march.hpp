@@ -698,6 +698,10 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
     constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
+    // lazy stage: U^n of rows r, r+1 stays in registers from the lazy load
+    // that formed the stage input there, instead of being loaded again for
+    // the combination
+    constexpr bool KEEP_U0 = LAZY && DIM >= 2;
     constexpr bool CHECK = MODE == MODE_RK2_1 || MODE == MODE_RK3_2;  // the step's last stage
     constexpr bool PRIM = recon_prim<Recon>::value;
     constexpr bool FW = recon_face_weights<Recon>::value && !PRIM;
@@ -763,6 +767,7 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
     };
 
     T yc[NV], yp[NV];  // rows r, r+1 of the stage input
+    T u_yc[NV], u_yp[NV];  // KEEP_U0: U^n of rows r, r+1
     T xmn[NV], xpn[NV];  // XLOAD: x-neighbours of the next row to sweep
     auto load_xn = [&](int py) {
         if constexpr (XLOAD) {
@@ -787,8 +792,22 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
         T qa[NV], ym[NV];
         load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 2), qa);
         load_cell<T, NV>(in, in2, LAZY, c, row_index(pr - 1), ym);
-        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), yc);
-        load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), yp);
+        if constexpr (KEEP_U0) {
+            // rows pr, pr + 1 <= ny + gy lie inside the padded volume
+            PendingCell<T, NV> p0, p1;
+            p0.issue(in, in2, LAZY, row_at(pr));
+            p1.issue(in, in2, LAZY, row_at(pr + 1));
+            p0.finish(LAZY, c, yc);
+            p1.finish(LAZY, c, yp);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                u_yc[v] = p0.b[v];
+                u_yp[v] = p1.b[v];
+            }
+        } else {
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr), yc);
+            load_cell<T, NV>(in, in2, LAZY, c, row_index(pr + 1), yp);
+        }
         T Lt[NV], Rt[NV], L0[NV], R0[NV];
         if constexpr (PRIM) {
             T Pa[NV], Pm[NV];
@@ -827,7 +846,13 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
         // the sweep (loaded after it, its latency would stall the wave);
         // every lane's column lies inside the padded row
         T u0[NV];
-        if constexpr (MODE != MODE_RHS) {
+        if constexpr (KEEP_U0) {
+            // loaded at row_index(pr) two rows ago: the same cell (an
+            // interior row lies gy or more rows inside the padded volume,
+            // where no clamping reaches)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u0[v] = u_yc[v];
+        } else if constexpr (MODE != MODE_RHS) {
             const auto i0 = row_at(pr);
 #pragma unroll
             for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
@@ -913,6 +938,10 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
             for (int v = 0; v < NV; ++v) {
                 yc[v] = yp[v];
                 yp[v] = qn[v];
+                if constexpr (KEEP_U0) {
+                    u_yc[v] = u_yp[v];
+                    u_yp[v] = pn.b[v];
+                }
                 if constexpr (PRIM) {
                     Pc[v] = Pp[v];
                     Pp[v] = Pn[v];
