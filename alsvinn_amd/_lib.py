@@ -50,6 +50,8 @@ EXPORTS = [
     "alsvinn_hip_ensemble_autotune",
     "alsvinn_hip_ensemble_get_stage_variants",
     "alsvinn_hip_ensemble_get_stage_waves",
+    "alsvinn_hip_ensemble_set_small_1d",
+    "alsvinn_hip_ensemble_get_small_1d",
     "alsvinn_hip_ensemble_step",
     "alsvinn_hip_ensemble_run",
     "alsvinn_hip_ensemble_set_hold",
@@ -186,6 +188,8 @@ def load():
     L.alsvinn_hip_ensemble_get_schedule.argtypes = [vp, ip, ip, ip]
     L.alsvinn_hip_ensemble_get_stage_variants.argtypes = [vp, ip]
     L.alsvinn_hip_ensemble_get_stage_waves.argtypes = [vp, ip]
+    L.alsvinn_hip_ensemble_set_small_1d.argtypes = [vp, ctypes.c_int]
+    L.alsvinn_hip_ensemble_get_small_1d.argtypes = [vp, ip]
     L.alsvinn_hip_ensemble_autotune.argtypes = [vp, ip, ip]
     L.alsvinn_hip_ensemble_step.argtypes = [vp, ctypes.c_int]
     L.alsvinn_hip_ensemble_run.argtypes = [vp, ctypes.c_int]

@@ -438,7 +438,51 @@ int enqueue_step(alsvinn_hip_ensemble* e, const KernelSet<T>& k, KernelTimer* tm
     return 0;
 }
 
+// Whether the ensemble's steps run as whole-step launches of
+// solve_kernel_1d (sets solve1d_threads: 0, or 64 per wavefront of a
+// sample's workgroup, one wavefront per strip up to 16).
+void update_solve1d(alsvinn_hip_ensemble* e) {
+    e->solve1d_threads = 0;
+    const bool has = e->esize == 8 ? e->k64.solve1d != nullptr : e->k32.solve1d != nullptr;
+    if (!has || e->dim != 1 || e->pool || e->solve1d_rounds <= 0 || e->nsamples > kSolve1DSamples ||
+        e->scheme.integrator == ALSVINN_HIP_INTEGRATOR_RK4)
+        return;
+    const int W = 64 - 2 * e->ghosts;
+    long long strips = 1;
+    for (const SampleDesc& D : e->desc) strips = std::max<long long>(strips, (D.nx + W - 1) / W);
+    if (strips > 16LL * e->solve1d_rounds) return;
+    e->solve1d_threads = 64 * (int)std::min<long long>(16, strips);
+}
+
+template <class T>
+void enqueue_solve1d(alsvinn_hip_ensemble* e, const KernelSet<T>& k, int nsteps) {
+    Solve1DArgs<T> p;
+    std::memset(&p, 0, sizeof(p));
+    p.a.desc = e->d_desc;
+    p.a.state = e->d_state;
+    p.a.ws = e->d_ws;
+    p.a.gamma = e->scheme.gamma;
+    p.U = (T*)e->U;
+    p.A = (T*)e->A;
+    p.B = (T*)e->B;
+    p.integrator = e->scheme.integrator;
+    p.nsteps = nsteps;
+    p.dt_hist = e->d_dt_hist;
+    p.dt_cap = e->dt_cap;
+    k.solve1d(p, e->nsamples, e->solve1d_threads, e->stream);
+}
+
 int enqueue_steps(alsvinn_hip_ensemble* e, int nsteps, KernelTimer* tm = nullptr) {
+    if (e->solve1d_threads > 0 && !tm) {  // per-kernel timing keeps the multi-launch step
+        if (nsteps > 0) {
+            if (e->esize == 8)
+                enqueue_solve1d<double>(e, e->k64, nsteps);
+            else
+                enqueue_solve1d<float>(e, e->k32, nsteps);
+        }
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     for (int i = 0; i < nsteps; ++i) {
         if (e->esize == 8)
             enqueue_step<double>(e, e->k64, tm);
@@ -721,6 +765,7 @@ static int finish_create(alsvinn_hip_ensemble* e, const alsvinn_hip_scheme* sche
     HIP_TRY_C(hipEventCreate(&e->ev0));
     HIP_TRY_C(hipEventCreate(&e->ev1));
 #undef HIP_TRY_C
+    update_solve1d(e);
     if (int r = init_state(e)) return cleanup(r);
     *out = e;
     return 0;
@@ -1124,6 +1169,21 @@ int alsvinn_hip_ensemble_set_schedule(alsvinn_hip_ensemble* e, int rows, int var
     e->variant[0] = e->variant[1] = e->variant[2] = variant;
     e->tuned = true;
     return upload_tasks(e);
+}
+
+int alsvinn_hip_ensemble_set_small_1d(alsvinn_hip_ensemble* e, int max_rounds) {
+    if (!e || max_rounds < 0) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    destroy_graph(e);
+    e->solve1d_rounds = max_rounds;
+    update_solve1d(e);
+    return 0;
+}
+
+int alsvinn_hip_ensemble_get_small_1d(const alsvinn_hip_ensemble* e, int* threads) {
+    if (!e || !threads) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    *threads = e->solve1d_threads;
+    return 0;
 }
 
 int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* variants) {

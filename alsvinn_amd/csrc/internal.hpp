@@ -26,6 +26,12 @@ constexpr int kDefaultVariant = 1;
 // A sample spanning more wavefronts than this reads its wave-speed slots
 // before raising them (StageArgs::ws_filter).
 constexpr int kWsFilterWaves = 128;
+// Small 1D solves run whole timesteps per launch (solve_kernel_1d) when
+// each sample's strips take at most this many turns of a workgroup's 16
+// wavefronts and the ensemble has at most kSolve1DSamples samples (one
+// workgroup each; larger ensembles fill the GPU with the multi-launch step).
+constexpr int kSolve1DRounds = 4;
+constexpr int kSolve1DSamples = 256;
 // Stage kind of a stage mode, for the per-kind register budget.
 inline int stage_kind(int mode) { return mode == MODE_RK3_1 ? 1 : (mode == MODE_RK3_2 ? 2 : 0); }
 
@@ -46,6 +52,9 @@ struct KernelSet {
     void (*op_flux)(const StageArgs<T>& a, int blocks, hipStream_t s) = nullptr;
     void (*op_recon)(const ReconArgs<T>& a, int blocks, hipStream_t s) = nullptr;
     int recon_radius = 1;
+    // whole timesteps of small 1D samples, one workgroup per sample (1D
+    // schemes on the marching kernel; null otherwise)
+    void (*solve1d)(const Solve1DArgs<T>& p, int nsamples, int threads, hipStream_t s) = nullptr;
 };
 
 // Each stage kind (first stage, RK2 second stage, SSP-RK3 second and third
@@ -129,8 +138,15 @@ void launch_op_recon(const ReconArgs<T>& a, int blocks, hipStream_t s) {
     hipLaunchKernelGGL((recon_volume_kernel<T, Eq, Recon>), dim3(blocks), dim3(256), 0, s, a);
 }
 
+template <class T, class Eq, class Recon, class Flux>
+void launch_solve1d(const Solve1DArgs<T>& p, int nsamples, int threads, hipStream_t s) {
+    hipLaunchKernelGGL((solve_kernel_1d<T, Eq, Recon, Flux>), dim3(nsamples), dim3(threads), 0, s, p);
+}
+
 template <class T, class Eq, class Recon, class Flux, int DIM>
 void set_stage(KernelSet<T>& k) {
+    if constexpr (DIM == 1 && recon_radius<Recon>::value <= 1)
+        k.solve1d = &launch_solve1d<T, Eq, Recon, Flux>;
     k.op_flux = &launch_op_flux<T, Eq, Recon, Flux, DIM>;
     k.op_recon = &launch_op_recon<T, Eq, Recon>;
     k.recon_radius = recon_radius<Recon>::value;
@@ -272,6 +288,11 @@ struct alsvinn_hip_ensemble {
                       alsvinn_hip::kDefaultVariant};
     int band_rows = 0;  // rows per wave task; 0 = heuristic
     bool tuned = false;
+    // small 1D solves (solve_kernel_1d): eligible when every sample's strips
+    // take at most solve1d_rounds turns of a 16-wavefront workgroup (0: never);
+    // solve1d_threads is the workgroup size in use (0: the multi-launch step)
+    int solve1d_rounds = alsvinn_hip::kSolve1DRounds;
+    int solve1d_threads = 0;
     // pools: the work list lives in a preallocated device buffer (an int
     // count, then up to task_cap tasks) that asynchronous copies from pinned
     // staging buffers update, and the stage kernels read the count at run

@@ -250,6 +250,18 @@ __device__ __forceinline__ double sample_dt(const SampleDesc& D, const SampleSta
 // End of a timestep of sample s (Simulator::incrementSolution tail,
 // Simulator::atEnd, FixedIntervalWriter::write): t += dt, save counters,
 // stop flag, wave-speed reset; the dt is recorded when asked.
+__device__ __forceinline__ void advance_sample(const SampleDesc& D, SampleState& S, double dt) {
+    S.t = S.t + dt;
+    S.steps += 1;
+    S.last_dt = dt;
+    // constant indices: the state stays in registers
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (k < D.nadj && S.t >= double(S.saved[k]) * D.interval[k] + 0.0) S.saved[k] += 1;
+    if (S.status != 0 || S.t >= D.end_time || (D.max_steps >= 0 && S.steps >= D.max_steps))
+        S.active = 0;
+}
+
 __device__ __forceinline__ void finish_sample(const SampleDesc* __restrict__ desc, SampleState* state,
                                               unsigned long long* ws, double* dt_hist,
                                               long long dt_cap, int s) {
@@ -258,20 +270,22 @@ __device__ __forceinline__ void finish_sample(const SampleDesc* __restrict__ des
     if (!runnable(S)) return;
     const double dt = sample_dt(D, S, ws + 3 * s);
     if (dt_hist && S.steps < dt_cap) dt_hist[(long long)s * dt_cap + S.steps] = dt;
-    S.t = S.t + dt;
-    S.steps += 1;
-    S.last_dt = dt;
-    // constant indices: the state stays in registers
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-        if (k < D.nadj && S.t >= double(S.saved[k]) * D.interval[k] + 0.0) S.saved[k] += 1;
     ws[3 * s + 0] = 0;
     ws[3 * s + 1] = 0;
     ws[3 * s + 2] = 0;
-    if (S.status != 0 || S.t >= D.end_time || (D.max_steps >= 0 && S.steps >= D.max_steps))
-        S.active = 0;
+    advance_sample(D, S, dt);
     state[s] = S;
 }
+
+// A sample that one workgroup advances alone for whole steps
+// (solve_kernel_1d): the stages read the step's dt from here and raise the
+// wave-speed maxima and the failure flag in LDS, instead of the sample's
+// global state and slots (its copy of the state lives in registers).
+struct LocalStep {
+    unsigned long long* ws;  // LDS: this step's maxima, bit patterns (3)
+    int* status;             // LDS: 2 once a constraint check failed
+    double dt;               // the step's dt (the stages after the first)
+};
 
 // The end of a step fused into the step's last kernel: every workgroup
 // counts itself out once its work is done, and the last one to finish (its
@@ -691,10 +705,12 @@ __device__ __forceinline__ void store_result_uni(const UniVol<T>& out, const Sam
 // [y0, y1). UNI: every lane of the wave belongs to this segment's sample,
 // so sample, x0, ncols and lane0 are wave-uniform and the volumes are
 // addressed from scalar bases; otherwise each lane passes its own segment.
-template <class T, class Eq, class Recon, class Flux, int DIM, int MODE, int OP, bool UNI>
+template <class T, class Eq, class Recon, class Flux, int DIM, int MODE, int OP, bool UNI,
+          bool LOCAL = false>
 __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, int task, int sample,
                                          int x0, int ncols, int lane0, bool in_strip, int seg,
-                                         int y0, int y1, bool& flagged) {
+                                         int y0, int y1, bool& flagged,
+                                         const LocalStep ls = LocalStep{}) {
     constexpr bool LAZY = MODE == MODE_RK3_1;  // stage input U1 = L0*c + U0, formed on the fly
     constexpr int NV = Eq::nvar;
     constexpr int G = Recon::ghosts;
@@ -713,11 +729,12 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
             return a.desc[sample];
     }();
     SampleState* Sp = a.state + sample;
-    const bool active = in_strip && runnable(*Sp);
+    const bool active = in_strip && (LOCAL || runnable(*Sp));
     if (!__any(active)) return;  // wave-uniform
     const bool need_dt = LAZY || MODE != MODE_RHS;
     T c = T(0);
-    if (need_dt && active) c = T(sample_dt(D, *Sp, a.ws + 3 * sample) / D.dx[0]);
+    if (need_dt && active)
+        c = T((LOCAL ? ls.dt : sample_dt(D, *Sp, a.ws + 3 * sample)) / D.dx[0]);
 
     const int li = lane - lane0;
     const int col = x0 - G + li;  // interior x coordinate of this lane
@@ -924,7 +941,10 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
             T res[NV];
             combine<T, NV>(MODE, c, out_v, yc, u0, res);
             if (CHECK && !eq.obeys(res)) {
-                atomicOr(&Sp->status, 2);
+                if constexpr (LOCAL)
+                    *ls.status = 2;
+                else
+                    atomicOr(&Sp->status, 2);
                 flagged = true;
             }
             if constexpr (UNI)
@@ -956,9 +976,18 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
         // y: the interior columns)
         const T mx = segment_max(xface ? wsx : T(0), seg);
         const T my = segment_max(interior ? wsy : T(0), seg);
-        const bool leader = in_strip && li == 0, run = leader && runnable(*Sp);
-        report_wave_speed(a, task, seg, sample, 0, double(mx), leader, run);
-        if (DIM >= 2) report_wave_speed(a, task, seg, sample, 1, double(my), leader, run);
+        const bool leader = in_strip && li == 0;
+        if constexpr (LOCAL) {
+            if (leader) {
+                atomicMax(ls.ws + 0, (unsigned long long)__double_as_longlong(double(mx)));
+                if (DIM >= 2)
+                    atomicMax(ls.ws + 1, (unsigned long long)__double_as_longlong(double(my)));
+            }
+        } else {
+            const bool run = leader && runnable(*Sp);
+            report_wave_speed(a, task, seg, sample, 0, double(mx), leader, run);
+            if (DIM >= 2) report_wave_speed(a, task, seg, sample, 1, double(my), leader, run);
+        }
     }
 }
 

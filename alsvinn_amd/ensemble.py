@@ -130,6 +130,18 @@ class Ensemble:
         return {"rows": r.value, "variant": v.value, "waves_per_simd": waves[0],
                 "stage_waves_per_simd": list(waves), "tasks": n.value}
 
+    def set_small_1d(self, max_rounds):
+        """Whole-step launches for small 1D solves: eligible when each sample's
+        strips take at most max_rounds turns of 16 wavefronts (0: never).
+        Returns the workgroup size in use (0: the multi-launch step)."""
+        check(_lib.load().alsvinn_hip_ensemble_set_small_1d(self.h, max_rounds))
+        return self.small_1d()
+
+    def small_1d(self):
+        t = ctypes.c_int()
+        check(_lib.load().alsvinn_hip_ensemble_get_small_1d(self.h, ctypes.byref(t)))
+        return t.value
+
     def autotune(self):
         r, v = ctypes.c_int(), ctypes.c_int()
         check(_lib.load().alsvinn_hip_ensemble_autotune(self.h, ctypes.byref(r), ctypes.byref(v)))

@@ -224,6 +224,19 @@ int alsvinn_hip_ensemble_get_stage_variants(const alsvinn_hip_ensemble* e, int* 
  * autotuned schedule). */
 int alsvinn_hip_ensemble_get_stage_waves(const alsvinn_hip_ensemble* e, int* waves);
 
+/* Small 1D solves (no counterpart in the reference, whose Simulator runs
+ * one sample's step as a sequence of host calls): when the ensemble is 1D,
+ * not a pool, integrated by forward Euler, RK2 or SSP-RK3, has at most 256
+ * samples and every sample's strips of 64 - 2*ghosts cells take at most
+ * max_rounds turns of 16 wavefronts, one workgroup per sample runs whole
+ * timesteps in one launch (workgroup barriers between the stages) instead of
+ * two or three launches per step. The results are the same bits either way.
+ * max_rounds = 0 keeps the multi-launch step; the default is 4 (up to 3840
+ * cells with two ghost layers). get_small_1d reports the workgroup size in
+ * use (0: the multi-launch step). */
+int alsvinn_hip_ensemble_set_small_1d(alsvinn_hip_ensemble* e, int max_rounds);
+int alsvinn_hip_ensemble_get_small_1d(const alsvinn_hip_ensemble* e, int* threads);
+
 /* Enqueues nsteps timesteps (Simulator::performStep, Simulator.cpp:81-85) for
  * every sample that has not reached its end time. Asynchronous. */
 int alsvinn_hip_ensemble_step(alsvinn_hip_ensemble* e, int nsteps);
