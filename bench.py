@@ -589,12 +589,30 @@ def main():
         sod.run(50)  # same solve again on the instantiated graph
         t3 = time.perf_counter()
         assert sod.info(0)["steps"] == info["steps"]
+        whole_step = sod.small_1d()
+        # the same solve with the multi-launch step (two launches per step)
+        sod.set_small_1d(0)
+        sod.reset_time()
+        sod.upload(0, ic.sod(1024))
+        sod.apply_boundary()
+        sod.run(50)
+        sod.reset_time()
+        sod.upload(0, ic.sod(1024))
+        sod.apply_boundary()
+        sod.synchronize()
+        t4 = time.perf_counter()
+        sod.run(50)
+        t5 = time.perf_counter()
+        assert sod.info(0)["steps"] == info["steps"]
         sod.close()
         extra["sod"] = {"solve_ms": 1e3 * (t1 - t0), "warm_solve_ms": 1e3 * (t3 - t2),
                         "steps": info["steps"], "t": info["t"],
+                        "whole_step_launch_threads": whole_step,
+                        "multi_launch_warm_solve_ms": 1e3 * (t5 - t4),
                         "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
-        log("  sod: %d steps in %.2f ms (first solve, with graph capture), %.2f ms (again)"
-            % (info["steps"], 1e3 * (t1 - t0), 1e3 * (t3 - t2)))
+        log("  sod: %d steps in %.2f ms (first solve, with graph capture), %.2f ms (again); "
+            "multi-launch step %.2f ms" % (info["steps"], 1e3 * (t1 - t0), 1e3 * (t3 - t2),
+                                           1e3 * (t5 - t4)))
 
     if args.only == "all":
         log("fp32 errors per configuration")
