@@ -70,8 +70,7 @@ __device__ __forceinline__ T std_max(T a, T b) {
 // when y > 0). The refined reciprocal depends on y only, so quotients with a
 // common denominator share it (the compiler merges the identical chains).
 // The square root is the compiler's sequence (v_rsq_f64 and Newton
-// corrections) without the rescaling of inputs below 2^-767, with zero passed
-// through. Every operand of a valid state (positive density and pressure,
+// corrections) without the rescaling of inputs below 2^-767, exact for zero. Every operand of a valid state (positive density and pressure,
 // finite values) is in these ranges; zero, infinite or NaN denominators and
 // infinite square-root inputs, which arise only in states the constraint
 // check rejects, give NaN here instead of the IEEE special values.
@@ -104,7 +103,11 @@ __device__ __forceinline__ double dv(double a, double b) {
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float dvp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double dsqrt(double x) {
-    const double y0 = __builtin_amdgcn_rsq(x);
+    // x = +-0: the estimate 1/sqrt(x) is +-inf; 2^1023 in its place (the
+    // same low word, 0) keeps every step below exact, and the sequence then
+    // returns x itself, +-0, the IEEE square root of +-0
+    const double y1 = __builtin_amdgcn_rsq(x);
+    const double y0 = __hiloint2double(x == 0.0 ? 0x7fe00000 : __double2hiint(y1), __double2loint(y1));
     double g = x * y0;
     double h = y0 * 0.5;
     const double r = __builtin_fma(-h, g, 0.5);
@@ -113,8 +116,7 @@ __device__ __forceinline__ double dsqrt(double x) {
     double d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, h, g);
     d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    return x == 0.0 ? x : g;
+    return __builtin_fma(d, h, g);
 }
 // The same sequence without the zero pass-through, for the square roots of
 // face densities: a zero density is also the divisor of the face's velocity,
@@ -313,7 +315,7 @@ struct FluxHLL3 {
             T fX[nv];
             eq.template pointFlux<d>(X, fX);
             const T sX = left ? sL : sR;
-            const T udx = left ? udl : udr;
+            const T udx = X.u[d] - sX;  // udl or udr: the same difference
             const T rhoX = X.q[0];
             const T pX = X.p;
             const T uX = X.u[d];

@@ -59,9 +59,8 @@ def _div(a, b, est):
 
 
 def _sqrt(x, est):
-    if x == 0.0:
-        return x
-    y0 = est
+    # dsqrt(): the +-inf estimate of a zero operand is replaced by 2^1023
+    y0 = 2.0 ** 1023 if x == 0.0 else est
     g = x * y0
     h = y0 * 0.5
     r = _fma(-h, g, 0.5)
@@ -100,6 +99,15 @@ def test_sqrt_sequence_is_ieee_for_ordinary_operands():
         est = (1.0 / math.sqrt(v) if v > 0 else math.inf) * (1.0 + d)
         got = _sqrt(v, est)
         assert _bits(got) == _bits(math.sqrt(v)), v
+
+
+@pytest.mark.parametrize("zero", [0.0, -0.0])
+def test_sqrt_sequence_of_signed_zero(zero):
+    """A zero operand runs the whole sequence from the substituted estimate
+    2^1023 (no select on the result) and gives the IEEE square root, the
+    zero itself with its sign."""
+    got = _sqrt(zero, math.copysign(math.inf, zero))
+    assert _bits(got) == _bits(math.sqrt(zero)) == _bits(zero)
 
 
 def test_shared_reciprocal_gives_each_quotient():
