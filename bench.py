@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--serving-tmin", type=float, default=0.1)
     ap.add_argument("--serving-tmax", type=float, default=1.0)
     ap.add_argument("--only", choices=["all", "main", "fe-calibration", "serving", "full-uq", "kh3d",
-                                       "stats", "single"],
+                                       "stats", "single", "sod"],
                     default="all")
     ap.add_argument("--samples3d", type=int, default=64)
     ap.add_argument("--n3d", type=int, default=256)
@@ -335,6 +335,54 @@ def serving_bench(args):
     return out
 
 
+def sod_solve(hip):
+    """Sod 1024 (HLL, forward Euler, CFL 0.9, T = 0.2): a latency-bound
+    small solve, timed with the whole-step launch and with the multi-launch
+    step (the same 248 steps either way)."""
+    from alsvinn_amd import initial_data as ic
+    sod = hip.Ensemble(hip.make_scheme(hip.EQ_EULER1, hip.FLUX_HLL, hip.RECON_NONE, hip.INT_FE,
+                                       hip.BC_NEUMANN, cfl=0.9),
+                       [hip.make_sample((1024, 1, 1), (-1, 0, 0), (1, 0, 0), 0.2)])
+    sod.upload(0, ic.sod(1024))
+    sod.apply_boundary()
+    t0 = time.perf_counter()
+    sod.run(50)  # first solve: includes capturing and instantiating the step graph
+    t1 = time.perf_counter()
+    info = sod.info(0)
+    sod.reset_time()
+    sod.upload(0, ic.sod(1024))
+    sod.apply_boundary()
+    sod.synchronize()
+    t2 = time.perf_counter()
+    sod.run(50)  # same solve again on the instantiated graph
+    t3 = time.perf_counter()
+    assert sod.info(0)["steps"] == info["steps"]
+    whole_step = sod.small_1d()
+    # the same solve with the multi-launch step (two launches per step)
+    sod.set_small_1d(0)
+    sod.reset_time()
+    sod.upload(0, ic.sod(1024))
+    sod.apply_boundary()
+    sod.run(50)
+    sod.reset_time()
+    sod.upload(0, ic.sod(1024))
+    sod.apply_boundary()
+    sod.synchronize()
+    t4 = time.perf_counter()
+    sod.run(50)
+    t5 = time.perf_counter()
+    assert sod.info(0)["steps"] == info["steps"]
+    sod.close()
+    out = {"solve_ms": 1e3 * (t1 - t0), "warm_solve_ms": 1e3 * (t3 - t2),
+           "steps": info["steps"], "t": info["t"], "whole_step_launch_threads": whole_step,
+           "multi_launch_warm_solve_ms": 1e3 * (t5 - t4),
+           "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
+    log("  sod: %d steps in %.2f ms (first solve, with graph capture), %.2f ms (again); "
+        "multi-launch step %.2f ms" % (info["steps"], 1e3 * (t1 - t0), 1e3 * (t3 - t2),
+                                       1e3 * (t5 - t4)))
+    return out
+
+
 def main():
     args = parse()
     host_cpu()
@@ -404,6 +452,10 @@ def main():
             log("  %s: %s ms" % (name, ", ".join("%.2f" % t for t in ts)))
         e.close()
         print(json.dumps(out), flush=True)
+        return
+
+    if args.only == "sod":
+        print(json.dumps(sod_solve(hip)), flush=True)
         return
 
     if args.only == "single":
@@ -571,48 +623,7 @@ def main():
                               "workload": "KH 3D 256^3 x 64 samples, FP32 state"}
         log("  3d: %.3e cell-updates/s" % v3)
         log("sod")
-        from alsvinn_amd import initial_data as ic
-        sod = hip.Ensemble(hip.make_scheme(hip.EQ_EULER1, hip.FLUX_HLL, hip.RECON_NONE, hip.INT_FE,
-                                           hip.BC_NEUMANN, cfl=0.9),
-                           [hip.make_sample((1024, 1, 1), (-1, 0, 0), (1, 0, 0), 0.2)])
-        sod.upload(0, ic.sod(1024))
-        sod.apply_boundary()
-        t0 = time.perf_counter()
-        sod.run(50)  # first solve: includes capturing and instantiating the step graph
-        t1 = time.perf_counter()
-        info = sod.info(0)
-        sod.reset_time()
-        sod.upload(0, ic.sod(1024))
-        sod.apply_boundary()
-        sod.synchronize()
-        t2 = time.perf_counter()
-        sod.run(50)  # same solve again on the instantiated graph
-        t3 = time.perf_counter()
-        assert sod.info(0)["steps"] == info["steps"]
-        whole_step = sod.small_1d()
-        # the same solve with the multi-launch step (two launches per step)
-        sod.set_small_1d(0)
-        sod.reset_time()
-        sod.upload(0, ic.sod(1024))
-        sod.apply_boundary()
-        sod.run(50)
-        sod.reset_time()
-        sod.upload(0, ic.sod(1024))
-        sod.apply_boundary()
-        sod.synchronize()
-        t4 = time.perf_counter()
-        sod.run(50)
-        t5 = time.perf_counter()
-        assert sod.info(0)["steps"] == info["steps"]
-        sod.close()
-        extra["sod"] = {"solve_ms": 1e3 * (t1 - t0), "warm_solve_ms": 1e3 * (t3 - t2),
-                        "steps": info["steps"], "t": info["t"],
-                        "whole_step_launch_threads": whole_step,
-                        "multi_launch_warm_solve_ms": 1e3 * (t5 - t4),
-                        "workload": "Sod 1024 cells, HLL, FE, CFL 0.9, T=0.2 (latency-bound)"}
-        log("  sod: %d steps in %.2f ms (first solve, with graph capture), %.2f ms (again); "
-            "multi-launch step %.2f ms" % (info["steps"], 1e3 * (t1 - t0), 1e3 * (t3 - t2),
-                                           1e3 * (t5 - t4)))
+        extra["sod"] = sod_solve(hip)
 
     if args.only == "all":
         log("fp32 errors per configuration")
