@@ -38,7 +38,7 @@ __device__ __forceinline__ bool axpy_cells(const SampleDesc& D, T c, const T* X,
             const int pz = int(i / ((long long)D.tx * D.ty));
             const bool inside = px >= D.gx && px < D.gx + D.nx && py >= D.gy && py < D.gy + D.ny &&
                                 pz >= D.gz && pz < D.gz + D.nz;
-            if (inside && !eq.obeys(r)) bad = true;
+            if (inside && !eq.valid(r)) bad = true;
         }
     }
     return bad;
@@ -221,7 +221,7 @@ __device__ __forceinline__ bool rk4_sample(const SampleDesc& D, SampleState* Sp,
             const int pz = int(i / ((long long)D.tx * D.ty));
             const bool inside = px >= D.gx && px < D.gx + D.nx && py >= D.gy && py < D.gy + D.ny &&
                                 pz >= D.gz && pz < D.gz + D.nz;
-            if (inside && !eq.obeys(r)) bad = true;
+            if (inside && !eq.valid(r)) bad = true;
         }
     }
     if (bad) atomicOr(&Sp->status, 2);

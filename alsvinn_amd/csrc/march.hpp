@@ -940,7 +940,7 @@ __device__ __forceinline__ void march_12(const StageArgs<T>& a, const Eq& eq, in
         if (interior) {
             T res[NV];
             combine<T, NV>(MODE, c, out_v, yc, u0, res);
-            if (CHECK && !eq.obeys(res)) {
+            if (CHECK && !eq.valid(res)) {
                 if constexpr (LOCAL)
                     *ls.status = 2;
                 else
@@ -1360,7 +1360,7 @@ __device__ __forceinline__ void march_3(const StageArgs<T>& a, const Eq& eq, int
         if (interior) {
             T res[NV];
             combine<T, NV>(MODE, c, out_v, zc, u0, res);
-            if (CHECK && !eq.obeys(res)) {
+            if (CHECK && !eq.valid(res)) {
                 atomicOr(&Sp->status, 2);
                 flagged = true;
             }
@@ -1641,7 +1641,7 @@ __global__ __launch_bounds__(256, WPS) void stage_kernel_wide(StageArgs<T> a) {
                         for (int v = 0; v < NV; ++v) u0[v] = in2(v, i0);
                     }
                     combine<T, NV>(MODE, c, out_v, uin, u0, res);
-                    if (CHECK && !eq.obeys(res)) {
+                    if (CHECK && !eq.valid(res)) {
                         atomicOr(&Sp->status, 2);
                         flagged = true;
                     }

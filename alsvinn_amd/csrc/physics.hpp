@@ -189,6 +189,34 @@ struct EulerEq {
         ok = ok && __builtin_isfinite(a.p);
         return ok && (q[0] > T(0)) && (a.p > T(0));
     }
+    // A sufficient condition for obeys(q) without its divisions (FP64):
+    // rho > 2^-100, E > 2^-100, rho*E < 2^200 and rho*E > K*(1 + 2^-20) with
+    // K = 0.5*m.m formed as make() forms it, for 1 + 2^-40 < gamma < 2^100.
+    // Then every q is finite, |m_k| < 2^101 and |u_k| < 2^201; E exceeds
+    // K/rho by a relative margin above 2^-21, more than the roundings of the
+    // products, the quotient K/rho and the subtraction E - K/rho can take
+    // away, so the internal energy, and p = (gamma - 1)*ie, are positive
+    // normal numbers below 2^400. A false result decides nothing: the
+    // caller then evaluates obeys(q) itself.
+    __device__ __forceinline__ bool surely_obeys(const T* q) const {
+        if constexpr (!std::is_same_v<T, double>) {
+            return false;
+        } else {
+            T dot = q[1] * q[1];
+            if (NSD > 1) dot = dot + q[2] * q[2];
+            if (NSD > 2) dot = dot + q[3] * q[3];
+            const T K = T(0.5) * dot;
+            const T rE = q[0] * q[nvar - 1];
+            return gamma > 1.0 + 0x1p-40 && gamma < 0x1p100 && q[0] > 0x1p-100 &&
+                   q[nvar - 1] > 0x1p-100 && rE < 0x1p200 && rE > K * (1.0 + 0x1p-20);
+        }
+    }
+    // The constraint check, with the division-free test first: obeys(q)
+    // runs only on lanes the test does not clear.
+    __device__ __forceinline__ bool valid(const T* q) const {
+        if (surely_obeys(q)) return true;
+        return obeys(q);
+    }
 };
 
 template <class T>
@@ -206,6 +234,7 @@ struct BurgersEq {
         F[0] = a.q[0] * a.q[0] / T(2);
     }
     __device__ __forceinline__ bool obeys(const T* q) const { return __builtin_isfinite(q[0]); }
+    __device__ __forceinline__ bool valid(const T* q) const { return obeys(q); }
     // Burgers::computeWaveSpeed (Burgers.hpp:203-209): |u|
     template <int d>
     __device__ __forceinline__ T waveSpeed(const All& a) const {
