@@ -284,10 +284,12 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
 // wrap or clamp applied as it is loaded), so each of a cell's 8h shell
 // neighbours is an LDS read. Every thread keeps one running sum per shell
 // side (the same terms as structure_cube_kernel, in four independent chains
-// for the FP64 adder); the block total of each h (wave sums, then the four
-// waves in order) goes to partial[((k*nvar + var)*ntiles + tile)*H + h],
-// tiles numbered row-major. The total is a parallel sum either way, within
-// rounding of the reference's sequential loop.
+// for the FP64 adder; on periodic grids half of the shell, doubled, which
+// has the same total over the sample); the block total of each h (wave
+// sums, then the four waves in order) goes to
+// partial[((k*nvar + var)*ntiles + tile)*H + h], tiles numbered row-major.
+// The total is a parallel sum either way, within rounding of the
+// reference's sequential loop.
 constexpr int kTile = 16;
 
 __host__ __device__ constexpr int tile_width(int H) { return kTile + 2 * (H - 1); }
@@ -330,7 +332,30 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
     auto term = [&](double v) { return PK > 0 ? pow_int<PK>(fabs(uc - v)) : pow(fabs(uc - v), p); };
     for (int h = 1; h < H; ++h) {
         double acc = 0;
-        if (valid) {
+        if (valid && g.periodic) {
+            // Periodic grid: cell -> cell + d and cell -> cell - d are both
+            // bijections of the sample's cells and the term is even in the
+            // difference, so the shell's opposite halves give the same
+            // total over the sample: the right column (rows j-h+1 .. j+h-1)
+            // and the top row (columns i-h .. i+h), 4h terms, doubled.
+            // Four running sums (even and odd terms of each side).
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            const double* cr = c + h + (1 - h) * W;
+            const double* rt = c + h * W - h;
+            int y = 0;
+            for (; y + 1 < 2 * h - 1; y += 2) {
+                a0 += term(cr[y * W]);
+                a1 += term(cr[(y + 1) * W]);
+            }
+            a0 += term(cr[y * W]);  // 2h - 1 terms: one left over
+            int x = 0;
+            for (; x + 1 < 2 * h + 1; x += 2) {
+                a2 += term(rt[x]);
+                a3 += term(rt[x + 1]);
+            }
+            a2 += term(rt[x]);  // 2h + 1 terms: one left over
+            acc = 2.0 * ((a0 + a1) + (a2 + a3));
+        } else if (valid) {
             // one running sum per shell side (independent chains for the
             // FP64 adder), added together at the end of the shell
             double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
