@@ -286,13 +286,34 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
 // side (the same terms as structure_cube_kernel, in four independent chains
 // for the FP64 adder; on periodic grids half of the shell, doubled, which
 // has the same total over the sample); the block total of each h (wave
-// sums, then the four waves in order) goes to
+// sums by DPP, then the four waves in order) goes to
 // partial[((k*nvar + var)*ntiles + tile)*H + h], tiles numbered row-major.
 // The total is a parallel sum either way, within rounding of the
 // reference's sequential loop.
 constexpr int kTile = 16;
 
 __host__ __device__ constexpr int tile_width(int H) { return kTile + 2 * (H - 1); }
+
+// The sum of v over the wave's 64 lanes, left in lane 63: DPP row shifts
+// add within each row of 16 lanes, then the row broadcasts add the rows
+// (vector moves only; a shuffle tree would use the LDS pipe, which the
+// tiled kernel's shell reads keep busy). update_dpp with old = 0 and
+// bound_ctrl gives 0 to lanes without a source and to rows masked off.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add_source(double v) {
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_to_last(double v) {
+    v += dpp_add_source<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_add_source<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_add_source<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_add_source<0x118, 0xf>(v);  // row_shr:8: lane 15 of a row holds its sum
+    v += dpp_add_source<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_add_source<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
 
 // |x|^PK for a positive integer PK, as pow_p evaluates it.
 template <int PK>
@@ -330,6 +351,17 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
     const double uc = *c;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     auto term = [&](double v) { return PK > 0 ? pow_int<PK>(fabs(uc - v)) : pow(fabs(uc - v), p); };
+    // a running sum plus one term; for p = 2 the square and the addition are
+    // one FMA (the sum is a parallel sum within rounding of the reference's
+    // sequential loop either way; the FMA drops the product's rounding)
+    auto add = [&](double acc, double v) {
+        if constexpr (PK == 2) {
+            const double d = uc - v;
+            return __builtin_fma(d, d, acc);
+        } else {
+            return acc + term(v);
+        }
+    };
     for (int h = 1; h < H; ++h) {
         double acc = 0;
         if (valid && g.periodic) {
@@ -344,16 +376,16 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
             const double* rt = c + h * W - h;
             int y = 0;
             for (; y + 1 < 2 * h - 1; y += 2) {
-                a0 += term(cr[y * W]);
-                a1 += term(cr[(y + 1) * W]);
+                a0 = add(a0, cr[y * W]);
+                a1 = add(a1, cr[(y + 1) * W]);
             }
-            a0 += term(cr[y * W]);  // 2h - 1 terms: one left over
+            a0 = add(a0, cr[y * W]);  // 2h - 1 terms: one left over
             int x = 0;
             for (; x + 1 < 2 * h + 1; x += 2) {
-                a2 += term(rt[x]);
-                a3 += term(rt[x + 1]);
+                a2 = add(a2, rt[x]);
+                a3 = add(a3, rt[x + 1]);
             }
-            a2 += term(rt[x]);  // 2h + 1 terms: one left over
+            a2 = add(a2, rt[x]);  // 2h + 1 terms: one left over
             acc = 2.0 * ((a0 + a1) + (a2 + a3));
         } else if (valid) {
             // one running sum per shell side (independent chains for the
@@ -363,21 +395,20 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
             const double* cl = c - h + (1 - h) * W;
             const double* cr = c + h + (1 - h) * W;
             for (int y = 0; y < 2 * h - 1; ++y) {
-                a0 += term(cl[y * W]);
-                a1 += term(cr[y * W]);
+                a0 = add(a0, cl[y * W]);
+                a1 = add(a1, cr[y * W]);
             }
             // y sides: rows j-h and j+h, columns i-h .. i+h
             const double* rb = c - h * W - h;
             const double* rt = c + h * W - h;
             for (int x = 0; x < 2 * h + 1; ++x) {
-                a2 += term(rb[x]);
-                a3 += term(rt[x]);
+                a2 = add(a2, rb[x]);
+                a3 = add(a3, rt[x]);
             }
             acc = (a0 + a1) + (a2 + a3);
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) wsum[wave * H + h] = acc;
+        acc = wave_sum_to_last(acc);
+        if (lane == 63) wsum[wave * H + h] = acc;
     }
     __syncthreads();
     for (int h = 1 + (int)threadIdx.x; h < H; h += 256) {
