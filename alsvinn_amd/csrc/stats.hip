@@ -48,7 +48,7 @@ struct alsvinn_hip_statistics {
     double* part = nullptr;  // structure functions: nvar x nblocks x H partial sums
     double* tv = nullptr;    // bv: nvar total variations of the last sample
     int nblocks = 0, tiles_x = 0;
-    bool tiled = false;
+    int tile = 0;  // structure_cube: cells per tile side of the LDS-tiled kernels, 0 = per-cell kernel
     size_t lds = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -417,6 +417,101 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
     }
 }
 
+// Periodic 2D structure_cube with 2 x 2 cells per thread: a block of 256
+// threads owns a 32 x 32 tile of one sample and variable, staged in LDS with
+// an (H-1)-cell halo on every side (the periodic wrap applied at the load).
+// Each cell sums half of its shell, doubled (as the tiled kernel does on
+// periodic grids: the top row and the right column), and the four cells of
+// a thread share the LDS reads: the top rows j+h and j+1+h over columns
+// i-h .. i+1+h and the right columns i+h and i+1+h over rows j-h+1 .. j+h,
+// 8h + 4 reads per h for the four cells instead of 16h. Four running sums
+// (one per cell, independent chains), DPP wave sums, block partials as in
+// the tiled kernel (the tiles numbered row-major, 32 x 32 cells each).
+constexpr int kQuadTile = 32;
+
+__host__ __device__ constexpr int quad_tile_width(int H) { return kQuadTile + 2 * (H - 1); }
+
+template <class T, int PK, class Src>
+__global__ __launch_bounds__(256) void structure_cube_quad_kernel(Src src, StructGeom g, int H,
+                                                                 double p, int nvar, int tiles_x,
+                                                                 double* partial) {
+    extern __shared__ double lds[];
+    const int W = quad_tile_width(H);
+    const int R = H - 1;
+    double* tile = lds;          // W x W staged values
+    double* wsum = lds + W * W;  // [4][H] wave sums
+    const int k = blockIdx.z, var = blockIdx.y;
+    const int x0 = (blockIdx.x % tiles_x) * kQuadTile;
+    const int y0 = (blockIdx.x / tiles_x) * kQuadTile;
+    const T* u = src.at(k, var);
+    for (int idx = threadIdx.x; idx < W * W; idx += 256) {
+        const int lx = idx % W, ly = idx / W;
+        tile[idx] = value_at(u, g, x0 - R + lx, y0 - R + ly, 0);
+    }
+    __syncthreads();
+    // this thread's cells (i + a, j + b), a, b in {0, 1}, tile coordinates
+    const int i = 2 * ((int)threadIdx.x % 16), j = 2 * ((int)threadIdx.x / 16);
+    const double* c = tile + (j + R) * W + (i + R);
+    const double u00 = c[0], u10 = c[1], u01 = c[W], u11 = c[W + 1];
+    const bool in_x0 = x0 + i < g.nx, in_x1 = x0 + i + 1 < g.nx;
+    const bool in_y0 = y0 + j < g.ny, in_y1 = y0 + j + 1 < g.ny;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // a running sum plus the term of cell value uc and shell value v (p = 2:
+    // one FMA, as in the tiled kernel)
+    auto add = [&](double acc, double uc, double v) {
+        if constexpr (PK == 2) {
+            const double d = uc - v;
+            return __builtin_fma(d, d, acc);
+        } else {
+            const double d = fabs(uc - v);
+            return acc + (PK > 0 ? pow_int<PK>(d) : pow(d, p));
+        }
+    };
+    for (int h = 1; h < H; ++h) {
+        double a00 = 0, a10 = 0, a01 = 0, a11 = 0;
+        // top rows: r0 (row j+h) for the cells of row j, r1 (row j+1+h) for
+        // those of row j+1; column offsets 0 .. 2h+1 are columns i-h .. i+1+h
+        const double* r0 = c + h * W - h;
+        const double* r1 = r0 + W;
+        a00 = add(a00, u00, r0[0]);
+        a01 = add(a01, u01, r1[0]);
+        for (int x = 1; x <= 2 * h; ++x) {
+            const double v = r0[x], w = r1[x];
+            a00 = add(a00, u00, v);
+            a10 = add(a10, u10, v);
+            a01 = add(a01, u01, w);
+            a11 = add(a11, u11, w);
+        }
+        a10 = add(a10, u10, r0[2 * h + 1]);
+        a11 = add(a11, u11, r1[2 * h + 1]);
+        // right columns: q0 (column i+h) for the cells of column i, q1
+        // (column i+1+h) for those of column i+1; row offsets 0 .. 2h-1 are
+        // rows j-h+1 .. j+h
+        const double* q0 = c + h + (1 - h) * W;
+        const double* q1 = q0 + 1;
+        a00 = add(a00, u00, q0[0]);
+        a10 = add(a10, u10, q1[0]);
+        for (int y = 1; y <= 2 * h - 2; ++y) {
+            const double v = q0[y * W], w = q1[y * W];
+            a00 = add(a00, u00, v);
+            a01 = add(a01, u01, v);
+            a10 = add(a10, u10, w);
+            a11 = add(a11, u11, w);
+        }
+        a01 = add(a01, u01, q0[(2 * h - 1) * W]);
+        a11 = add(a11, u11, q1[(2 * h - 1) * W]);
+        const double acc = ((in_x0 && in_y0 ? a00 : 0.0) + (in_x1 && in_y0 ? a10 : 0.0)) +
+                           ((in_x0 && in_y1 ? a01 : 0.0) + (in_x1 && in_y1 ? a11 : 0.0));
+        const double tot = wave_sum_to_last(2.0 * acc);
+        if (lane == 63) wsum[wave * H + h] = tot;
+    }
+    __syncthreads();
+    for (int h = 1 + (int)threadIdx.x; h < H; h += 256) {
+        const double tot = ((wsum[h] + wsum[H + h]) + wsum[2 * H + h]) + wsum[3 * H + h];
+        partial[(((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H + h] = tot;
+    }
+}
+
 // PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
 // other p (device pow).
 template <class T, int PK, class Src>
@@ -616,6 +711,44 @@ void launch_tiled(int pk, dim3 grid, size_t lds, hipStream_t st, Src src, Struct
         case 4: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 4, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
         case 5: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 5, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
         default: hipLaunchKernelGGL((structure_cube_tiled_kernel<T, 0, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+    }
+}
+
+// How a 2D structure_cube runs: periodic grids on 32 x 32 tiles with 2 x 2
+// cells per thread (the staged window and the wave sums within the 160 KB
+// of LDS a block may use on gfx950), other grids on 16 x 16 tiles (within
+// 64 KB); 0: the per-cell kernel reading global memory (3D, H < 2, or a
+// halo too wide). *lds: the block's dynamic LDS bytes.
+int cube_tile(int dims, int H, bool periodic, size_t* lds) {
+    if (dims != 2 || H < 2) return 0;
+    const size_t wsum = 4 * size_t(H) * sizeof(double);
+    const size_t quad = size_t(quad_tile_width(H)) * quad_tile_width(H) * sizeof(double) + wsum;
+    const size_t single = size_t(tile_width(H)) * tile_width(H) * sizeof(double) + wsum;
+    if (periodic && quad <= 160 * 1024) {
+        *lds = quad;
+        return kQuadTile;
+    }
+    if (single <= 65536) {
+        *lds = single;
+        return kTile;
+    }
+    return 0;
+}
+
+template <class T, class Src>
+void launch_cube(int tile, int pk, dim3 grid, size_t lds, hipStream_t st, Src src, StructGeom g,
+                 int H, double p, int nvar, int tiles_x, double* part) {
+    if (tile != kQuadTile) {
+        launch_tiled<T>(pk, grid, lds, st, src, g, H, p, nvar, tiles_x, part);
+        return;
+    }
+    switch (pk) {
+        case 1: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 1, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 2: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 2, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 3: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 3, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 4: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 4, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        case 5: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 5, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
+        default: hipLaunchKernelGGL((structure_cube_quad_kernel<T, 0, Src>), grid, dim3(256), lds, st, src, g, H, p, nvar, tiles_x, part); break;
     }
 }
 
@@ -825,28 +958,32 @@ int alsvinn_hip_stats_structure_cube(alsvinn_hip_ensemble* e, const int* samples
                                      int number_of_h, double p, double* host_out) {
     if (!e || number_of_h < 1 || !host_out) return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
     const int H = number_of_h;
-    // 2D with a halo that fits in 64 KB of LDS (H <= 38): the tiled kernel
-    const size_t lds = (size_t(tile_width(H)) * tile_width(H) + 4 * size_t(H)) * sizeof(double);
-    if (e->dim == 2 && H >= 2 && lds <= 65536 && m > 0 && samples) {
+    // 2D with a halo that fits in LDS: the tiled kernels
+    if (m > 0 && samples) {
         if (samples[0] < 0 || samples[0] >= e->nsamples)
             return fail(ALSVINN_HIP_E_INVALID, "sample index out of range");
         const SampleDesc& D = e->desc[samples[0]];
-        const int tiles_x = (D.nx + kTile - 1) / kTile;
-        const int tiles = tiles_x * ((D.ny + kTile - 1) / kTile);
-        return run_structure(
-            e, samples, m, H, host_out,
-            [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
-                const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0) ? int(p) : 0;
-                if (e->esize == 8)
-                    launch_tiled<double>(pk, grid, lds, e->stream,
-                                         EnsembleSrc<double>{(const double*)e->U, bases, g.vstride},
-                                         g, H, p, e->nvar, tiles_x, part);
-                else
-                    launch_tiled<float>(pk, grid, lds, e->stream,
-                                        EnsembleSrc<float>{(const float*)e->U, bases, g.vstride}, g,
-                                        H, p, e->nvar, tiles_x, part);
-            },
-            tiles);
+        size_t lds = 0;
+        const int tile = cube_tile(e->dim, H, D.bc == 1, &lds);
+        if (tile) {
+            const int tiles_x = (D.nx + tile - 1) / tile;
+            const int tiles = tiles_x * ((D.ny + tile - 1) / tile);
+            return run_structure(
+                e, samples, m, H, host_out,
+                [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
+                    const int pk =
+                        (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0) ? int(p) : 0;
+                    if (e->esize == 8)
+                        launch_cube<double>(tile, pk, grid, lds, e->stream,
+                                            EnsembleSrc<double>{(const double*)e->U, bases, g.vstride},
+                                            g, H, p, e->nvar, tiles_x, part);
+                    else
+                        launch_cube<float>(tile, pk, grid, lds, e->stream,
+                                           EnsembleSrc<float>{(const float*)e->U, bases, g.vstride},
+                                           g, H, p, e->nvar, tiles_x, part);
+                },
+                tiles);
+        }
     }
     return run_structure(e, samples, m, number_of_h, host_out,
                          [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
@@ -956,11 +1093,10 @@ int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direc
     } else if (structure) {
         st->elements = (long long)st->nvar * st->H;
         const int H = st->H;
-        st->lds = (size_t(tile_width(H)) * tile_width(H) + 4 * size_t(H)) * sizeof(double);
-        st->tiled = kind == ALSVINN_HIP_STAT_STRUCTURE_CUBE && dims == 2 && H >= 2 && st->lds <= 65536;
-        if (st->tiled) {
-            st->tiles_x = (shape->nx + kTile - 1) / kTile;
-            st->nblocks = st->tiles_x * ((shape->ny + kTile - 1) / kTile);
+        st->tile = kind == ALSVINN_HIP_STAT_STRUCTURE_CUBE ? cube_tile(dims, H, st->periodic, &st->lds) : 0;
+        if (st->tile) {
+            st->tiles_x = (shape->nx + st->tile - 1) / st->tile;
+            st->nblocks = st->tiles_x * ((shape->ny + st->tile - 1) / st->tile);
         } else {
             st->nblocks = int((ncell + 255) / 256);
         }
@@ -1053,9 +1189,9 @@ static void statistics_compute(alsvinn_hip_statistics* st, const alsvinn_hip_vol
             if (st->kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC)
                 launch_basic<T>(pk, grid, st->stream, src, g, st->H, st->p, st->direction, st->nvar,
                                 st->part);
-            else if (st->tiled)
-                launch_tiled<T>(pk, grid, st->lds, st->stream, src, g, st->H, st->p, st->nvar,
-                                st->tiles_x, st->part);
+            else if (st->tile)
+                launch_cube<T>(st->tile, pk, grid, st->lds, st->stream, src, g, st->H, st->p,
+                               st->nvar, st->tiles_x, st->part);
             else
                 hipLaunchKernelGGL((structure_cube_kernel<T, VolumeSrc<T>>), grid, dim3(256), 0,
                                    st->stream, src, g, st->H, st->p, st->nvar, st->part);
