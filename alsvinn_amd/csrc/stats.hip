@@ -282,11 +282,11 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
 // of cells of one sample (blockIdx.z) and variable (blockIdx.y) and first
 // stages the tile with a halo of R = H-1 cells on every side (the boundary's
 // wrap or clamp applied as it is loaded), so each of a cell's 8h shell
-// neighbours is an LDS read. Every thread keeps one running sum per shell
-// side (the same terms as structure_cube_kernel, in four independent chains
-// for the FP64 adder; on periodic grids half of the shell, doubled, which
-// has the same total over the sample); the block total of each h (wave
-// sums by DPP, then the four waves in order) goes to
+// neighbours is an LDS read (periodic grids run the quad kernel below).
+// Every thread keeps one running sum per shell side (the same terms as
+// structure_cube_kernel, in four independent chains for the FP64 adder);
+// the block total of each h (wave sums by DPP, then the four waves in
+// order) goes to
 // partial[((k*nvar + var)*ntiles + tile)*H + h], tiles numbered row-major.
 // The total is a parallel sum either way, within rounding of the
 // reference's sequential loop.
@@ -364,30 +364,7 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
     };
     for (int h = 1; h < H; ++h) {
         double acc = 0;
-        if (valid && g.periodic) {
-            // Periodic grid: cell -> cell + d and cell -> cell - d are both
-            // bijections of the sample's cells and the term is even in the
-            // difference, so the shell's opposite halves give the same
-            // total over the sample: the right column (rows j-h+1 .. j+h-1)
-            // and the top row (columns i-h .. i+h), 4h terms, doubled.
-            // Four running sums (even and odd terms of each side).
-            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-            const double* cr = c + h + (1 - h) * W;
-            const double* rt = c + h * W - h;
-            int y = 0;
-            for (; y + 1 < 2 * h - 1; y += 2) {
-                a0 = add(a0, cr[y * W]);
-                a1 = add(a1, cr[(y + 1) * W]);
-            }
-            a0 = add(a0, cr[y * W]);  // 2h - 1 terms: one left over
-            int x = 0;
-            for (; x + 1 < 2 * h + 1; x += 2) {
-                a2 = add(a2, rt[x]);
-                a3 = add(a3, rt[x + 1]);
-            }
-            a2 = add(a2, rt[x]);  // 2h + 1 terms: one left over
-            acc = 2.0 * ((a0 + a1) + (a2 + a3));
-        } else if (valid) {
+        if (valid) {
             // one running sum per shell side (independent chains for the
             // FP64 adder), added together at the end of the shell
             double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -420,8 +397,11 @@ __global__ __launch_bounds__(256) void structure_cube_tiled_kernel(Src src, Stru
 // Periodic 2D structure_cube with 2 x 2 cells per thread: a block of 256
 // threads owns a 32 x 32 tile of one sample and variable, staged in LDS with
 // an (H-1)-cell halo on every side (the periodic wrap applied at the load).
-// Each cell sums half of its shell, doubled (as the tiled kernel does on
-// periodic grids: the top row and the right column), and the four cells of
+// Each cell sums half of its shell, doubled: cell -> cell + d and
+// cell -> cell - d are both bijections of the sample's cells on a periodic
+// grid and the term is even in the difference, so the shell's opposite
+// halves give the same total over the sample; the top row and the right
+// column are the half kept (4h terms). The four cells of
 // a thread share the LDS reads: the top rows j+h and j+1+h over columns
 // i-h .. i+1+h and the right columns i+h and i+1+h over rows j-h+1 .. j+h,
 // 8h + 4 reads per h for the four cells instead of 16h. Four running sums
