@@ -92,6 +92,35 @@ def test_whole_step_launch_matches_multi_launch(hip, case):
         _same_bits(out1[s], out0[s])
 
 
+@pytest.mark.parametrize("eq", ["EQ_EULER1", "EQ_BURGERS"])
+def test_whole_step_launch_fp32_state(hip, eq):
+    """FP32-state ensembles take the same whole-step launch; its results
+    are the multi-launch step's, bit for bit."""
+    _print_case("small 1d: fp32 " + eq)
+    eq = getattr(hip, eq)
+    if eq == hip.EQ_BURGERS:
+        scheme = hip.make_scheme(eq, hip.FLUX_GODUNOV, hip.RECON_ENO2, hip.INT_RK2, hip.BC_PERIODIC,
+                                 precision=hip.FP32)
+        lo, hi, g = (0, 0, 0), (1, 0, 0), 2
+    else:
+        scheme = hip.make_scheme(eq, hip.FLUX_HLL3, hip.RECON_WENO2, hip.INT_RK3, hip.BC_NEUMANN,
+                                 precision=hip.FP32)
+        lo, hi, g = (-1, 0, 0), (1, 0, 0), 2
+    sizes = [1000, 130, 7]
+    samples = [hip.make_sample((n, 1, 1), lo, hi, 0.05 * (1 + i), saves=(2,))
+               for i, n in enumerate(sizes)]
+    u0s = [_initial(hip, eq, n, g, seed=20 + i) for i, n in enumerate(sizes)]
+    run = lambda e: e.run(steps_per_sync=8)
+    t1, out1, info1, dts1 = _solve(hip, scheme, samples, u0s, 64, run)
+    t0, out0, info0, dts0 = _solve(hip, scheme, samples, u0s, 0, run)
+    assert t1 > 0 and t0 == 0
+    for s in range(len(sizes)):
+        assert info1[s]["steps"] == info0[s]["steps"] > 0
+        assert info1[s]["t"] == info0[s]["t"]
+        _same_bits(dts1[s], dts0[s])
+        _same_bits(out1[s], out0[s])  # FP32 states, widened exactly by the download
+
+
 def test_whole_step_launch_pauses_at_holds_and_step_counts(hip):
     """Samples held at a save stop inside the launch; run_steps advances each
     runnable sample exactly nsteps steps; releasing the hold continues the
