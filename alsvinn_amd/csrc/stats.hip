@@ -207,6 +207,27 @@ __device__ __forceinline__ double value_at(const T* u, const StructGeom& g, int 
     return double(u[((long long)Z * g.ty + Y) * g.tx + X]);
 }
 
+// The sum of v over the wave's 64 lanes, left in lane 63: DPP row shifts
+// add within each row of 16 lanes, then the row broadcasts add the rows
+// (vector moves only; a shuffle tree would use the LDS pipe, which the
+// structure kernels' reads keep busy). update_dpp with old = 0 and
+// bound_ctrl gives 0 to lanes without a source and to rows masked off.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add_source(double v) {
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_to_last(double v) {
+    v += dpp_add_source<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_add_source<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_add_source<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_add_source<0x118, 0xf>(v);  // row_shr:8: lane 15 of a row holds its sum
+    v += dpp_add_source<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_add_source<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Block-wide sum (256 threads); thread 0 gets the result.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -265,9 +286,8 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
                     }
                 }
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-            if (lane == 0) wsum[wave][h - h0] = acc;
+            acc = wave_sum_to_last(acc);
+            if (lane == 63) wsum[wave][h - h0] = acc;
         }
         __syncthreads();
         for (int h = h0 + (int)threadIdx.x; h < h1; h += 256) {
@@ -293,27 +313,6 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
 constexpr int kTile = 16;
 
 __host__ __device__ constexpr int tile_width(int H) { return kTile + 2 * (H - 1); }
-
-// The sum of v over the wave's 64 lanes, left in lane 63: DPP row shifts
-// add within each row of 16 lanes, then the row broadcasts add the rows
-// (vector moves only; a shuffle tree would use the LDS pipe, which the
-// tiled kernel's shell reads keep busy). update_dpp with old = 0 and
-// bound_ctrl gives 0 to lanes without a source and to rows masked off.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_add_source(double v) {
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double wave_sum_to_last(double v) {
-    v += dpp_add_source<0x111, 0xf>(v);  // row_shr:1
-    v += dpp_add_source<0x112, 0xf>(v);  // row_shr:2
-    v += dpp_add_source<0x114, 0xf>(v);  // row_shr:4
-    v += dpp_add_source<0x118, 0xf>(v);  // row_shr:8: lane 15 of a row holds its sum
-    v += dpp_add_source<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_add_source<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    return v;
-}
 
 // |x|^PK for a positive integer PK, as pow_p evaluates it.
 template <int PK>
@@ -521,9 +520,8 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeo
                                            j + h * (direction == 1), l + h * (direction == 2));
                 acc = PK > 0 ? pow_int<PK>(fabs(uc - uh)) : pow(fabs(uc - uh), p);
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-            if (lane == 0) wsum[wave][h - h0] = acc;
+            acc = wave_sum_to_last(acc);
+            if (lane == 63) wsum[wave][h - h0] = acc;
         }
         __syncthreads();
         for (int h = h0 + (int)threadIdx.x; h < h1; h += 256) {
