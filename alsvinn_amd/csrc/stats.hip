@@ -492,7 +492,11 @@ __global__ __launch_bounds__(256) void structure_cube_quad_kernel(Src src, Struc
 }
 
 // PK: the exponent p when it is 1..5 (evaluated as pow_p does), 0 for any
-// other p (device pow).
+// other p (device pow). A block takes kBasicCells x 256 cells, each thread
+// kBasicCells of them 256 apart, so that one wave sum per h covers
+// kBasicCells cells per lane.
+constexpr int kBasicCells = 4;
+
 template <class T, int PK, class Src>
 __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeom g, int H, double p,
                                                              int direction, int nvar,
@@ -502,23 +506,33 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeo
     __shared__ double wsum[4][256];
     const int k = blockIdx.z, var = blockIdx.y;
     const long long ncell = (long long)g.nx * g.ny * g.nz;
-    const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
-    const bool valid = cell < ncell;
-    const int i = valid ? int(cell % g.nx) : 0;
-    const int j = valid ? int((cell / g.nx) % g.ny) : 0;
-    const int l = valid ? int(cell / ((long long)g.nx * g.ny)) : 0;
     const T* u = src.at(k, var);
-    const double uc = value_at(u, g, i, j, l);
+    bool valid[kBasicCells];
+    int ci[kBasicCells], cj[kBasicCells], cl[kBasicCells];
+    double uc[kBasicCells];
+#pragma unroll
+    for (int c = 0; c < kBasicCells; ++c) {
+        const long long cell = ((long long)blockIdx.x * kBasicCells + c) * 256 + threadIdx.x;
+        valid[c] = cell < ncell;
+        ci[c] = valid[c] ? int(cell % g.nx) : 0;
+        cj[c] = valid[c] ? int((cell / g.nx) % g.ny) : 0;
+        cl[c] = valid[c] ? int(cell / ((long long)g.nx * g.ny)) : 0;
+        uc[c] = value_at(u, g, ci[c], cj[c], cl[c]);
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double* out = partial + (((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H;
     for (int h0 = 0; h0 < H; h0 += 256) {
         const int h1 = H < h0 + 256 ? H : h0 + 256;
         for (int h = h0; h < h1; ++h) {
             double acc = 0;
-            if (valid) {
-                const double uh = value_at(u, g, i + h * (direction == 0),
-                                           j + h * (direction == 1), l + h * (direction == 2));
-                acc = PK > 0 ? pow_int<PK>(fabs(uc - uh)) : pow(fabs(uc - uh), p);
+#pragma unroll
+            for (int c = 0; c < kBasicCells; ++c) {
+                if (valid[c]) {
+                    const double uh = value_at(u, g, ci[c] + h * (direction == 0),
+                                               cj[c] + h * (direction == 1),
+                                               cl[c] + h * (direction == 2));
+                    acc += PK > 0 ? pow_int<PK>(fabs(uc[c] - uh)) : pow(fabs(uc[c] - uh), p);
+                }
             }
             acc = wave_sum_to_last(acc);
             if (lane == 63) wsum[wave][h - h0] = acc;
@@ -731,7 +745,8 @@ void launch_cube(int tile, int pk, dim3 grid, size_t lds, hipStream_t st, Src sr
 }
 
 // nblocks_override > 0: the kernel's own block count per (sample, variable)
-// (the tiled kernel's tiles); otherwise one block per 256 cells.
+// (the tiled kernels' tiles, structure_basic's blocks of kBasicCells x 256
+// cells); otherwise one block per 256 cells.
 template <class Launch>
 int run_structure(alsvinn_hip_ensemble* e, const int* samples, int m, int H, double* host_out,
                   Launch launch, int nblocks_override = 0) {
@@ -983,6 +998,12 @@ int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* sample
                                       double* host_out) {
     if (!e || number_of_h < 1 || !host_out || direction < 0 || direction >= e->dim)
         return fail(ALSVINN_HIP_E_INVALID, "bad arguments");
+    int nblocks = 0;  // kBasicCells x 256 cells per block
+    if (m > 0 && samples && samples[0] >= 0 && samples[0] < e->nsamples) {
+        const SampleDesc& D = e->desc[samples[0]];
+        const long long ncell = (long long)D.nx * D.ny * D.nz;
+        nblocks = int((ncell + 256 * kBasicCells - 1) / (256 * kBasicCells));
+    }
     return run_structure(e, samples, m, number_of_h, host_out,
                          [&](dim3 grid, const long long* bases, StructGeom g, double* part) {
                              const int pk = (p == 1.0 || p == 2.0 || p == 3.0 || p == 4.0 || p == 5.0)
@@ -995,7 +1016,8 @@ int alsvinn_hip_stats_structure_basic(alsvinn_hip_ensemble* e, const int* sample
                                  launch_basic<float>(pk, grid, e->stream,
                                                      EnsembleSrc<float>{(const float*)e->U, bases, g.vstride},
                                                      g, number_of_h, p, direction, e->nvar, part);
-                         });
+                         },
+                         nblocks);
 }
 
 int alsvinn_hip_ensemble_totals(alsvinn_hip_ensemble* e, double* host) {
@@ -1075,6 +1097,8 @@ int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direc
         if (st->tile) {
             st->tiles_x = (shape->nx + st->tile - 1) / st->tile;
             st->nblocks = st->tiles_x * ((shape->ny + st->tile - 1) / st->tile);
+        } else if (kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC) {
+            st->nblocks = int((ncell + 256 * kBasicCells - 1) / (256 * kBasicCells));
         } else {
             st->nblocks = int((ncell + 255) / 256);
         }
