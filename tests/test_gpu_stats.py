@@ -2,11 +2,12 @@
 
 meanvar (reference formula), onepointmoment (p = 1, 2) and bv (p = 1, 2) are
 compared bit for bit with the oracle's sequential loops; the Welford variant,
-the structure functions (parallel reductions) and p = 3 (device pow) within a
-stated tolerance.
+the structure functions (parallel reductions) and p = 3 (correctly rounded
+power against libm's pow) within a stated tolerance.
 """
 import hashlib
 import sys
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -153,13 +154,30 @@ def test_structure_cube_2d_tiles_and_wide_halos(hip, bc, prec):
             np.testing.assert_allclose(basic[d], ref_b, rtol=RTOL_REDUCTION, atol=0)
 
 
+def _rounded_power_moment(vols, p):
+    """Per cell, the sum in sample order of the correctly rounded u**p
+    (Fraction arithmetic is exact; its conversion to float rounds once),
+    divided by the number of samples."""
+    flat = np.asarray(vols, dtype=np.float64).reshape(len(vols), -1)
+    out = np.empty(flat.shape[1])
+    for i in range(flat.shape[1]):
+        acc = 0.0
+        for s in range(flat.shape[0]):
+            acc += float(Fraction(float(flat[s, i])) ** p)
+        out[i] = acc / flat.shape[0]
+    return out
+
+
 @pytest.mark.parametrize("dim", [1, 2])
 def test_onepoint_moment_and_bv_match_oracle(hip, dim):
     """OnePointMoment (p = 1, 2 bit for bit: u and u*u summed in sample order;
-    p = 3 within 4 ulp per term, device pow vs libm pow) and BoundedVariation
-    (each sample's sum sequential in the reference's loop order, so bit for
-    bit for p = 1, 2; p = 3 within 1e-14) over 5 samples with random data,
-    ghost layers filled by the boundary, against the oracle."""
+    p = 3 within 4 ulp per term of the oracle, whose libm pow is within 1 ulp
+    of the correctly rounded power; p = 3, 4 bit for bit against the
+    correctly rounded powers, from exact rational arithmetic, summed in sample
+    order) and BoundedVariation (each sample's sum sequential in the
+    reference's loop order, so bit for bit for p = 1, 2; p = 3 within 1e-14)
+    over 5 samples with random data, ghost layers filled by the boundary,
+    against the oracle."""
     _case("onepoint/bv dim=%d" % dim)
     rng = np.random.default_rng(7 + dim)
     M = 5
@@ -178,7 +196,7 @@ def test_onepoint_moment_and_bv_match_oracle(hip, dim):
             e.upload(i, rng.standard_normal(e.volume_shape(i)) + 2.0)
         e.apply_boundary()
         vols = np.stack([e.download(i) for i in range(M)])
-        m = {p: e.onepoint_moment(p) for p in (1, 2, 3)}
+        m = {p: e.onepoint_moment(p) for p in (1, 2, 3, 4)}
         bv = {p: e.bounded_variation(p) for p in (1, 2, 3)}
         order = [3, 0, 4]
         m_sub = e.onepoint_moment(2, samples=order)
@@ -189,6 +207,9 @@ def test_onepoint_moment_and_bv_match_oracle(hip, dim):
     ref3 = O.onepoint_moment(vols, 3)
     scale = O.onepoint_moment(np.abs(vols), 3)
     assert np.all(np.abs(m[3] - ref3) <= 4 * np.spacing(scale))
+    for p in (3, 4):
+        exact = _rounded_power_moment(vols, p).reshape(np.shape(m[p]))
+        assert np.array_equal(m[p].view(np.uint64), exact.view(np.uint64)), "m%d" % p
     assert np.array_equal(m_sub, O.onepoint_moment(vols[order], 2))
     for p in (1, 2):
         ref = O.bounded_variation(vols, ghosts, p)
