@@ -18,7 +18,9 @@
 //                    (HostMemory.cpp:325-362): per variable, sum over samples
 //                    of sum_cells pow(sqrt((u - u_below)^2 + (u - u_left)^2), p)
 //                    over the reference's index ranges, divided by M (1D/2D);
-//                    each sample's sum is sequential, in the reference's order.
+//                    each sample's sum is sequential, in the reference's order
+//                    (one lane per sample and variable adds LDS tiles of
+//                    terms that other waves form).
 // Accumulation is FP64 whatever the storage precision.
 #include <hip/hip_runtime.h>
 
@@ -566,55 +568,116 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeo
     }
 }
 
-// Total variation of one variable of one sample over the reference's ranges:
-// x in [gx+1, tx-gx), y in [gy+(ty>1), ty-gy), z in [0, tz-gz); the cell
-// below is y-1 (y itself in 1D), the cell to the left x-1. One wave per
-// (variable, sample) walks the cells in the reference's loop order (x
-// fastest), 64 at a time: every lane forms one term, then the wave adds the
-// 64 terms in lane order into one FP64 accumulator, so the sum is the
-// reference's sequential sum, bit for bit. The next batch's loads are issued
-// before the current batch is summed. out[k * nvar + var].
+// Total variation of one variable of one sample (a chain) over the
+// reference's ranges: x in [gx+1, tx-gx), y in [gy+(ty>1), ty-gy) (1D and 2D
+// volumes: one z plane); the cell below is y-1 (y itself in 1D), the cell to
+// the left x-1. Each chain's terms are added in the reference's loop order (x
+// fastest) into one FP64 accumulator, so the sum is the reference's
+// sequential sum, bit for bit. A block takes kBvChains chains: producer
+// waves form kBvTile terms of every chain into one of two LDS tiles (term t
+// of chain c at t * kBvPad + c) while the last wave adds the other tile's
+// terms, one lane per chain; each producer keeps the loads of its next tile
+// in flight across the barrier. Cells past the range give the term +0,
+// which leaves the (non-negative) sum as it is. out[chain], chain = k * nvar
+// + var.
+constexpr int kBvChains = 8;
+constexpr int kBvProducers = 4;
+constexpr int kBvTile = 256;
+constexpr int kBvPad = kBvChains + 1;  // tile row stride: producer stores spread over the banks
+constexpr int kBvThreads = 64 * (kBvProducers + 1);
+constexpr int kBvBatches = kBvTile / 64;
+constexpr int kBvPerWave = kBvChains / kBvProducers;
+
 template <class T, class Src>
-__global__ __launch_bounds__(64) void bv_kernel(Src src, StructGeom g, int tz, double p, int nvar,
-                                               double* out) {
-    const int var = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+__global__ __launch_bounds__(kBvThreads) void bv_kernel(Src src, StructGeom g, double p, int nvar,
+                                                       int nchains, double* out) {
+    __shared__ double tile_buf[2][kBvTile * kBvPad];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chain0 = blockIdx.x * kBvChains;
     const int x0 = g.gx + 1, x1 = g.tx - g.gx;
     const int y0 = g.gy + (g.ty > 1 ? 1 : 0), y1 = g.ty - g.gy;
-    const int z1 = tz - g.gz;
     const long long wx = x1 > x0 ? x1 - x0 : 0, wy = y1 > y0 ? y1 - y0 : 0;
-    const long long ncell = wx * wy * (z1 > 0 ? z1 : 0);
-    const T* u = src.at(k, var);
-    auto load = [&](long long c, double& uc, double& ub, double& ul) {
-        uc = ub = ul = 0;
-        if (c < ncell) {
-            const int x = x0 + int(c % wx);
-            const int y = y0 + int((c / wx) % wy);
-            const int z = int(c / (wx * wy));
-            const int yb = g.ty > 1 ? y - 1 : 0;
-            const long long plane = (long long)z * g.tx * g.ty;
-            uc = double(u[plane + (long long)y * g.tx + x]);
-            ub = double(u[plane + (long long)yb * g.tx + x]);
-            ul = double(u[plane + (long long)y * g.tx + (x - 1)]);
+    const long long ncell = wx * wy;
+    const long long ntiles = (ncell + kBvTile - 1) / kBvTile;
+    if (wave == kBvProducers) {
+        double acc = 0;
+        for (long long i = 0; i < ntiles; ++i) {
+            __syncthreads();
+            if (lane < kBvChains) {
+                const double* b = tile_buf[i & 1] + lane;
+#pragma unroll 16
+                for (int t = 0; t < kBvTile; ++t) acc += b[t * kBvPad];
+            }
+        }
+        __syncthreads();
+        if (lane < kBvChains && chain0 + lane < nchains) out[chain0 + lane] = acc;
+        return;
+    }
+    // producers: chains wave + kBvProducers * i; cell c = tile * kBvTile + j * 64 + lane
+    // sits at (x[j], y[j]), advanced by kBvTile cells per tile
+    const T* u[kBvPerWave];
+#pragma unroll
+    for (int i = 0; i < kBvPerWave; ++i) {
+        const int chain = chain0 + wave + kBvProducers * i;
+        u[i] = chain < nchains ? src.at(chain / nvar, chain % nvar) : nullptr;
+    }
+    const int dy_tile = wx > 0 ? int(kBvTile / wx) : 0, dx_tile = wx > 0 ? int(kBvTile % wx) : 0;
+    int x[kBvBatches], y[kBvBatches];
+#pragma unroll
+    for (int j = 0; j < kBvBatches; ++j) {
+        const int c = j * 64 + lane;
+        x[j] = wx > 0 ? x0 + int(c % wx) : x0;
+        y[j] = wx > 0 ? y0 + int(c / wx) : y0;
+    }
+    double uc[kBvPerWave][kBvBatches], ub[kBvPerWave][kBvBatches], ul[kBvPerWave][kBvBatches];
+    auto load_tile = [&](long long tile) {
+#pragma unroll
+        for (int j = 0; j < kBvBatches; ++j) {
+            const bool in = tile * kBvTile + j * 64 + lane < ncell;
+            const long long row = (long long)y[j] * g.tx + x[j];
+            const long long below = g.ty > 1 ? row - g.tx : x[j];
+#pragma unroll
+            for (int i = 0; i < kBvPerWave; ++i) {
+                uc[i][j] = ub[i][j] = ul[i][j] = 0;
+                if (in && u[i]) {
+                    uc[i][j] = double(u[i][row]);
+                    ub[i][j] = double(u[i][below]);
+                    ul[i][j] = double(u[i][row - 1]);
+                }
+            }
+            x[j] += dx_tile;
+            y[j] += dy_tile;
+            if (x[j] >= x1) {
+                x[j] -= wx;
+                ++y[j];
+            }
         }
     };
-    double acc = 0;
-    double uc, ub, ul;
-    load(lane, uc, ub, ul);
-    for (long long base = 0; base < ncell; base += 64) {
-        const double dyv = uc - ub, dxv = uc - ul;
-        // out-of-range lanes give +0, which leaves the (non-negative) sum as is
-        const double t = ref_pow(sqrt(dyv * dyv + dxv * dxv), p);
-        load(base + 64 + lane, uc, ub, ul);
-        const long long bits = __double_as_longlong(t);
-        const int lo = int(bits), hi = int(bits >> 32);
+    auto store_tile = [&](long long tile, double* b) {
 #pragma unroll
-        for (int j = 0; j < 64; ++j) {
-            const unsigned l = unsigned(__builtin_amdgcn_readlane(lo, j));
-            const unsigned h = unsigned(__builtin_amdgcn_readlane(hi, j));
-            acc += __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+        for (int j = 0; j < kBvBatches; ++j) {
+            const bool in = tile * kBvTile + j * 64 + lane < ncell;
+#pragma unroll
+            for (int i = 0; i < kBvPerWave; ++i) {
+                const double dyv = uc[i][j] - ub[i][j], dxv = uc[i][j] - ul[i][j];
+                const double t = in ? ref_pow(sqrt(dyv * dyv + dxv * dxv), p) : 0.0;
+                b[(j * 64 + lane) * kBvPad + wave + kBvProducers * i] = t;
+            }
+        }
+    };
+    if (ntiles > 0) {
+        load_tile(0);
+        store_tile(0, tile_buf[0]);
+        if (ntiles > 1) load_tile(1);
+    }
+    for (long long i = 0; i < ntiles; ++i) {
+        __syncthreads();
+        if (i + 1 < ntiles) {
+            store_tile(i + 1, tile_buf[(i + 1) & 1]);
+            if (i + 2 < ntiles) load_tile(i + 2);
         }
     }
-    if (lane == 0) out[(long long)k * nvar + var] = acc;
+    __syncthreads();
 }
 
 // FP64 sum of every variable over the interior cells of each sample:
@@ -934,6 +997,7 @@ int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samp
     std::vector<long long> bases;
     if (int rc = group_bases(e, samples, m, bases)) return rc;
     const SampleDesc& D = e->desc[samples[0]];
+    if (D.tz != 1) return fail(ALSVINN_HIP_E_INVALID, "bv: not supported for 3d (as in the reference)");
     const size_t np = (size_t)m * e->nvar;
     DeviceArray<long long> d_bases;
     DeviceArray<double> d_tv;
@@ -941,16 +1005,17 @@ int alsvinn_hip_stats_bounded_variation(alsvinn_hip_ensemble* e, const int* samp
     HIP_TRY(d_bases.alloc(m));
     HIP_TRY(d_tv.alloc(np));
     HIP_TRY(hipMemcpy(d_bases.p, bases.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
-    const dim3 grid(e->nvar, m);
+    const int nchains = int(np);
+    const dim3 grid((nchains + kBvChains - 1) / kBvChains);
     const StructGeom gd = geom_of(e, D);
     if (e->esize == 8)
-        hipLaunchKernelGGL((bv_kernel<double, EnsembleSrc<double>>), grid, dim3(64), 0, e->stream,
-                           EnsembleSrc<double>{(const double*)e->U, d_bases.p, gd.vstride}, gd, D.tz,
-                           p, e->nvar, d_tv.p);
+        hipLaunchKernelGGL((bv_kernel<double, EnsembleSrc<double>>), grid, dim3(kBvThreads), 0,
+                           e->stream, EnsembleSrc<double>{(const double*)e->U, d_bases.p, gd.vstride},
+                           gd, p, e->nvar, nchains, d_tv.p);
     else
-        hipLaunchKernelGGL((bv_kernel<float, EnsembleSrc<float>>), grid, dim3(64), 0, e->stream,
-                           EnsembleSrc<float>{(const float*)e->U, d_bases.p, gd.vstride}, gd, D.tz,
-                           p, e->nvar, d_tv.p);
+        hipLaunchKernelGGL((bv_kernel<float, EnsembleSrc<float>>), grid, dim3(kBvThreads), 0,
+                           e->stream, EnsembleSrc<float>{(const float*)e->U, d_bases.p, gd.vstride},
+                           gd, p, e->nvar, nchains, d_tv.p);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(e->stream);
     std::vector<double> tv(np);
@@ -1087,7 +1152,9 @@ int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direc
     if (structure && number_of_h < 1) return fail(ALSVINN_HIP_E_INVALID, "numberOfH must be >= 1");
     if (kind == ALSVINN_HIP_STAT_STRUCTURE_BASIC && (direction < 0 || direction >= dims))
         return fail(ALSVINN_HIP_E_INVALID, "structure_basic direction must be an active axis");
-    if (kind == ALSVINN_HIP_STAT_BV && dims > 2)
+    // HostMemory::getTotalVariation throws for any memory more than one cell
+    // deep in z, ghost layers included
+    if (kind == ALSVINN_HIP_STAT_BV && (dims > 2 || shape->nz + 2 * shape->gz > 1))
         return fail(ALSVINN_HIP_E_INVALID, "bv: not supported for 3d (as in the reference)");
     auto* st = new alsvinn_hip_statistics();
     st->kind = kind;
@@ -1223,8 +1290,9 @@ static void statistics_compute(alsvinn_hip_statistics* st, const alsvinn_hip_vol
             break;
         }
         case ALSVINN_HIP_STAT_BV:
-            hipLaunchKernelGGL((bv_kernel<T, VolumeSrc<T>>), dim3(st->nvar, 1), dim3(64), 0,
-                               st->stream, src, g, st->n[2] + 2 * st->g[2], st->p, st->nvar, st->tv);
+            hipLaunchKernelGGL((bv_kernel<T, VolumeSrc<T>>), dim3((st->nvar + kBvChains - 1) / kBvChains),
+                               dim3(kBvThreads), 0, st->stream, src, g, st->p, st->nvar, st->nvar,
+                               st->tv);
             hipLaunchKernelGGL(add_vec_kernel, dim3(1), dim3(64), 0, st->stream, st->out[0], st->tv,
                                st->nvar);
             break;

@@ -175,7 +175,7 @@ def test_onepoint_moment_and_bv_match_oracle(hip, dim):
     of the correctly rounded power; p = 3, 4 bit for bit against the
     correctly rounded powers, from exact rational arithmetic, summed in sample
     order) and BoundedVariation (each sample's sum sequential in the
-    reference's loop order, so bit for bit for p = 1, 2; p = 3 within 1e-14)
+    reference's loop order, so bit for bit for p = 0, 1, 2; p = 3 within 1e-14)
     over 5 samples with random data, ghost layers filled by the boundary,
     against the oracle."""
     _case("onepoint/bv dim=%d" % dim)
@@ -197,7 +197,7 @@ def test_onepoint_moment_and_bv_match_oracle(hip, dim):
         e.apply_boundary()
         vols = np.stack([e.download(i) for i in range(M)])
         m = {p: e.onepoint_moment(p) for p in (1, 2, 3, 4)}
-        bv = {p: e.bounded_variation(p) for p in (1, 2, 3)}
+        bv = {p: e.bounded_variation(p) for p in (0, 1, 2, 3)}
         order = [3, 0, 4]
         m_sub = e.onepoint_moment(2, samples=order)
         bv_sub = e.bounded_variation(1, samples=order)
@@ -211,9 +211,14 @@ def test_onepoint_moment_and_bv_match_oracle(hip, dim):
         exact = _rounded_power_moment(vols, p).reshape(np.shape(m[p]))
         assert np.array_equal(m[p].view(np.uint64), exact.view(np.uint64)), "m%d" % p
     assert np.array_equal(m_sub, O.onepoint_moment(vols[order], 2))
-    for p in (1, 2):
+    for p in (0, 1, 2):
         ref = O.bounded_variation(vols, ghosts, p)
         assert np.array_equal(bv[p].view(np.uint64), ref.view(np.uint64)), "bv%d" % p
+    # p = 0: every term is 1, so the total counts the cells of the range and
+    # nothing past it
+    wx = vols.shape[-1] - 2 * ghosts[0] - 1
+    wy = vols.shape[-2] - 2 * ghosts[1] - 1 if dim == 2 else 1
+    assert np.all(bv[0] == wx * wy)
     np.testing.assert_allclose(bv[3], O.bounded_variation(vols, ghosts, 3), rtol=1e-14)
     assert np.array_equal(bv_sub, O.bounded_variation(vols[order], ghosts, 1))
 
