@@ -155,6 +155,26 @@ def test_empty_ensemble_and_pool_rejected_before_any_gpu_call():
     assert b"bad pool arguments" in L.alsvinn_hip_last_error()
 
 
+def test_per_sample_bv_rejects_memory_deeper_than_one_cell_before_any_gpu_call():
+    """HostMemory::getTotalVariation (HostMemory.cpp:333-335) throws for any
+    memory with more than one cell in z, ghost layers included: a 2D shape
+    with z ghost layers is refused like a 3D one, and a plain 2D shape
+    passes the argument checks (without a GPU it then stops at the first HIP
+    call, which reports the missing device)."""
+    import alsvinn_amd as hip
+    from alsvinn_amd import _lib, operators
+
+    for n, g in [((16, 16, 1), (2, 2, 1)), ((8, 8, 8), (1, 1, 1))]:
+        shape = operators.make_volume(4, n, g, [0, 0, 0, 0])
+        with pytest.raises(hip.HipError, match="not supported for 3d"):
+            operators.Statistics(_lib.STAT_BV, shape, p=1.0)
+    shape = operators.make_volume(4, (16, 16, 1), (2, 2, 0), [0, 0, 0, 0])
+    try:
+        operators.Statistics(_lib.STAT_BV, shape, p=1.0).close()
+    except hip.HipError as err:
+        assert "not supported for 3d" not in str(err)
+
+
 class _FakePool:
     """A pool without a GPU for the serving loop: every solve finishes after
     int(end_time * 1000) steps; snapshots are taken at tick time."""
