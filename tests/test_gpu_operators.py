@@ -261,8 +261,10 @@ def test_statistics_per_sample_equal_ensemble_and_oracle(hip, dev, dim):
     """Statistics fed one sample at a time from per-variable device memories
     (compute per sample, then combine and finalize) equal the ensemble entry
     points over the same samples bit for bit, and the oracle: meanvar and
-    onepointmoment (p = 1, 2) bit for bit, the structure functions within
-    1e-12 relative (parallel sums), bv (p = 1) bit for bit. The wide
+    onepointmoment (p = 1, 2) bit for bit, p = 3 within 4 ulp per term (the
+    correctly rounded power against libm's pow), the structure functions
+    within 1e-12 relative (parallel sums), bv (p = 1) bit for bit, p = 3
+    within 1e-14. The wide
     structure_cube (the untiled kernel: H = 40 in 1D/2D) uses H = 10 in 3D,
     where 40 shells make ~4e8 sequential additions in the oracle, whose own
     rounding then reaches ~1e-11."""
@@ -287,12 +289,14 @@ def test_statistics_per_sample_equal_ensemble_and_oracle(hip, dev, dim):
     hw = 40 if dim < 3 else 10
     kinds = [("meanvar", hip._lib.STAT_MEANVAR, {}),
              ("m2", hip._lib.STAT_ONEPOINTMOMENT, {"p": 2.0}),
+             ("m3", hip._lib.STAT_ONEPOINTMOMENT, {"p": 3.0}),
              ("cube8", hip._lib.STAT_STRUCTURE_CUBE, {"p": 2.0, "number_of_h": 8}),
              ("cubew", hip._lib.STAT_STRUCTURE_CUBE, {"p": 1.0, "number_of_h": hw}),
              ("basic", hip._lib.STAT_STRUCTURE_BASIC, {"p": 2.0, "number_of_h": 9,
                                                        "direction": dim - 1})]
     if dim < 3:
         kinds.append(("bv1", hip._lib.STAT_BV, {"p": 1.0}))
+        kinds.append(("bv3", hip._lib.STAT_BV, {"p": 3.0}))
     got = {}
     for name, kind, kw in kinds:
         with ops.Statistics(kind, shape, boundary=bc, **kw) as st:
@@ -306,11 +310,12 @@ def test_statistics_per_sample_equal_ensemble_and_oracle(hip, dev, dim):
     with hip.Ensemble(scheme, [hip.make_sample(n, (0, 0, 0), (1, 1, 1), 1.0) for _ in range(M)]) as e:
         for i, v in enumerate(vols):
             e.upload(i, v)
-        ens = {"meanvar": e.meanvar(), "m2": e.onepoint_moment(2),
+        ens = {"meanvar": e.meanvar(), "m2": e.onepoint_moment(2), "m3": e.onepoint_moment(3),
                "cube8": e.structure_cube(8, 2.0), "cubew": e.structure_cube(hw, 1.0),
                "basic": e.structure_basic(9, dim - 1, 2.0)}
         if dim < 3:
             ens["bv1"] = e.bounded_variation(1)
+            ens["bv3"] = e.bounded_variation(3)
     mean, var = ens["meanvar"]
     _bits_equal(mean.ravel(), got["meanvar"][0], "mean vs ensemble")
     _bits_equal(var.ravel(), got["meanvar"][1], "variance vs ensemble")
@@ -319,6 +324,10 @@ def test_statistics_per_sample_equal_ensemble_and_oracle(hip, dev, dim):
     _bits_equal(rv.ravel(), got["meanvar"][1], "variance vs oracle")
     _bits_equal(ens["m2"].ravel(), got["m2"][0], "m2 vs ensemble")
     _bits_equal(O.onepoint_moment(np.stack(vols), 2).ravel(), got["m2"][0], "m2 vs oracle")
+    _bits_equal(ens["m3"].ravel(), got["m3"][0], "m3 vs ensemble")
+    scale = O.onepoint_moment(np.abs(np.stack(vols)), 3).ravel()
+    assert np.all(np.abs(got["m3"][0] - O.onepoint_moment(np.stack(vols), 3).ravel())
+                  <= 4 * np.spacing(scale)), "m3 vs oracle"
     periodic = bc == O.BC_PERIODIC
     for key, H, p in (("cube8", 8, 2.0), ("cubew", hw, 1.0)):
         _bits_equal(ens[key].ravel(), got[key][0], key + " vs ensemble")
@@ -330,6 +339,9 @@ def test_statistics_per_sample_equal_ensemble_and_oracle(hip, dev, dim):
     if dim < 3:
         _bits_equal(ens["bv1"].ravel(), got["bv1"][0], "bv vs ensemble")
         _bits_equal(O.bounded_variation(np.stack(vols), gv, 1), got["bv1"][0], "bv vs oracle")
+        _bits_equal(ens["bv3"].ravel(), got["bv3"][0], "bv3 vs ensemble")
+        np.testing.assert_allclose(got["bv3"][0], O.bounded_variation(np.stack(vols), gv, 3),
+                                   rtol=1e-14, atol=0)
 
 
 def test_operator_argument_errors(hip, dev):
