@@ -605,7 +605,7 @@ __global__ __launch_bounds__(kBvThreads) void bv_kernel(Src src, StructGeom g, d
             __syncthreads();
             if (lane < kBvChains) {
                 const double* b = tile_buf[i & 1] + lane;
-#pragma unroll 16
+#pragma unroll 64
                 for (int t = 0; t < kBvTile; ++t) acc += b[t * kBvPad];
             }
         }
