@@ -529,17 +529,27 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeo
     const int k = blockIdx.z, var = blockIdx.y;
     const long long ncell = (long long)g.nx * g.ny * g.nz;
     const T* u = src.at(k, var);
+    // the neighbour at h along `direction` sits at pos = (c_d + h) mod N
+    // (periodic) or min(c_d + h, N - 1) (Neumann), the wrap_index of c_d + h:
+    // each cell keeps pos and its offset and steps them once per h
+    const int N = direction == 0 ? g.nx : (direction == 1 ? g.ny : g.nz);
+    const long long stride = direction == 0 ? 1 : (direction == 1 ? (long long)g.tx
+                                                                  : (long long)g.tx * g.ty);
+    const long long wrap_back = (long long)(N - 1) * stride;
     bool valid[kBasicCells];
-    int ci[kBasicCells], cj[kBasicCells], cl[kBasicCells];
+    int pos[kBasicCells];
+    long long off[kBasicCells];
     double uc[kBasicCells];
 #pragma unroll
     for (int c = 0; c < kBasicCells; ++c) {
         const long long cell = ((long long)blockIdx.x * kBasicCells + c) * 256 + threadIdx.x;
         valid[c] = cell < ncell;
-        ci[c] = valid[c] ? int(cell % g.nx) : 0;
-        cj[c] = valid[c] ? int((cell / g.nx) % g.ny) : 0;
-        cl[c] = valid[c] ? int(cell / ((long long)g.nx * g.ny)) : 0;
-        uc[c] = value_at(u, g, ci[c], cj[c], cl[c]);
+        const int ci = valid[c] ? int(cell % g.nx) : 0;
+        const int cj = valid[c] ? int((cell / g.nx) % g.ny) : 0;
+        const int cl = valid[c] ? int(cell / ((long long)g.nx * g.ny)) : 0;
+        pos[c] = direction == 0 ? ci : (direction == 1 ? cj : cl);
+        off[c] = ((long long)(cl + g.gz) * g.ty + (cj + g.gy)) * g.tx + (ci + g.gx);
+        uc[c] = double(u[off[c]]);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double* out = partial + (((long long)k * nvar + var) * gridDim.x + blockIdx.x) * H;
@@ -550,10 +560,15 @@ __global__ __launch_bounds__(256) void structure_basic_kernel(Src src, StructGeo
 #pragma unroll
             for (int c = 0; c < kBasicCells; ++c) {
                 if (valid[c]) {
-                    const double uh = value_at(u, g, ci[c] + h * (direction == 0),
-                                               cj[c] + h * (direction == 1),
-                                               cl[c] + h * (direction == 2));
+                    const double uh = double(u[off[c]]);
                     acc += PK > 0 ? pow_int<PK>(fabs(uc[c] - uh)) : pow(fabs(uc[c] - uh), p);
+                }
+                if (pos[c] + 1 < N) {
+                    ++pos[c];
+                    off[c] += stride;
+                } else if (g.periodic) {
+                    pos[c] = 0;
+                    off[c] -= wrap_back;
                 }
             }
             acc = wave_sum_to_last(acc);
