@@ -299,12 +299,28 @@ __global__ __launch_bounds__(256) void structure_cube_kernel(Src src, StructGeom
                             xs += 1;
                             xe -= 1;
                         }
-                        for (int z = zs; z < ze; z++)
-                            for (int y = ys; y < ye; y++)
-                                for (int x = xs; x < xe; x++) {
-                                    const double uh = value_at(u, g, x, y, z);
-                                    acc += pow_p(fabs(uc - uh), p);
+                        // value_at(u, g, x, y, z) along each row: the row's
+                        // base once, then x wrapped by stepping (periodic) or
+                        // clamped (Neumann)
+                        for (int z = zs; z < ze; z++) {
+                            const int Z = wrap_index(z, g.nz, g.periodic) + g.gz;
+                            for (int y = ys; y < ye; y++) {
+                                const int Y = wrap_index(y, g.ny, g.periodic) + g.gy;
+                                const T* row = u + ((long long)Z * g.ty + Y) * g.tx + g.gx;
+                                if (g.periodic) {
+                                    int X = wrap_index(xs, g.nx, 1);
+                                    for (int x = xs; x < xe; x++) {
+                                        acc += pow_p(fabs(uc - double(row[X])), p);
+                                        X = X + 1 == g.nx ? 0 : X + 1;
+                                    }
+                                } else {
+                                    for (int x = xs; x < xe; x++) {
+                                        const int X = x < 0 ? 0 : (x > g.nx - 1 ? g.nx - 1 : x);
+                                        acc += pow_p(fabs(uc - double(row[X])), p);
+                                    }
                                 }
+                            }
+                        }
                     }
                 }
             }
