@@ -435,7 +435,9 @@ typedef struct alsvinn_hip_statistics alsvinn_hip_statistics;
  * gives nvar, precision, extents and ghosts of the samples (its pointers are
  * not used). Outputs: meanvar 2 volumes (mean, variance) of nvar x padded
  * cells; onepointmoment 1 such volume; structure functions nvar x number_of_h
- * values (out[var*H + h]); bv nvar values. */
+ * values (out[var*H + h]); bv nvar values. bv refuses a shape more than
+ * one cell deep in z, ghost layers included, as HostMemory::
+ * getTotalVariation throws for such memory (HostMemory.cpp:333-335). */
 int alsvinn_hip_statistics_create(int kind, double p, int number_of_h, int direction, int boundary,
                                   const alsvinn_hip_volume* shape, void* stream,
                                   alsvinn_hip_statistics** out);
